@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""PD steps/s on the 64^3-hex cube (1,572,864 tets, 274,625 vertices), one MI355X.
+
+Workload (one step = one PdSolver::Update of the whole body): 64x64x64 hex
+grid, 0.1 m spacing, 6 Kuhn tets per hex, E = 1e5 Pa, nu = 0.3, rho = 1000,
+top face pinned, gravity -9.8 m/s^2, seeded +-1 mm jitter, dt = 1/60 s;
+20 local/global iterations per step, each a Jacobi-PCG global solve
+(||D^-1 r|| reduced 1e-6, floor 1e-8 ||D^-1 b||) followed by the Chebyshev
+recurrence (rho = 0.9992); FP32 arithmetic.  Inputs are resident in HBM
+before the timed region.
+
+Prints one JSON line: steps/s, ms/step, the roofline of the dominant kernel
+(timed live with HIP events on the solver's stream), the error of the FP32
+run against an FP64 run of the same steps, and the CPU baseline (the oracle's
+C++ restatement of the same step with OpenMP) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "soft-body-simulation-cuda_amd"))
+
+from sbs_amd import mesh  # noqa: E402
+from sbs_amd.pd import PdSolver, SolverData, SolverParams, SolverType  # noqa: E402
+
+METRIC = "PD steps/sec at 64^3-hex cube vs plain FP32 at stated error; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(fp64: bool = False) -> SolverParams:
+    return SolverParams(numIterations=20, dt=1.0 / 60.0, gravity=9.8, tol=1e-6, rho=0.9992, pcg_tol=1e-6,
+                        pcg_floor=0.0 if fp64 else 1e-8, pcg_max_iter=200, pcg_chebyshev=True,
+                        pcg_adaptive=True)
+
+
+def local_step_bytes(nt: int, nv: int) -> int:
+    # per tet: 4 u32 indices + 9 f32 Dm^-1 + 1 f32 w; per vertex: read q, read
+    # sn_old, read m/h^2, write b (the unit of the reference's computeLocal +
+    # addM_h2Sn pass)
+    return 56 * nt + 40 * nv
+
+
+def spmv_bytes(nnz_off: int, nv: int) -> int:
+    # q = A p with p.q: off-diagonal values + u32 columns, diagonal, read p, write q
+    return 8 * nnz_off + 4 * nv + 12 * nv + 12 * nv
+
+
+def run_gpu(scene, steps: int, warmup: int, precision: int, time_kernels: bool):
+    data = SolverData(scene, precision)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(SolverType.PCG)
+    p = workload_params(precision == 64)
+    for _ in range(warmup):
+        pd.Update(data, p)
+    pd.synchronize()
+    stats_w = pd.stats()
+    if time_kernels:
+        pd.set_kernel_timing(["local_step", "pcg_spmv", "pcg_update", "pcg_direction", "pcg_init", "chebyshev"])
+    pd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pd.Update(data, p)
+    pd.synchronize()
+    t1 = time.perf_counter()
+    kt = {}
+    if time_kernels:
+        for k in ("local_step", "pcg_spmv", "pcg_update", "pcg_direction", "pcg_init", "chebyshev"):
+            kt[k] = pd.kernel_time(k)
+    st = pd.stats()
+    X = data.get("X")
+    out = dict(seconds=t1 - t0, X=X, stats=st, stats_warm=stats_w, kernel_times=kt,
+               nnz_off=int(st.nnz_offdiag))
+    pd.close()
+    data.free()
+    return out
+
+
+def cpu_baseline(scene, budget_s: float, max_steps: int):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    p = O.make_params(solver=O.SOLVER_PCG, num_iterations=20, dt=1.0 / 60.0, gravity=9.8, tol=1e-6, rho=0.9992,
+                      pcg_tol=1e-6, pcg_floor=1e-8, pcg_max_iter=200, pcg_chebyshev=1)
+    o = O.PdOracle(scene, p, fp64=False, bench=True)
+    o.step(1)  # warm-up step
+    n, t0 = 0, time.perf_counter()
+    while n < max_steps:
+        o.step(1)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    threads = O.num_threads(bench=True)
+    o.close()
+    return dict(value=n / el, unit="steps/s", cores=threads, kind="port",
+                sample=f"{n} step(s) of the same 64^3 workload after 1 warm-up step, FP32, "
+                       f"oracle C++ restatement with OpenMP ({threads} threads), {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = the measured workload)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference run")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU (the PD step of one body on one MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
+    print(f"# scene: {scene.nv} vertices, {scene.nt} tets", file=sys.stderr, flush=True)
+    g = run_gpu(scene, args.steps, args.warmup, 32, time_kernels=True)
+    print(f"# gpu fp32: {args.steps} steps in {g['seconds']:.3f} s", file=sys.stderr, flush=True)
+    sps = args.steps / g["seconds"]
+
+    err = None
+    if not args.no_fp64:
+        r = run_gpu(scene, args.steps, args.warmup, 64, time_kernels=False)
+        print(f"# gpu fp64 reference run: {r['seconds']:.3f} s", file=sys.stderr, flush=True)
+        disp = np.abs(r["X"] - scene.X).max()
+        err = dict(rel_l2=float(np.linalg.norm(g["X"] - r["X"]) / np.linalg.norm(r["X"])),
+                   max_abs_m=float(np.abs(g["X"] - r["X"]).max()),
+                   max_abs_over_max_displacement=float(np.abs(g["X"] - r["X"]).max() / disp),
+                   reference="FP64 run of the same steps on the GPU (FP64 path parity-tested vs the FP64 oracle)")
+
+    kt = g["kernel_times"]
+    nl, ml = kt["local_step"]
+    ns, ms = kt["pcg_spmv"]
+    t_local = ml / max(nl, 1) * 1e-3
+    t_spmv = ms / max(ns, 1) * 1e-3
+    b_local = local_step_bytes(scene.nt, scene.nv)
+    b_spmv = spmv_bytes(g["nnz_off"], scene.nv)
+    total = {k: v[1] for k, v in kt.items()}
+    dominant = max(total, key=total.get)
+    if dominant == "pcg_spmv":
+        ach, dom, bytes_per = b_spmv / t_spmv / 1e9, "pcg_spmv", b_spmv
+    else:
+        ach, dom, bytes_per = b_local / t_local / 1e9, "local_step", b_local
+    roof = dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, algorithmic_bytes_per_launch=bytes_per,
+                kernels={k: dict(launches=v[0], total_ms=round(v[1], 3),
+                                 avg_us=round(v[1] / max(v[0], 1) * 1e3, 2)) for k, v in kt.items()},
+                local_step=dict(avg_us=round(t_local * 1e6, 2), gbs=round(b_local / t_local / 1e9, 1)),
+                pcg_spmv=dict(avg_us=round(t_spmv * 1e6, 2), gbs=round(b_spmv / t_spmv / 1e9, 1)))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, args.cpu_budget, 10)
+        print(f"# cpu baseline: {cpu['value']:.4f} steps/s", file=sys.stderr, flush=True)
+
+    st = g["stats"]
+    line = {
+        "metric": METRIC, "value": round(sps, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * g["seconds"] / args.steps, 4),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.n}^3-hex cube PD step (Jacobi-PCG + Chebyshev, 20 it/step)",
+                   "verts": scene.nv, "tets": scene.nt, "nnz_scalar": g["nnz_off"] + scene.nv,
+                   "iterations_per_step": 20, "dt": 1.0 / 60.0, "pcg_tol": 1e-6, "pcg_floor": 1e-8},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "error_vs_fp64": err,
+        "pcg": {"iterations_last_step": st.pcg_iterations_last_step, "launch_cap": st.pcg_cap,
+                "cap_hits": int(st.pcg_cap_hits)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

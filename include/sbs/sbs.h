@@ -1,0 +1,164 @@
+/* sbs.h -- C ABI of the MI355X (gfx950) projective-dynamics library.
+ *
+ * The boundary is the reference's PD solver and linear-solver interface,
+ * flattened to plain pointers and sizes so any host language can bind it:
+ *
+ *   PdSolver(int threadsPerBlock, const SolverData<float>&)
+ *        src/simulation/solver/projective/pdSolver.h:18      -> sbs_pd_create
+ *   PdSolver::SetGlobalSolver(SolverType)    pdSolver.h:20   -> sbs_pd_set_global_solver
+ *   PdSolver::Update(SolverData&, const SolverParams&)
+ *        pdSolver.h:21, pdSolver.cu:210-232                  -> sbs_pd_update
+ *   PdSolver::Reset()                         pdSolver.h:22  -> sbs_pd_reset
+ *   Solver::SetPerf(bool)        src/simulation/solver/solver.h:18 -> sbs_pd_set_perf
+ *   Solver::GetPerformanceData() solver.h:19                 -> sbs_pd_perf_count / sbs_pd_perf_entry
+ *   LinearSolver<T>::Solve(N, d_b, d_x, ...) with the prefactored PD matrix
+ *        src/simulation/solver/linear/linear.h:62, pdSolver.cu:188 -> sbs_pd_solve
+ *
+ * Data: SbsSolverData mirrors SolverData<Scalar> (src/def.h:22-56) -- raw
+ * device pointers, vectors as interleaved xyz (glm::vec3 AoS), tets as
+ * u32[4*N_t].  SbsSolverParams mirrors SolverParams<Scalar> (src/def.h:83-100)
+ * plus the settings of the PCG global step.  Scalars are float or double as
+ * chosen at creation (the reference's PD path is float; double gives the FP64
+ * form of the same step).
+ *
+ * Errors: every entry point returns 0 on success or a negative SBS_E_* code
+ * and records a message readable with sbs_last_error() (thread-local).  The
+ * reference prints and continues (linear.h:16-49) or exits; this library never
+ * exits the process.
+ *
+ * Threads and streams: a handle is used by one host thread at a time.  Work
+ * is queued on the handle's HIP stream (default: the null stream; see
+ * sbs_pd_set_stream); sbs_pd_update returns without waiting for the GPU.
+ */
+#ifndef SBS_SBS_H
+#define SBS_SBS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBS_OK 0
+#define SBS_E_INVALID (-1)
+#define SBS_E_HIP (-2)
+#define SBS_E_NOT_READY (-3)
+#define SBS_E_FACTOR (-4)
+#define SBS_E_OOM (-5)
+#define SBS_E_INTERNAL (-6)
+
+/* PdSolver::SolverType (pdSolver.h:14-17) plus the Jacobi-PCG global step. */
+#define SBS_SOLVER_JACOBI 0
+#define SBS_SOLVER_CUSOLVER_CHOLESKY 1
+#define SBS_SOLVER_EIGEN_CHOLESKY 2
+#define SBS_SOLVER_PCG 3
+
+/* Mirrors SolverData<Scalar> (src/def.h:22-56); pointers are device memory
+ * holding float (precision 32) or double (precision 64). */
+typedef struct SbsSolverData {
+    int num_verts;
+    int num_tets;
+    int num_dbc;
+    void* X;              /* 3*num_verts: positions x_n, written by update   */
+    void* V;              /* 3*num_verts: velocities, written by update      */
+    void* X0;             /* 3*num_verts: rest positions                      */
+    void* XTilde;         /* 3*num_verts: end-of-step positions               */
+    void* DBCX;           /* 3*num_verts: pin targets (set from X0 at prepare) */
+    const uint32_t* Tet;  /* 4*num_tets, 16-byte aligned                      */
+    const void* mass;     /* num_verts                                        */
+    const void* mu;       /* num_tets                                         */
+    const void* DBC;      /* num_verts: 1 for pinned vertices, else 0         */
+    const void* moreDBC;  /* num_verts or NULL: extra soft pins (0 = none)    */
+    void* V0;             /* num_tets or NULL: rest volumes, written at create */
+} SbsSolverData;
+
+/* Mirrors SolverParams<Scalar> (src/def.h:83-100) for the PD step. */
+typedef struct SbsSolverParams {
+    int num_iterations;   /* local/global iterations per step                 */
+    double dt;
+    double gravity;       /* magnitude along -y                               */
+    double tol;           /* early exit of the direct/PCG loop on the iterate */
+    double rho;           /* Chebyshev spectral-radius estimate               */
+    int handle_collision; /* reserved: mesh contact after the solve           */
+    double pcg_tol;       /* PCG: ||D^-1 r_k|| <= pcg_tol * ||D^-1 r_0||       */
+    double pcg_floor;     /*  ... or <= pcg_floor * ||D^-1 b||                 */
+    int pcg_max_iter;     /* PCG iteration limit per solve                     */
+    int pcg_chebyshev;    /* apply the Chebyshev recurrence after each solve   */
+    int pcg_adaptive;     /* 1: learned launch cap (no host sync); 0: exact    */
+} SbsSolverParams;
+
+typedef struct SbsPd SbsPd;
+
+/* Step statistics of the last completed update (see sbs_pd_stats). */
+typedef struct SbsPdStats {
+    int pcg_iterations_last_step; /* PCG iterations summed over the last step */
+    int pcg_cap;                  /* current adaptive launch cap              */
+    long long pcg_cap_hits;       /* solves that ran out of launched iterations */
+    long long nnz_offdiag;        /* off-diagonal nonzeros of the scalar matrix */
+} SbsPdStats;
+
+const char* sbs_last_error(void);
+int sbs_version(void);
+
+/* PdSolver(threadsPerBlock, solverData): computes Dm^-1 and V0 (FEMSolver ctor). */
+int sbs_pd_create(int precision, int threads_per_block, const SbsSolverData* data, SbsPd** out);
+int sbs_pd_destroy(SbsPd* pd);
+int sbs_pd_set_global_solver(SbsPd* pd, int solver_type);
+/* One PdSolver::Update: prepares on the first call, then one time step. */
+int sbs_pd_update(SbsPd* pd, SbsSolverData* data, const SbsSolverParams* params);
+int sbs_pd_reset(SbsPd* pd);
+int sbs_pd_set_perf(SbsPd* pd, int on);
+int sbs_pd_perf_count(SbsPd* pd, int* count);
+int sbs_pd_perf_entry(SbsPd* pd, int index, const char** name, double* ms);
+int sbs_pd_set_stream(SbsPd* pd, void* hip_stream);
+/* Waits for queued work of the handle and returns its statistics. */
+int sbs_pd_stats(SbsPd* pd, SbsPdStats* out);
+
+/* Kernel timing with HIP events on the handle's stream: set a bit mask of
+ * SBS_KERNEL_* ids, run updates, then read launches and summed milliseconds
+ * (reading waits for the stream).  Setting the mask clears the sums. */
+#define SBS_KERNEL_LOCAL_STEP 0
+#define SBS_KERNEL_PCG_INIT 1
+#define SBS_KERNEL_PCG_SPMV 2
+#define SBS_KERNEL_PCG_UPDATE 3
+#define SBS_KERNEL_PCG_DIRECTION 4
+#define SBS_KERNEL_JACOBI_CHEBYSHEV 5
+#define SBS_KERNEL_CHEBYSHEV 6
+#define SBS_KERNEL_DIRECT_SOLVE 7
+#define SBS_KERNEL_COUNT 8
+int sbs_pd_set_kernel_timing(SbsPd* pd, unsigned mask);
+int sbs_pd_kernel_time(SbsPd* pd, int kernel_id, const char** name, int* launches, double* ms);
+
+/* LinearSolver::Solve(3N, d_b, d_x[, d_guess]) against the prepared PD matrix
+ * (direct when a Cholesky solver type is set, else PCG); synchronous. */
+int sbs_pd_solve(SbsPd* pd, const void* d_b, void* d_x, const void* d_guess, const SbsSolverParams* params,
+                 int* iterations);
+
+/* computeLocal + computeDBCLocal alone on an iterate q (device arrays of
+ * 3*num_verts): b = c*sn_old + sum of per-tet projections, pins overwritten. */
+int sbs_pd_local_step(SbsPd* pd, const void* d_q, const void* d_sn_old, int residual_form, double dt, void* d_b);
+
+/* Prepared system copied to host buffers (any pointer may be NULL):
+ * c, diag(A_s), diag(L_s) (num_verts); V0, w=|V0|mu (num_tets); Dm^-1
+ * (9*num_tets, row-major per tet); CSR of A_s (rowptr num_verts+1, col/val nnz).
+ * sbs_pd_system_nnz gives nnz (diagonal included). */
+int sbs_pd_system_nnz(SbsPd* pd, long long* nnz);
+int sbs_pd_export_system(SbsPd* pd, void* c, void* diagA, void* diagL, void* V0, void* w, void* dminv, int* rowptr,
+                         int* col, void* val);
+
+/* Device memory and copies for callers without their own allocator. */
+int sbs_malloc(void** ptr, size_t bytes);
+int sbs_free(void* ptr);
+int sbs_memcpy_htod(void* dst, const void* src, size_t bytes);
+int sbs_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int sbs_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int sbs_memset(void* dst, int value, size_t bytes);
+int sbs_device_synchronize(void);
+int sbs_device_count(int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBS_SBS_H */

@@ -1,0 +1,167 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+this module, and only as the checker or the timed CPU baseline.  The product
+library never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+SOLVER_JACOBI = 0
+SOLVER_CHOLESKY = 1
+SOLVER_EIGEN_CHOLESKY = 2
+SOLVER_PCG = 3
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int), ("num_iterations", ctypes.c_int), ("dt", ctypes.c_double),
+                ("gravity", ctypes.c_double), ("tol", ctypes.c_double), ("rho", ctypes.c_double),
+                ("positional_weight", ctypes.c_double), ("pcg_tol", ctypes.c_double),
+                ("pcg_floor", ctypes.c_double), ("pcg_max_iter", ctypes.c_int), ("pcg_chebyshev", ctypes.c_int)]
+
+
+def make_params(solver=SOLVER_JACOBI, num_iterations=10, dt=1.0 / 60.0, gravity=9.8, tol=1e-6, rho=0.9992,
+                positional_weight=1e6, pcg_tol=1e-6, pcg_floor=1e-7, pcg_max_iter=200,
+                pcg_chebyshev=0) -> OrcParams:
+    return OrcParams(solver, num_iterations, dt, gravity, tol, rho, positional_weight, pcg_tol, pcg_floor,
+                     pcg_max_iter, pcg_chebyshev)
+
+
+_LIBS: dict[str, ctypes.CDLL] = {}
+
+
+def build(bench: bool = False) -> str:
+    name = "liboracle_bench.so" if bench else "liboracle.so"
+    subprocess.run(["make", "-s", "-C", HERE, name], check=True)
+    return os.path.join(HERE, name)
+
+
+def lib(bench: bool = False) -> ctypes.CDLL:
+    name = "liboracle_bench.so" if bench else "liboracle.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        path = build(bench)
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    up = ctypes.POINTER(ctypes.c_uint32)
+    L.orc_svd3_f32.argtypes = [fp, fp, fp, fp]
+    L.orc_svd3_f64.argtypes = [dp, dp, dp, dp]
+    L.orc_local_rotation_f64.argtypes = [dp, ctypes.c_int, dp]
+    L.orc_local_rotation_f32.argtypes = [fp, ctypes.c_int, fp]
+    L.orc_pd_create.restype = ctypes.c_void_p
+    L.orc_pd_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, up, dp, dp, dp,
+                                ctypes.POINTER(OrcParams)]
+    L.orc_pd_destroy.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    L.orc_pd_step.restype = ctypes.c_int
+    L.orc_pd_step.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(OrcParams), ctypes.c_int, ip]
+    L.orc_pd_get.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, dp]
+    L.orc_pd_nnz.restype = ctypes.c_int
+    L.orc_pd_nnz.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    L.orc_pd_export.argtypes = [ctypes.c_int, ctypes.c_void_p, dp, dp, dp, dp, ip, ip, dp]
+    L.orc_pd_local.argtypes = [ctypes.c_int, ctypes.c_void_p, dp, dp, ctypes.c_int, ctypes.c_double,
+                               ctypes.c_double, dp]
+    L.orc_num_threads.restype = ctypes.c_int
+    _LIBS[name] = L
+    return L
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def svd3(A: np.ndarray, fp64: bool = True):
+    """TR1690 SVD of one 3x3 matrix: returns U, sigma, V with A = U diag(sigma) V^T."""
+    L = lib()
+    dt, ct = (np.float64, ctypes.c_double) if fp64 else (np.float32, ctypes.c_float)
+    A = np.ascontiguousarray(A, dtype=dt)
+    U = np.zeros((3, 3), dt); s = np.zeros(3, dt); V = np.zeros((3, 3), dt)
+    (L.orc_svd3_f64 if fp64 else L.orc_svd3_f32)(_p(A, ct), _p(U, ct), _p(s, ct), _p(V, ct))
+    return U, s, V
+
+
+def local_rotation(F: np.ndarray, jacobi: bool = False, fp64: bool = True) -> np.ndarray:
+    L = lib()
+    dt, ct = (np.float64, ctypes.c_double) if fp64 else (np.float32, ctypes.c_float)
+    F = np.ascontiguousarray(F, dtype=dt)
+    R = np.zeros((3, 3), dt)
+    (L.orc_local_rotation_f64 if fp64 else L.orc_local_rotation_f32)(_p(F, ct), int(jacobi), _p(R, ct))
+    return R
+
+
+class PdOracle:
+    """The reference PD step on the CPU for one flat scene (see pd_oracle.cpp)."""
+
+    def __init__(self, scene, params: OrcParams, fp64: bool = True, bench: bool = False, V0=None):
+        self.L = lib(bench)
+        self.fp64 = int(fp64)
+        self.params = params
+        self.nv, self.nt = scene.nv, scene.nt
+        X = np.ascontiguousarray(scene.X, dtype=np.float64)
+        tet = np.ascontiguousarray(scene.tet, dtype=np.uint32)
+        mass = np.ascontiguousarray(scene.mass, dtype=np.float64)
+        mu = np.ascontiguousarray(scene.mu, dtype=np.float64)
+        dbc = np.ascontiguousarray(scene.dbc, dtype=np.float64)
+        v0 = None if V0 is None else np.ascontiguousarray(V0, dtype=np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.h = self.L.orc_pd_create(self.fp64, self.nv, self.nt, _p(X, ctypes.c_double),
+                                      None if v0 is None else _p(v0, ctypes.c_double),
+                                      _p(tet, ctypes.c_uint32), _p(mass, ctypes.c_double),
+                                      _p(mu, ctypes.c_double), _p(dbc, ctypes.c_double), ctypes.byref(params))
+        if not self.h:
+            raise RuntimeError("oracle: factorisation failed")
+        del dp
+
+    def close(self):
+        if self.h:
+            self.L.orc_pd_destroy(self.fp64, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1) -> tuple[int, int]:
+        pcg = ctypes.c_int(0)
+        it = self.L.orc_pd_step(self.fp64, self.h, ctypes.byref(self.params), n, ctypes.byref(pcg))
+        return it, pcg.value
+
+    def get(self, which: str = "X") -> np.ndarray:
+        out = np.zeros(3 * self.nv)
+        self.L.orc_pd_get(self.fp64, self.h, {"X": 0, "V": 1, "XTilde": 2}[which], _p(out, ctypes.c_double))
+        return out.reshape(-1, 3)
+
+    def export(self) -> dict:
+        nnz = self.L.orc_pd_nnz(self.fp64, self.h)
+        d = dict(dminv=np.zeros(9 * self.nt), V0=np.zeros(self.nt), diagL=np.zeros(self.nv), c=np.zeros(self.nv),
+                 rowptr=np.zeros(self.nv + 1, np.int32), col=np.zeros(nnz, np.int32), val=np.zeros(nnz))
+        cd, ci = ctypes.c_double, ctypes.c_int
+        self.L.orc_pd_export(self.fp64, self.h, _p(d["dminv"], cd), _p(d["V0"], cd), _p(d["diagL"], cd),
+                             _p(d["c"], cd), _p(d["rowptr"], ci), _p(d["col"], ci), _p(d["val"], cd))
+        d["dminv"] = d["dminv"].reshape(-1, 3, 3)
+        return d
+
+    def local(self, q: np.ndarray, sn_old: np.ndarray, jacobi: bool, dt: float, positional_weight: float = 1e6):
+        q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+        s = np.ascontiguousarray(sn_old, dtype=np.float64).ravel()
+        out = np.zeros(3 * self.nv)
+        cd = ctypes.c_double
+        self.L.orc_pd_local(self.fp64, self.h, _p(q, cd), _p(s, cd), int(jacobi), dt, positional_weight,
+                            _p(out, cd))
+        return out.reshape(-1, 3)
+
+
+def num_threads(bench: bool = False) -> int:
+    return int(lib(bench).orc_num_threads())

@@ -1,0 +1,298 @@
+// extern "C" entry points declared in include/sbs/sbs.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/sbs/sbs.h"
+#include "direct_solver.hpp"
+#include "pd_engine.hpp"
+
+namespace sbs {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+}  // namespace sbs
+
+struct SbsPd {
+    int precision = 32;
+    std::unique_ptr<sbs::PdEngine<float>> f32;
+    std::unique_ptr<sbs::PdEngine<double>> f64;
+};
+
+namespace {
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        sbs::set_last_error("");
+        return SBS_OK;
+    } catch (const sbs::Error& e) {
+        sbs::set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        sbs::set_last_error("host allocation failed");
+        return SBS_E_OOM;
+    } catch (const std::exception& e) {
+        sbs::set_last_error(e.what());
+        return SBS_E_INTERNAL;
+    }
+}
+
+template <class T> sbs::PdData<T> to_data(const SbsSolverData* d) {
+    sbs::PdData<T> o;
+    o.num_verts = d->num_verts;
+    o.num_tets = d->num_tets;
+    o.num_dbc = d->num_dbc;
+    o.X = static_cast<T*>(d->X);
+    o.V = static_cast<T*>(d->V);
+    o.X0 = static_cast<T*>(d->X0);
+    o.XTilde = static_cast<T*>(d->XTilde);
+    o.DBCX = static_cast<T*>(d->DBCX);
+    o.Tet = d->Tet;
+    o.mass = static_cast<const T*>(d->mass);
+    o.mu = static_cast<const T*>(d->mu);
+    o.DBC = static_cast<const T*>(d->DBC);
+    o.moreDBC = static_cast<const T*>(d->moreDBC);
+    o.V0 = static_cast<T*>(d->V0);
+    return o;
+}
+
+sbs::PdParams to_params(const SbsSolverParams* p) {
+    sbs::PdParams o;
+    o.num_iterations = p->num_iterations;
+    o.dt = p->dt;
+    o.gravity = p->gravity;
+    o.tol = p->tol;
+    o.rho = p->rho;
+    o.handle_collision = p->handle_collision;
+    o.pcg_tol = p->pcg_tol;
+    o.pcg_floor = p->pcg_floor;
+    o.pcg_max_iter = p->pcg_max_iter;
+    o.pcg_chebyshev = p->pcg_chebyshev;
+    o.pcg_adaptive = p->pcg_adaptive;
+    return o;
+}
+
+// Calls f with the engine of the handle's precision.
+template <class F> void with_engine(SbsPd* pd, F&& f) {
+    SBS_REQUIRE(pd != nullptr, "null SbsPd handle");
+    if (pd->precision == 64) f(*pd->f64);
+    else f(*pd->f32);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sbs_last_error(void) { return sbs::last_error(); }
+int sbs_version(void) { return 1; }
+
+int sbs_pd_create(int precision, int threads_per_block, const SbsSolverData* data, SbsPd** out) {
+    return guarded([&] {
+        SBS_REQUIRE(out != nullptr && data != nullptr, "sbs_pd_create: null argument");
+        SBS_REQUIRE(precision == 32 || precision == 64, "sbs_pd_create: precision must be 32 or 64");
+        SBS_REQUIRE(threads_per_block > 0, "sbs_pd_create: threads_per_block must be positive");
+        *out = nullptr;
+        auto h = std::make_unique<SbsPd>();
+        h->precision = precision;
+        if (precision == 64) h->f64 = std::make_unique<sbs::PdEngine<double>>(threads_per_block, to_data<double>(data));
+        else h->f32 = std::make_unique<sbs::PdEngine<float>>(threads_per_block, to_data<float>(data));
+        *out = h.release();
+    });
+}
+
+int sbs_pd_destroy(SbsPd* pd) {
+    return guarded([&] { delete pd; });
+}
+
+int sbs_pd_set_global_solver(SbsPd* pd, int solver_type) {
+    return guarded([&] {
+        SBS_REQUIRE(solver_type >= 0 && solver_type <= 3, "sbs_pd_set_global_solver: unknown solver type");
+        with_engine(pd, [&](auto& e) { e.set_global_solver(static_cast<sbs::GlobalSolver>(solver_type)); });
+    });
+}
+
+int sbs_pd_update(SbsPd* pd, SbsSolverData* data, const SbsSolverParams* params) {
+    return guarded([&] {
+        SBS_REQUIRE(data != nullptr && params != nullptr, "sbs_pd_update: null argument");
+        const sbs::PdParams p = to_params(params);
+        if (pd && pd->precision == 64) {
+            auto d = to_data<double>(data);
+            pd->f64->update(d, p);
+        } else {
+            SBS_REQUIRE(pd != nullptr, "null SbsPd handle");
+            auto d = to_data<float>(data);
+            pd->f32->update(d, p);
+        }
+    });
+}
+
+int sbs_pd_reset(SbsPd* pd) {
+    return guarded([&] { with_engine(pd, [&](auto& e) { e.reset(); }); });
+}
+
+int sbs_pd_set_perf(SbsPd* pd, int on) {
+    return guarded([&] { with_engine(pd, [&](auto& e) { e.set_perf(on != 0); }); });
+}
+
+int sbs_pd_perf_count(SbsPd* pd, int* count) {
+    return guarded([&] {
+        SBS_REQUIRE(count != nullptr, "null count");
+        with_engine(pd, [&](auto& e) { *count = (int)e.perf().size(); });
+    });
+}
+
+int sbs_pd_perf_entry(SbsPd* pd, int index, const char** name, double* ms) {
+    return guarded([&] {
+        with_engine(pd, [&](auto& e) {
+            SBS_REQUIRE(index >= 0 && index < (int)e.perf().size(), "perf index out of range");
+            if (name) *name = e.perf()[index].name.c_str();
+            if (ms) *ms = e.perf()[index].ms;
+        });
+    });
+}
+
+int sbs_pd_set_stream(SbsPd* pd, void* hip_stream) {
+    return guarded([&] { with_engine(pd, [&](auto& e) { e.set_stream(static_cast<hipStream_t>(hip_stream)); }); });
+}
+
+int sbs_pd_stats(SbsPd* pd, SbsPdStats* out) {
+    return guarded([&] {
+        SBS_REQUIRE(out != nullptr, "null stats");
+        with_engine(pd, [&](auto& e) {
+            e.sync_counters();
+            out->pcg_iterations_last_step = e.last_pcg_iterations();
+            out->pcg_cap = e.pcg_cap();
+            out->pcg_cap_hits = e.pcg_cap_hits();
+            out->nnz_offdiag = (long long)e.nnz_offdiag();
+        });
+    });
+}
+
+int sbs_pd_set_kernel_timing(SbsPd* pd, unsigned mask) {
+    return guarded([&] { with_engine(pd, [&](auto& e) { e.set_kernel_timing(mask); }); });
+}
+
+int sbs_pd_kernel_time(SbsPd* pd, int kernel_id, const char** name, int* launches, double* ms) {
+    return guarded([&] {
+        with_engine(pd, [&](auto& e) {
+            const sbs::KernelTime t = e.kernel_time(kernel_id);
+            if (name) *name = sbs::kernel_name(kernel_id);
+            if (launches) *launches = t.launches;
+            if (ms) *ms = t.ms;
+        });
+    });
+}
+
+int sbs_pd_solve(SbsPd* pd, const void* d_b, void* d_x, const void* d_guess, const SbsSolverParams* params,
+                 int* iterations) {
+    return guarded([&] {
+        SBS_REQUIRE(d_b && d_x && params, "sbs_pd_solve: null argument");
+        const sbs::PdParams p = to_params(params);
+        with_engine(pd, [&](auto& e) {
+            using T = std::remove_reference_t<decltype(e)>;
+            using S = std::conditional_t<std::is_same_v<T, sbs::PdEngine<double>>, double, float>;
+            const size_t bytes = 3 * (size_t)e.num_verts() * sizeof(S);
+            if (d_guess && d_guess != d_x)
+                SBS_HIP(hipMemcpyAsync(d_x, d_guess, bytes, hipMemcpyDeviceToDevice, e.stream()));
+            const int it = e.solve(static_cast<const S*>(d_b), static_cast<S*>(d_x), p, d_guess != nullptr);
+            if (iterations) *iterations = it;
+        });
+    });
+}
+
+int sbs_pd_local_step(SbsPd* pd, const void* d_q, const void* d_sn_old, int residual_form, double dt, void* d_b) {
+    return guarded([&] {
+        SBS_REQUIRE(d_q && d_sn_old && d_b, "sbs_pd_local_step: null argument");
+        with_engine(pd, [&](auto& e) {
+            using T = std::remove_reference_t<decltype(e)>;
+            using S = std::conditional_t<std::is_same_v<T, sbs::PdEngine<double>>, double, float>;
+            e.local_only(static_cast<const S*>(d_q), static_cast<const S*>(d_sn_old), residual_form != 0, dt,
+                         static_cast<S*>(d_b));
+            SBS_HIP(hipStreamSynchronize(e.stream()));
+        });
+    });
+}
+
+int sbs_pd_system_nnz(SbsPd* pd, long long* nnz) {
+    return guarded([&] {
+        SBS_REQUIRE(nnz != nullptr, "null nnz");
+        with_engine(pd, [&](auto& e) {
+            SBS_REQUIRE(e.prepared(), "system not prepared (call sbs_pd_update first)");
+            *nnz = (long long)e.nnz_offdiag() + e.num_verts();
+        });
+    });
+}
+
+int sbs_pd_export_system(SbsPd* pd, void* c, void* diagA, void* diagL, void* V0, void* w, void* dminv, int* rowptr,
+                         int* col, void* val) {
+    return guarded([&] {
+        with_engine(pd, [&](auto& e) {
+            auto ex = e.export_system();
+            using S = typename decltype(ex.c)::value_type;
+            auto put = [](void* dst, const std::vector<S>& src) {
+                if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(S));
+            };
+            put(V0, ex.V0);
+            put(w, ex.w);
+            if (dminv) {  // engine planes -> row-major per tet
+                const size_t nt = ex.V0.size();
+                S* o = static_cast<S*>(dminv);
+                for (size_t t = 0; t < nt; ++t)
+                    for (int k = 0; k < 9; ++k) o[9 * t + k] = ex.dminv[(size_t)k * nt + t];
+            }
+            if (e.prepared()) {
+                put(c, ex.c);
+                put(diagA, ex.diagA);
+                put(diagL, ex.diagL);
+                if (rowptr) std::memcpy(rowptr, ex.rowptr.data(), ex.rowptr.size() * sizeof(int));
+                if (col) std::memcpy(col, ex.col.data(), ex.col.size() * sizeof(int));
+                put(val, ex.val);
+            } else {
+                SBS_REQUIRE(!(c || diagA || diagL || rowptr || col || val), "system not prepared (call sbs_pd_update first)");
+            }
+        });
+    });
+}
+
+int sbs_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        SBS_REQUIRE(ptr != nullptr, "null pointer");
+        *ptr = nullptr;
+        SBS_HIP(hipMalloc(ptr, bytes));
+    });
+}
+int sbs_free(void* ptr) {
+    return guarded([&] { if (ptr) SBS_HIP(hipFree(ptr)); });
+}
+int sbs_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { SBS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int sbs_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { SBS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int sbs_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { SBS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
+}
+int sbs_memset(void* dst, int value, size_t bytes) {
+    return guarded([&] { SBS_HIP(hipMemset(dst, value, bytes)); });
+}
+int sbs_device_synchronize(void) {
+    return guarded([&] { SBS_HIP(hipDeviceSynchronize()); });
+}
+int sbs_device_count(int* count) {
+    return guarded([&] {
+        SBS_REQUIRE(count != nullptr, "null count");
+        *count = 0;
+        SBS_HIP(hipGetDeviceCount(count));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+// Projective-dynamics engine for gfx950: state, prepared system and the step.
+//
+// Reference behaviour this engine reproduces:
+//   PdSolver::SolverPrepare   src/simulation/solver/projective/pdSolver.cu:40-137
+//   PdSolver::SolverStep      src/simulation/solver/projective/pdSolver.cu:141-208
+//   PdSolver::Update          src/simulation/solver/projective/pdSolver.cu:210-232
+//   FEMSolver ctor            src/simulation/solver/femSolver.cu:7-17
+//
+// Layout in HBM (one body or many bodies concatenated, like the reference):
+//   * vertex vectors are interleaved xyz (3*N_v scalars), the reference's
+//     glm::vec3 AoS layout, so a gather of one neighbour is one 12-byte run;
+//   * tets stay u32[4*N_t] (read as one 16-byte load per tet);
+//   * per-tet constants are SoA: Dm^-1 as 9 planes and w = |V0|*mu as one,
+//     so each plane is read fully coalesced;
+//   * the system is the scalar matrix A_s = L_s + diag(c) (the reference's
+//     3N x 3N matrix is A_s (x) I_3): its diagonal is kept as one N_v array
+//     and the off-diagonal part in sliced ELL with 64-row slices (one slice
+//     per wavefront, column-major inside the slice).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sbs {
+
+template <class T> class BandCholesky;
+
+enum class GlobalSolver : int { Jacobi = 0, CuSolverCholesky = 1, EigenCholesky = 2, PCG = 3 };
+
+// Mirrors the fields of SolverParams<Scalar> (src/def.h:83-100) used by the
+// PD step, plus the settings of the PCG global step.
+struct PdParams {
+    int num_iterations = 1;
+    double dt = 0.001;
+    double gravity = 9.8;
+    double tol = 1e-2;
+    double rho = 0.9992;
+    int handle_collision = 0;
+    double pcg_tol = 1e-6;      // residual reduction ||D^-1 r_k|| <= pcg_tol ||D^-1 r_0||
+    double pcg_floor = 1e-8;    // ... or <= pcg_floor ||D^-1 b||
+    int pcg_max_iter = 100;
+    int pcg_chebyshev = 0;      // apply the Chebyshev recurrence after each PCG solve
+    int pcg_adaptive = 1;       // launch a learned iteration cap instead of syncing per chunk
+};
+
+// Caller-owned device arrays: the reference's SolverData<Scalar> (src/def.h:22-56).
+template <class T> struct PdData {
+    int num_verts = 0, num_tets = 0, num_dbc = 0;
+    T* X = nullptr;        // 3*N_v
+    T* V = nullptr;        // 3*N_v
+    T* X0 = nullptr;       // 3*N_v, rest positions (Dm^-1, V0 and DBCX source)
+    T* XTilde = nullptr;   // 3*N_v
+    T* DBCX = nullptr;     // 3*N_v, pin targets (set to X0 at prepare)
+    const uint32_t* Tet = nullptr;  // 4*N_t
+    const T* mass = nullptr;        // N_v
+    const T* mu = nullptr;          // N_t
+    const T* DBC = nullptr;         // N_v, 0/1 pin flags
+    const T* moreDBC = nullptr;     // N_v, extra soft pins (may be null: all zero)
+    T* V0 = nullptr;       // N_t, filled at construction when non-null
+};
+
+struct PerfEntry {
+    std::string name;
+    double ms = 0.0;
+};
+
+// Kernels that can be timed with HIP events on the engine's stream.
+enum KernelId : int {
+    kKLocal = 0,
+    kKPcgInit,
+    kKPcgSpmv,
+    kKPcgUpdate,
+    kKPcgDirection,
+    kKJacobi,
+    kKChebyshev,
+    kKDirect,
+    kKCount
+};
+const char* kernel_name(int id);
+
+struct KernelTime {
+    int launches = 0;
+    double ms = 0.0;
+};
+
+template <class T> class PdEngine {
+public:
+    PdEngine(int threads_per_block, const PdData<T>& data);
+    ~PdEngine();
+
+    void set_global_solver(GlobalSolver s) { solver_ = s; }
+    GlobalSolver global_solver() const { return solver_; }
+    void set_perf(bool on) { perf_ = on; }
+    void set_stream(hipStream_t s) { stream_ = s; }
+    hipStream_t stream() const { return stream_; }
+    const std::vector<PerfEntry>& perf() const { return perf_data_; }
+    void reset();
+
+    // One PdSolver::Update (prepare on first call, then the step).
+    void update(PdData<T>& data, const PdParams& params);
+
+    // Local step alone on a given iterate, for kernel-level tests:
+    // b = c*sn_old + sum_t w_t P_t Dm^-T G^T, then the pinned overwrite.
+    void local_only(const T* q, const T* sn_old, bool residual_form, double dt, T* b_out);
+
+    // Scalar linear solve A_s x = b for 3 interleaved RHS with the prepared
+    // matrix (direct when factored, else PCG); exposes the Solve(b, x) call.
+    int solve(const T* b, T* x, const PdParams& params, bool use_guess);
+
+    // Prepared system, for tests: c (N_v), diag(A_s) (N_v), sliced-ELL parts.
+    struct Export {
+        std::vector<T> c, diagA, diagL, V0, w, dminv;
+        std::vector<int> rowptr, col;
+        std::vector<T> val;
+    };
+    Export export_system() const;
+
+    int last_iterations() const { return last_iters_; }
+    int last_pcg_iterations() const { return last_pcg_iters_; }
+    int pcg_cap() const { return cap_; }
+    // Event pairs around every launch of the kernels in `mask` (bit = KernelId);
+    // kernel_times() waits for the stream and sums the elapsed times.
+    void set_kernel_timing(unsigned mask);
+    KernelTime kernel_time(int id);
+    long long pcg_cap_hits() const { return cap_hits_total_; }
+    // Waits for the last step and folds its counters into the cap statistics.
+    void sync_counters() {
+        if (step_pending_) { SBS_HIP(hipEventSynchronize(step_event_)); review_counters(); }
+    }
+    int num_verts() const { return nv_; }
+    int num_tets() const { return nt_; }
+    size_t nnz_offdiag() const { return nnz_off_; }
+    bool prepared() const { return ready_; }
+    void prepare(PdData<T>& data, const PdParams& params);
+
+private:
+    void build_structure();
+    void assemble(const PdParams& params);
+    void pcg_solve(const T* b_src, T* x, bool reset_b, const T* sn_old, bool use_guess, const PdParams& params,
+                   bool dbc_override, bool exact, const T* guess_src);
+    void pcg_iterations(T* x, int count);
+    void review_counters();
+    void direct_factor();
+    void direct_solve(const T* b, T* x);
+
+    int tpb_ = 256;
+    int nv_ = 0, nt_ = 0;
+    PdData<T> data_{};
+    GlobalSolver solver_ = GlobalSolver::Jacobi;
+    bool ready_ = false;
+    bool perf_ = false;
+    hipStream_t stream_ = nullptr;
+    std::vector<PerfEntry> perf_data_;
+    double prepared_dt_ = 0.0;
+
+    // per tet
+    DevBuf<T> dminv_;   // 9 planes of N_t
+    DevBuf<T> w_;       // |V0| * mu
+    DevBuf<T> v0_;      // V0
+    // per vertex
+    DevBuf<T> c_, diagL_, diagA_, dinv_;
+    DevBuf<T> sn_, sn_old_, prev_, next_, b_;
+    // matrix: sliced ELL (64-row slices) of the off-diagonal part
+    int nslices_ = 0;
+    size_t nnz_off_ = 0;
+    DevBuf<int> slice_ptr_;   // nslices+1 (offset of each slice in the ELL arrays)
+    DevBuf<int> slice_w_;     // nslices (width of each slice)
+    DevBuf<int> rowlen_;      // N_v
+    DevBuf<uint32_t> ecol_;   // padded entries
+    DevBuf<T> eval_;
+    // PCG workspace
+    DevBuf<T> r_, z_, p_, q_;
+    DevBuf<double> partial_;
+    DevBuf<unsigned char> ctl_;
+    int red_blocks_ = 0;
+    // direct solver (band Cholesky factor, GPU triangular solves) for small bodies
+    std::unique_ptr<BandCholesky<T>> direct_;
+    int last_iters_ = 0, last_pcg_iters_ = 0;
+    // pinned host mirror of iteration counters: [total, cap_hits, max_used, solves, scratch]
+    int* host_counters_ = nullptr;
+    hipEvent_t step_event_ = nullptr;
+    bool step_pending_ = false;
+    int cap_ = 0;                 // adaptive PCG launch cap (0 = unknown)
+    unsigned timing_mask_ = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tev_[kKCount];
+    size_t tev_used_[kKCount] = {};
+    KernelTime ktime_[kKCount];
+    void tick(int id, bool start);
+    void flush_timing();
+    long long cap_hits_total_ = 0;
+    std::vector<uint32_t> host_tets_;
+};
+
+}  // namespace sbs

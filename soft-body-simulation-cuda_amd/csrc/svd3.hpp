@@ -1,0 +1,188 @@
+// 3x3 rotation-variant SVD held in registers, for the per-tet local step.
+//
+// Algorithm: McAdams, Selle, Tamstorf, Teran, Sifakis, "Computing the
+// Singular Value Decomposition of 3x3 matrices with minimal branching and
+// elementary floating point operations", UW-Madison TR1690 (2011) -- the
+// method the reference vendors as external/svd3_cuda/svd3_cuda.h:36-1043 and
+// calls through svdGLM (src/simulation/solver/svd.cuh:8-14):
+//   1. S = A^T A;
+//   2. 4 cyclic Jacobi sweeps over the pairs (0,1), (1,2), (2,0) with the
+//      approximate half-angle Givens rotation (gamma = 3 + 2 sqrt 2 test,
+//      pi/8 fallback), accumulating V;
+//   3. B = A V, columns sorted by descending norm; each swap negates one
+//      column (1-2: the 2nd, 1-3: the 1st, 2-3: the 3rd) so det V stays +1;
+//   4. Givens QR of B (zeroing B10, B20, B21), U = G1 G2 G3, sigma = diag(B).
+// U and V are proper rotations; only sigma[2] can be negative.
+//
+// Everything is unrolled with compile-time indices so the 3x3 matrices stay
+// in VGPRs.  The sequence of operations matches the CPU oracle so FP64 runs
+// agree to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sbs {
+
+template <class T> __device__ __forceinline__ T rsq(T x);
+template <> __device__ __forceinline__ float rsq<float>(float x) { return rsqrtf(x); }
+template <> __device__ __forceinline__ double rsq<double>(double x) { return rsqrt(x); }
+
+template <class T> struct Svd3 {
+    __device__ __forceinline__ static void jacobi_half(T a, T b, T d, T& ch, T& sh) {
+        const T tiny = (T)1e-20, four_gamma_sq = (T)5.8284271247461900976;
+        const T cpi8 = (T)0.92387953251128675613, spi8 = (T)0.38268343236508977173;
+        sh = (T)0.5 * b;
+        ch = a - d;
+        const bool small = sh * sh < tiny;
+        sh = small ? (T)0 : sh;
+        ch = small ? (T)1 : ch;
+        const T w = rsq<T>(ch * ch + sh * sh);
+        const bool keep = four_gamma_sq * sh * sh < ch * ch;
+        ch = keep ? w * ch : cpi8;
+        sh = keep ? w * sh : spi8;
+    }
+
+    __device__ __forceinline__ static void qr_half(T a1, T a2, T& ch, T& sh) {
+        const T eps = (T)1e-12;
+        const T s = a2 * a2 >= eps ? a2 : (T)0;
+        const T m = fmax(fabs(a1), eps);
+        const T c = m + sqrt(m * m + s * s);
+        const bool keep = a1 >= (T)0;
+        ch = keep ? c : s;
+        sh = keep ? s : c;
+        const T w = rsq<T>(ch * ch + sh * sh);
+        ch *= w;
+        sh *= w;
+    }
+
+    // A row-major: A[r][c].  Outputs U, V row-major, sig.
+    __device__ __forceinline__ static void run(const T (&A)[3][3], T (&U)[3][3], T (&sig)[3], T (&V)[3][3]) {
+        T M[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) M[r][c] = A[0][r] * A[0][c] + A[1][r] * A[1][c] + A[2][r] * A[2][c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) V[r][c] = (r == c) ? (T)1 : (T)0;
+#pragma unroll
+        for (int sweep = 0; sweep < 4; ++sweep) {
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const int p = e;             // pairs (0,1), (1,2), (2,0)
+                const int q = (e + 1) % 3;
+                const int k = 3 - p - q;
+                T ch, sh;
+                jacobi_half(M[p][p], M[p][q], M[q][q], ch, sh);
+                const T c = ch * ch - sh * sh, s = (T)2 * ch * sh;
+                const T a = M[p][p], b = M[p][q], d = M[q][q];
+                const T app = c * c * a + (T)2 * c * s * b + s * s * d;
+                const T aqq = s * s * a - (T)2 * c * s * b + c * c * d;
+                const T apq = (c * c - s * s) * b + c * s * (d - a);
+                const T apk = c * M[p][k] + s * M[q][k];
+                const T aqk = -s * M[p][k] + c * M[q][k];
+                M[p][p] = app;
+                M[q][q] = aqq;
+                M[p][q] = apq;
+                M[q][p] = apq;
+                M[p][k] = apk;
+                M[k][p] = apk;
+                M[q][k] = aqk;
+                M[k][q] = aqk;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const T vp = V[r][p], vq = V[r][q];
+                    V[r][p] = c * vp + s * vq;
+                    V[r][q] = -s * vp + c * vq;
+                }
+            }
+        }
+        T B[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B[r][c] = A[r][0] * V[0][c] + A[r][1] * V[1][c] + A[r][2] * V[2][c];
+        T rho[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rho[c] = B[0][c] * B[0][c] + B[1][c] * B[1][c] + B[2][c] * B[2][c];
+        swap_neg<0, 1, 1>(B, V, rho);
+        swap_neg<0, 2, 0>(B, V, rho);
+        swap_neg<1, 2, 2>(B, V, rho);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) U[r][c] = (r == c) ? (T)1 : (T)0;
+        qr_step<0, 1>(B, U);
+        qr_step<0, 2>(B, U);
+        qr_step<1, 2>(B, U);
+        sig[0] = B[0][0];
+        sig[1] = B[1][1];
+        sig[2] = B[2][2];
+    }
+
+    // When rho_i < rho_j: swap columns i, j and negate column N (branch-free).
+    template <int I, int J, int N>
+    __device__ __forceinline__ static void swap_neg(T (&B)[3][3], T (&V)[3][3], T (&rho)[3]) {
+        const bool sw = rho[I] < rho[J];
+        const T sgn_i = (sw && N == I) ? (T)-1 : (T)1;
+        const T sgn_j = (sw && N == J) ? (T)-1 : (T)1;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const T bi = B[r][I], bj = B[r][J], vi = V[r][I], vj = V[r][J];
+            B[r][I] = sgn_i * (sw ? bj : bi);
+            B[r][J] = sgn_j * (sw ? bi : bj);
+            V[r][I] = sgn_i * (sw ? vj : vi);
+            V[r][J] = sgn_j * (sw ? vi : vj);
+        }
+        const T ri = rho[I], rj = rho[J];
+        rho[I] = sw ? rj : ri;
+        rho[J] = sw ? ri : rj;
+    }
+
+    template <int P, int Q> __device__ __forceinline__ static void qr_step(T (&B)[3][3], T (&U)[3][3]) {
+        T ch, sh;
+        qr_half(B[P][P], B[Q][P], ch, sh);
+        const T c = ch * ch - sh * sh, s = (T)2 * ch * sh;
+#pragma unroll
+        for (int col = 0; col < 3; ++col) {
+            const T bp = B[P][col], bq = B[Q][col];
+            B[P][col] = c * bp + s * bq;
+            B[Q][col] = -s * bp + c * bq;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const T up = U[r][P], uq = U[r][Q];
+            U[r][P] = c * up + s * uq;
+            U[r][Q] = -s * up + c * uq;
+        }
+    }
+};
+
+template <class T> __device__ __forceinline__ T det3(const T (&m)[3][3]) {
+    return m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[0][1] * (m[1][0] * m[2][2] - m[2][0] * m[1][2]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[2][0] * m[1][1]);
+}
+
+// Projection of computeLocal (src/simulation/solver/projective/pdUtil.cu:112-125):
+// R = U V^T, negate column 2 if det R < 0, and R - F in Jacobi (residual) mode.
+template <class T>
+__device__ __forceinline__ void pd_projection(const T (&F)[3][3], bool residual_form, T (&P)[3][3]) {
+    T U[3][3], sig[3], V[3][3];
+    Svd3<T>::run(F, U, sig, V);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) P[r][c] = U[r][0] * V[c][0] + U[r][1] * V[c][1] + U[r][2] * V[c][2];
+    const bool flip = det3(P) < (T)0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) P[r][2] = flip ? -P[r][2] : P[r][2];
+    if (residual_form) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) P[r][c] -= F[r][c];
+    }
+}
+
+}  // namespace sbs

@@ -1,0 +1,220 @@
+"""GPU parity of the PD step: the HIP library through its C ABI vs the CPU oracle.
+
+Bars (stated per test): FP32 positions within 1e-5 relative of the FP32
+oracle, FP64 within 1e-10 of the FP64 oracle.  Relative = ||a-b||_2/||b||_2
+and max|a-b| / max|b| over all vertex coordinates; both are checked.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import max_rel, rel_err
+from sbs_amd import mesh
+from sbs_amd.pd import PdSolver, SolverData, SolverParams, SolverType
+
+pytestmark = pytest.mark.gpu
+
+TOL = {32: 1e-5, 64: 1e-10}
+ORC_SOLVER = {SolverType.Jacobi: O.SOLVER_JACOBI, SolverType.CuSolverCholesky: O.SOLVER_CHOLESKY,
+              SolverType.EigenCholesky: O.SOLVER_EIGEN_CHOLESKY, SolverType.PCG: O.SOLVER_PCG}
+
+
+def params_for(solver, n_iter=10, **kw):
+    p = SolverParams(numIterations=n_iter, dt=1.0 / 60.0, gravity=9.8, tol=kw.pop("tol", 1e-6), rho=0.9992,
+                     pcg_tol=kw.pop("pcg_tol", 1e-6), pcg_floor=kw.pop("pcg_floor", 1e-8),
+                     pcg_max_iter=kw.pop("pcg_max_iter", 200), pcg_chebyshev=kw.pop("pcg_chebyshev", False),
+                     pcg_adaptive=kw.pop("pcg_adaptive", False))
+    assert not kw
+    return p
+
+
+def oracle_params(p: SolverParams, solver):
+    return O.make_params(solver=ORC_SOLVER[solver], num_iterations=p.numIterations, dt=p.dt, gravity=p.gravity,
+                         tol=p.tol, rho=p.rho, pcg_tol=p.pcg_tol, pcg_floor=p.pcg_floor,
+                         pcg_max_iter=p.pcg_max_iter, pcg_chebyshev=int(p.pcg_chebyshev))
+
+
+def run_gpu(scene, solver, p, nsteps, precision):
+    data = SolverData(scene, precision)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(solver)
+    for _ in range(nsteps):
+        pd.Update(data, p)
+    pd.synchronize()
+    out = {k: data.get(k) for k in ("X", "V", "XTilde")}
+    stats = pd.stats()
+    pd.close()
+    data.free()
+    return out, stats
+
+
+def run_oracle(scene, solver, p, nsteps, precision):
+    o = O.PdOracle(scene, oracle_params(p, solver), fp64=(precision == 64))
+    o.step(nsteps)
+    return {k: o.get(k) for k in ("X", "V", "XTilde")}
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_fem_precompute_and_assembly(gpu, precision):
+    s = mesh.cube_scene(6)
+    p = params_for(SolverType.Jacobi)
+    data = SolverData(s, precision)
+    pd = PdSolver(128, data)
+    pd.Update(data, p)
+    ex = pd.export_system()
+    o = O.PdOracle(s, oracle_params(p, SolverType.Jacobi), fp64=(precision == 64))
+    ox = o.export()
+    tol = 1e-5 if precision == 32 else 1e-12
+    np.testing.assert_allclose(ex["dminv"], ox["dminv"], rtol=tol, atol=tol * np.abs(ox["dminv"]).max())
+    np.testing.assert_allclose(ex["V0"], ox["V0"], rtol=tol)
+    np.testing.assert_allclose(data.get("V0"), ox["V0"], rtol=tol)
+    np.testing.assert_array_equal(ex["rowptr"], ox["rowptr"])
+    np.testing.assert_array_equal(ex["col"], ox["col"])
+    np.testing.assert_allclose(ex["val"], ox["val"], rtol=tol, atol=tol * np.abs(ox["val"]).max() * 1e-3)
+    np.testing.assert_allclose(ex["diagL"], ox["diagL"], rtol=tol)
+    np.testing.assert_allclose(ex["c"], ox["c"], rtol=tol)
+    pd.close()
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("residual_form", [True, False])
+def test_local_step_matches_oracle(gpu, precision, residual_form):
+    from sbs_amd import capi
+    s = mesh.cube_scene(6)
+    rng = np.random.default_rng(7)
+    p = params_for(SolverType.Jacobi)
+    data = SolverData(s, precision)
+    pd = PdSolver(128, data)
+    pd.Update(data, p)   # prepares the system
+    q = s.X + rng.normal(size=s.X.shape) * 5e-3
+    sn_old = s.X + rng.normal(size=s.X.shape) * 1e-3
+    dt = np.float32 if precision == 32 else np.float64
+    dq = capi.DeviceArray.from_numpy(q.astype(dt).ravel())
+    ds = capi.DeviceArray.from_numpy(sn_old.astype(dt).ravel())
+    db = capi.DeviceArray(q.size * np.dtype(dt).itemsize)
+    pd.local_step(dq.ptr, ds.ptr, residual_form, p.dt, db.ptr)
+    b = db.download(dt, q.shape)
+    o = O.PdOracle(s, oracle_params(p, SolverType.Jacobi), fp64=(precision == 64))
+    ob = o.local(q, sn_old, jacobi=residual_form, dt=p.dt)
+    free = s.dbc == 0
+    # elastic contribution relative to its own scale (the c*sn_old term is exact)
+    c = o.export()["c"][:, None]
+    el_gpu, el_orc = (b - c * sn_old)[free], (ob - c * sn_old)[free]
+    tol = 2e-5 if precision == 32 else 1e-10
+    assert np.abs(el_gpu - el_orc).max() <= tol * np.abs(el_orc).max()
+    np.testing.assert_array_equal(b[~free], ob[~free].astype(dt))
+    pd.close()
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("solver", [SolverType.Jacobi, SolverType.CuSolverCholesky, SolverType.PCG])
+def test_plain_pd_cube_100_steps(gpu, precision, solver):
+    """The 8^3-hex cube (729 v / 3072 t), 10 iterations per step, 100 steps."""
+    s = mesh.cube_scene(8)
+    if precision == 32:
+        p = params_for(solver, n_iter=10)
+    else:
+        # FP64 bar is 1e-10: no threshold (early exit, PCG stop) may flip between
+        # the two implementations, so solve to machine precision without early exit
+        p = params_for(solver, n_iter=10, tol=0.0, pcg_tol=1e-13, pcg_floor=0.0, pcg_max_iter=400)
+    g, st = run_gpu(s, solver, p, 100, precision)
+    r = run_oracle(s, solver, p, 100, precision)
+    for k in ("X", "XTilde"):
+        assert rel_err(g[k], r[k]) < TOL[precision], (k, rel_err(g[k], r[k]))
+        assert max_rel(g[k], r[k]) < TOL[precision], (k, max_rel(g[k], r[k]))
+    # velocities: relative to their own scale
+    assert np.abs(g["V"] - r["V"]).max() < 50 * TOL[precision] * max(np.abs(r["V"]).max(), 1e-3)
+    assert st.pcg_cap_hits == 0
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_eigen_cholesky_mode_matches_cusolver_mode(gpu, precision):
+    # same factor and solve; only the order of the local step's float atomics
+    # differs between runs, so compare at rounding level (no early exit)
+    s = mesh.cube_scene(5)
+    p = params_for(SolverType.CuSolverCholesky, n_iter=10, tol=0.0)
+    a, _ = run_gpu(s, SolverType.CuSolverCholesky, p, 20, precision)
+    b, _ = run_gpu(s, SolverType.EigenCholesky, p, 20, precision)
+    assert rel_err(a["X"], b["X"]) < (1e-6 if precision == 32 else 1e-13)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_pcg_chebyshev_midsize(gpu, precision):
+    """The config-1 global step (PCG + Chebyshev, 20 iterations) on a 16^3 cube."""
+    s = mesh.cube_scene(16)
+    p = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True,
+                   pcg_floor=1e-8 if precision == 32 else 0.0, pcg_tol=1e-6 if precision == 32 else 1e-12)
+    g, st = run_gpu(s, SolverType.PCG, p, 5, precision)
+    r = run_oracle(s, SolverType.PCG, p, 5, precision)
+    assert rel_err(g["X"], r["X"]) < TOL[precision]
+    assert max_rel(g["X"], r["X"]) < TOL[precision]
+    assert st.pcg_cap_hits == 0
+
+
+def test_pcg_adaptive_launch_cap_matches_exact(gpu):
+    s = mesh.cube_scene(12)
+    pe = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_adaptive=False)
+    pa = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_adaptive=True)
+    e, _ = run_gpu(s, SolverType.PCG, pe, 10, 32)
+    a, st = run_gpu(s, SolverType.PCG, pa, 10, 32)
+    assert st.pcg_cap_hits == 0
+    assert st.pcg_cap > 0
+    assert rel_err(a["X"], e["X"]) < 1e-6
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_linear_solve_matches_dense_solve(gpu, precision):
+    from sbs_amd import capi
+    s = mesh.cube_scene(5)
+    rng = np.random.default_rng(3)
+    dt = np.float32 if precision == 32 else np.float64
+    for solver in (SolverType.PCG, SolverType.CuSolverCholesky):
+        data = SolverData(s, precision)
+        pd = PdSolver(128, data)
+        pd.SetGlobalSolver(solver)
+        p = params_for(solver, pcg_tol=1e-7 if precision == 32 else 1e-14, pcg_floor=0.0, pcg_max_iter=500)
+        pd.Update(data, p)
+        ex = pd.export_system()
+        A = np.zeros((s.nv, s.nv))
+        for r in range(s.nv):
+            sl = slice(ex["rowptr"][r], ex["rowptr"][r + 1])
+            A[r, ex["col"][sl]] = ex["val"][sl]
+        xs = rng.normal(size=(s.nv, 3))
+        b = A @ xs
+        db = capi.DeviceArray.from_numpy(b.astype(dt).ravel())
+        dx = capi.DeviceArray(b.size * np.dtype(dt).itemsize)
+        pd.solve(db.ptr, dx.ptr, p)
+        x = dx.download(dt, b.shape)
+        # compare in the energy-free Jacobi-scaled sense: error relative to |x|
+        tol = 1e-3 if precision == 32 else 1e-9
+        assert rel_err(x, xs) < tol, (solver, rel_err(x, xs))
+        pd.close()
+
+
+def test_free_fall_is_exact_at_full_size(gpu):
+    """64^3-hex cube without pins: every mode must give implicit-Euler free fall."""
+    s = mesh.cube_scene(64, pin_top=False)
+    p = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_adaptive=True)
+    n = 3
+    g, st = run_gpu(s, SolverType.PCG, p, n, 64)
+    expect = s.X.copy()
+    expect[:, 1] -= p.gravity * p.dt ** 2 * n * (n + 1) / 2
+    assert np.abs(g["X"] - expect).max() < 1e-9
+    assert st.pcg_cap_hits == 0
+
+
+def test_full_size_fp32_tracks_fp64(gpu):
+    """64^3-hex cube (274,625 v / 1,572,864 t), config-1 step: FP32 vs FP64 GPU."""
+    s = mesh.cube_scene(64)
+    p32 = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_adaptive=True)
+    p64 = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_adaptive=True, pcg_floor=0.0,
+                     pcg_tol=1e-10)
+    a, sa = run_gpu(s, SolverType.PCG, p32, 3, 32)
+    b, sb = run_gpu(s, SolverType.PCG, p64, 3, 64)
+    assert sa.pcg_cap_hits == 0 and sb.pcg_cap_hits == 0
+    assert rel_err(a["X"], b["X"]) < 1e-5
+    assert max_rel(a["X"], b["X"]) < 1e-5
+    # the cube sags under gravity but stays pinned at the top
+    top = s.dbc > 0
+    assert np.abs(b["X"][top] - s.X[top]).max() < 1e-4
+    assert (b["X"][~top, 1] - s.X[~top, 1]).max() < 1e-6
