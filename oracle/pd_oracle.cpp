@@ -624,7 +624,8 @@ void orc_pd_local(int fp64, void* h, const double* q, const double* sn_old, int 
         for (int v = 0; v < pd->nv; ++v)
             for (int k = 0; k < 3; ++k) b[3 * v + k] = pd->c[v] * (S)sn_old[3 * v + k];
         pd->local_step(qs.data(), jacobi != 0, b.data(), contrib);
-        const S wdbc = (S)positional_weight / ((S)dt * (S)dt);
+        const S dtInv = (S)1 / (S)dt, dt2Inv = dtInv * dtInv;
+        const S wdbc = (S)positional_weight * dt2Inv;
         for (int v = 0; v < pd->nv; ++v)
             if (pd->dbc[v] > 0)
                 for (int k = 0; k < 3; ++k) b[3 * v + k] = pd->DBCX[3 * v + k] * wdbc;

@@ -1,8 +1,5 @@
 // Band Cholesky factor (host, FP64) and blocked triangular solves (gfx950).
-#include <algorithm>
-#include <cmath>
-#include <queue>
-
+#include "band_factor.hpp"
 #include "direct_solver.hpp"
 
 namespace sbs {
@@ -142,58 +139,6 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, con
     }
 }
 
-// Reverse Cuthill-McKee ordering of a symmetric pattern (all components).
-std::vector<int> rcm_order(int n, const std::vector<int>& rowptr, const std::vector<int>& col) {
-    std::vector<int> deg(n);
-    for (int i = 0; i < n; ++i) deg[i] = rowptr[i + 1] - rowptr[i];
-    std::vector<int> order;
-    order.reserve(n);
-    std::vector<char> seen(n, 0);
-    auto bfs = [&](int s, std::vector<int>& out, std::vector<int>& level) {
-        out.clear();
-        std::queue<int> qu;
-        qu.push(s);
-        std::vector<char> vis(n, 0);
-        vis[s] = 1;
-        level.assign(n, -1);
-        level[s] = 0;
-        while (!qu.empty()) {
-            int u = qu.front();
-            qu.pop();
-            out.push_back(u);
-            std::vector<int> nb;
-            for (int k = rowptr[u]; k < rowptr[u + 1]; ++k)
-                if (!vis[col[k]] && !seen[col[k]]) { vis[col[k]] = 1; nb.push_back(col[k]); }
-            std::sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
-            for (int w : nb) { level[w] = level[u] + 1; qu.push(w); }
-        }
-    };
-    std::vector<int> comp, level;
-    for (int s0 = 0; s0 < n; ++s0) {
-        if (seen[s0]) continue;
-        // pseudo-peripheral start: two BFS sweeps
-        int s = s0;
-        for (int rep = 0; rep < 2; ++rep) {
-            bfs(s, comp, level);
-            int best = s;
-            for (int u : comp)
-                if (level[u] > level[best] || (level[u] == level[best] && deg[u] < deg[best])) best = u;
-            s = best;
-        }
-        bfs(s, comp, level);
-        for (int u : comp) { seen[u] = 1; order.push_back(u); }
-    }
-    std::reverse(order.begin(), order.end());
-    return order;
-}
-
-int bandwidth_of(int n, const std::vector<int>& rowptr, const std::vector<int>& col, const std::vector<int>& pos) {
-    int bw = 0;
-    for (int i = 0; i < n; ++i)
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) bw = std::max(bw, std::abs(pos[i] - pos[col[k]]));
-    return bw;
-}
-
 }  // namespace
 
 template <class T> int BandCholesky<T>::max_n() {
@@ -208,46 +153,24 @@ void BandCholesky<T>::factor(int n, const std::vector<int>& rowptr, const std::v
     if (n > max_n())
         throw Error(kInvalidArgument, "direct (Cholesky) global solve supports at most " + std::to_string(max_n()) +
                                           " vertices per system in this precision; use the PCG global solver");
-    std::vector<int> natural(n), pos_nat(n);
-    for (int i = 0; i < n; ++i) natural[i] = pos_nat[i] = i;
-    std::vector<int> rcm = rcm_order(n, rowptr, col), pos_rcm(n);
-    for (int i = 0; i < n; ++i) pos_rcm[rcm[i]] = i;
-    const int bw_nat = bandwidth_of(n, rowptr, col, pos_nat), bw_rcm = bandwidth_of(n, rowptr, col, pos_rcm);
-    const bool use_rcm = bw_rcm < bw_nat;
-    const std::vector<int>& perm = use_rcm ? rcm : natural;
-    const std::vector<int>& pos = use_rcm ? pos_rcm : pos_nat;
-    const int bw = use_rcm ? bw_rcm : bw_nat;
-    // band of the permuted matrix: Lb[i*(bw+1) + d] = A'[i][i-d]
-    std::vector<double> Lb((size_t)n * (bw + 1), 0.0);
-    for (int i = 0; i < n; ++i)
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            const int pi = pos[i], pj = pos[col[k]];
-            if (pj <= pi) Lb[(size_t)pi * (bw + 1) + (pi - pj)] = (double)val[k];
-        }
-    for (int i = 0; i < n; ++i) {
-        double* Li = &Lb[(size_t)i * (bw + 1)];
-        for (int j = std::max(0, i - bw); j <= i; ++j) {
-            const double* Lj = &Lb[(size_t)j * (bw + 1)];
-            double s = Li[i - j];
-            for (int k = std::max(0, i - bw); k < j; ++k) s -= Li[i - k] * Lj[j - k];
-            if (j < i) {
-                Li[i - j] = s / Lj[0];
-            } else {
-                if (!(s > 0.0)) throw Error(kFactorFailed, "Cholesky factorisation failed: matrix not SPD at row " +
-                                                                std::to_string(i));
-                Li[0] = std::sqrt(s);
-            }
-        }
+    BandFactor f;
+    try {
+        f = band_factor(n, rowptr, col, val);
+    } catch (const std::invalid_argument& e) {
+        throw Error(kInvalidArgument, e.what());
+    } catch (const std::domain_error& e) {
+        throw Error(kFactorFailed, e.what());
     }
+    const int bw = f.bw;
     std::vector<T> ld((size_t)(bw + 1) * n);
     for (int i = 0; i < n; ++i)
-        for (int d = 0; d <= bw; ++d) ld[(size_t)d * n + i] = (T)Lb[(size_t)i * (bw + 1) + d];
+        for (int d = 0; d <= bw; ++d) ld[(size_t)d * n + i] = (T)f.Lb[(size_t)i * (bw + 1) + d];
     n_ = n;
     bw_ = bw;
     ld_.alloc(ld.size());
     perm_.alloc(n);
     SBS_HIP(hipMemcpyAsync(ld_.get(), ld.data(), ld.size() * sizeof(T), hipMemcpyHostToDevice, stream));
-    SBS_HIP(hipMemcpyAsync(perm_.get(), perm.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+    SBS_HIP(hipMemcpyAsync(perm_.get(), f.perm.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
     SBS_HIP(hipStreamSynchronize(stream));
 }
 

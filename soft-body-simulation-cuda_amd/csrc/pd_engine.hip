@@ -18,6 +18,15 @@ constexpr int kSlice = 64;  // sliced-ELL slice height = one wavefront
 constexpr int kPcgChunk = 8;   // iterations per host check in exact mode
 constexpr int kPcgMargin = 2;  // adaptive cap = most iterations seen + margin
 
+// Pin weight of computeDBCLocal: positional_weight * dt2Inv with
+// dt2Inv = dtInv * dtInv in the solver precision (pdSolver.cu:145-147, :171;
+// positional_weight = 1e6, pdSolver.h:31).
+template <class T> T pin_weight(double dt) {
+    const T dt_inv = (T)1 / (T)dt;
+    const T dt2_inv = dt_inv * dt_inv;
+    return (T)1e6 * dt2_inv;
+}
+
 template <class T> __device__ __forceinline__ void ld3(const T* __restrict__ p, unsigned i, T& x, T& y, T& z) {
     x = p[3 * (size_t)i];
     y = p[3 * (size_t)i + 1];
@@ -765,6 +774,7 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
     SBS_HIP(hipMemsetAsync(ctl_.get(), 0, sizeof(CtlBlock), stream_));
     build_structure();
     assemble(params);
+    system_built_ = true;
     prepared_dt_ = params.dt;
     // DBCX <- X0 (pdSolver.cu:134)
     SBS_HIP(hipMemcpyAsync(data.DBCX, data.X0, n3 * sizeof(T), hipMemcpyDeviceToDevice, stream_));
@@ -775,6 +785,7 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
 }
 
 template <class T> void PdEngine<T>::direct_factor() {
+    SBS_REQUIRE(system_built_, "direct factorisation requested before the system was assembled");
     Export ex = export_system();
     direct_ = std::make_unique<BandCholesky<T>>();
     direct_->factor(nv_, ex.rowptr, ex.col, ex.val, stream_);
@@ -820,7 +831,7 @@ void PdEngine<T>::pcg_solve(const T* b_src, T* x, bool reset_b, const T* sn_old,
                             bool dbc_override, bool exact, const T* guess_src) {
     CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
     const unsigned nb = (unsigned)red_blocks_;
-    const T wdbc = (T)(1e6 / (params.dt * params.dt));
+    const T wdbc = pin_weight<T>(params.dt);
     if (!use_guess) SBS_HIP(hipMemsetAsync(x, 0, 3 * (size_t)nv_ * sizeof(T), stream_));
     tick(kKPcgInit, true);
     hipLaunchKernelGGL(pcg_init_kernel<T>, dim3(nb), dim3(kTpb), 0, stream_, nv_, const_cast<T*>(b_src), data_.DBC,
@@ -865,7 +876,7 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     if (direct && !direct_) direct_factor();
     const T dt = (T)params.dt;
     const unsigned vb = ceil_div(nv_, kTpb), tb = ceil_div(nt_, kTpb);
-    const T wdbc = (T)1e6 * ((T)1 / dt) * ((T)1 / dt);
+    const T wdbc = pin_weight<T>(params.dt);
     CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
     const bool jacobi = solver_ == GlobalSolver::Jacobi;
     const bool pcg = solver_ == GlobalSolver::PCG;
@@ -967,7 +978,7 @@ template <class T> void PdEngine<T>::local_only(const T* q, const T* sn_old, boo
                        (const int*)nullptr);
     hipLaunchKernelGGL(local_step_kernel<T>, dim3(tb), dim3(kTpb), 0, stream_, nt_, (const uint4*)data_.Tet,
                        dminv_.get(), w_.get(), q, b_.get(), (int)residual_form, (const int*)nullptr);
-    const T wdbc = (T)(1e6 / (dt * dt));
+    const T wdbc = pin_weight<T>(dt);
     hipLaunchKernelGGL(rhs_pinned_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)b_.get(), data_.DBC,
                        data_.moreDBC, data_.DBCX, wdbc, b_out, (const int*)nullptr);
     SBS_HIP(hipGetLastError());
@@ -1003,7 +1014,7 @@ template <class T> typename PdEngine<T>::Export PdEngine<T>::export_system() con
     d2h(ex.dminv, dminv_, 9 * (size_t)nt_);
     d2h(ex.V0, v0_, nt_);
     d2h(ex.w, w_, nt_);
-    if (!ready_) return ex;
+    if (!system_built_) return ex;
     d2h(ex.c, c_, nv_);
     d2h(ex.diagA, diagA_, nv_);
     d2h(ex.diagL, diagL_, nv_);

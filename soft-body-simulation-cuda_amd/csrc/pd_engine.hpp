@@ -152,6 +152,7 @@ private:
     PdData<T> data_{};
     GlobalSolver solver_ = GlobalSolver::Jacobi;
     bool ready_ = false;
+    bool system_built_ = false;   // matrix assembled (set inside prepare)
     bool perf_ = false;
     hipStream_t stream_ = nullptr;
     std::vector<PerfEntry> perf_data_;
