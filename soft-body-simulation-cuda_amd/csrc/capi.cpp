@@ -172,6 +172,8 @@ int sbs_pd_stats(SbsPd* pd, SbsPdStats* out) {
             out->pcg_cap = e.pcg_cap();
             out->pcg_cap_hits = e.pcg_cap_hits();
             out->nnz_offdiag = (long long)e.nnz_offdiag();
+            out->pcg_iterations_total = e.pcg_iterations_total();
+            out->pcg_solves_total = e.pcg_solves_total();
         });
     });
 }

@@ -1,7 +1,8 @@
 // gfx950 kernels and host orchestration of the projective-dynamics step.
 // See pd_engine.hpp for the layout; reference citations are per kernel.
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -9,14 +10,21 @@
 #include "pd_engine.hpp"
 #include "svd3.hpp"
 
+namespace cg = cooperative_groups;
+
 namespace sbs {
 
 namespace {
 
-constexpr int kTpb = 256;   // 4 wavefronts per block
-constexpr int kSlice = 64;  // sliced-ELL slice height = one wavefront
-constexpr int kPcgChunk = 8;   // iterations per host check in exact mode
-constexpr int kPcgMargin = 2;  // adaptive cap = most iterations seen + margin
+constexpr int kTpb = 256;      // threads per block of the streaming kernels (4 wavefronts)
+constexpr int kSlice = 64;     // sliced-ELL slice height = one wavefront
+constexpr int kPcgTpb = 256;   // threads per block of the persistent PCG kernel
+constexpr int kAccBank = 64;   // doubles per accumulator bank
+// accumulator slots inside a bank
+constexpr int kAccInit = 0;    // 9: r.z, z.z, (D^-1 b)^2 per component
+constexpr int kAccA = 16;      // 2 x 4: p.q per component, double-buffered by iteration parity
+constexpr int kAccB = 24;      // 2 x 8: r.z, z.z per component, double-buffered
+constexpr int kAccErr = 40;    // 1: sum (prev - x)^2
 
 // Pin weight of computeDBCLocal: positional_weight * dt2Inv with
 // dt2Inv = dtInv * dtInv in the solver precision (pdSolver.cu:145-147, :171;
@@ -27,29 +35,105 @@ template <class T> T pin_weight(double dt) {
     return (T)1e6 * dt2_inv;
 }
 
-template <class T> __device__ __forceinline__ void ld3(const T* __restrict__ p, unsigned i, T& x, T& y, T& z) {
-    x = p[3 * (size_t)i];
-    y = p[3 * (size_t)i + 1];
-    z = p[3 * (size_t)i + 2];
+template <class T> __device__ __forceinline__ void ld3(const T* __restrict__ p, size_t i, T& x, T& y, T& z) {
+    x = p[3 * i];
+    y = p[3 * i + 1];
+    z = p[3 * i + 2];
 }
-template <class T> __device__ __forceinline__ void st3(T* __restrict__ p, unsigned i, T x, T y, T z) {
-    p[3 * (size_t)i] = x;
-    p[3 * (size_t)i + 1] = y;
-    p[3 * (size_t)i + 2] = z;
+template <class T> __device__ __forceinline__ void st3(T* __restrict__ p, size_t i, T x, T y, T z) {
+    p[3 * i] = x;
+    p[3 * i + 1] = y;
+    p[3 * i + 2] = z;
+}
+
+__device__ __forceinline__ double acc_load(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void acc_store(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void acc_add(double* p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sums NQ per-thread values over the block; thread 0 adds them to acc[0..NQ).
+template <int NQ, int TPB>
+__device__ __forceinline__ void block_accumulate(const double (&vals)[NQ], double* acc) {
+    __shared__ double red[TPB / 64][NQ];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const double s = wave_sum(vals[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            double s = 0;
+            for (int w = 0; w < TPB / 64; ++w) s += red[w][k];
+            acc_add(&acc[k], s);
+        }
+    }
+    __syncthreads();
+}
+
+// Grid-wide barrier with explicit agent-scope release/acquire: the L2 of the
+// 8 XCDs are not coherent with each other, so rows written by one XCD and
+// gathered by another need a write-back before and an invalidate after.
+__device__ __forceinline__ void grid_barrier(cg::grid_group& grid) {
+    __syncthreads();
+    if (threadIdx.x == 0) __threadfence();
+    grid.sync();
+    if (threadIdx.x == 0) __threadfence();
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Device control block: per-step counters and the outer-loop early exit.
+struct CtlBlock {
+    int outer_done;      // local/global loop early exit (direct / PCG modes)
+    int pcg_iters;       // PCG iterations summed over the step
+    int unconverged;     // PCG solves that hit max_iter before the tolerance
+    int max_used;        // most iterations of one solve this step
+    int solves;          // PCG solves this step
+    int last_iters;      // iterations of the last solve
+    unsigned ticket;     // block arrival counter of the ticket reductions
+    int pad;
+    double err;          // last computeError value
+};
+
+__global__ void ctl_init_kernel(CtlBlock* ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ctl->outer_done = 0;
+    ctl->pcg_iters = 0;
+    ctl->unconverged = 0;
+    ctl->max_used = 0;
+    ctl->solves = 0;
+    ctl->last_iters = 0;
+    ctl->ticket = 0;
+    ctl->err = 1.0;
+}
+
+__global__ void zero_kernel(double* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
 // FEM precompute: computeInvDmV0 (src/simulation/solver/solverUtil.cuh:109-137).
-// Dm = [x1-x0, x2-x0, x3-x0] (columns), Dm^-1 stored row-major in 9 planes,
-// V0 = |det Dm| / 6, w = |V0| * mu (the coefficient of pdUtil.cu:19 and :126).
+// Dm = [x1-x0, x2-x0, x3-x0] (columns), Dm^-1 stored row-major in 9 planes
+// at the tet's internal position, V0 = |det Dm| / 6, w = |V0| * mu (the
+// coefficient of pdUtil.cu:19 and :126).
 template <class T>
 __global__ void __launch_bounds__(kTpb) fem_precompute_kernel(int nt, const T* __restrict__ X,
                                                               const uint4* __restrict__ tets,
+                                                              const int* __restrict__ perm,
                                                               const T* __restrict__ mu, T* __restrict__ dminv,
                                                               T* __restrict__ v0, T* __restrict__ w,
                                                               T* __restrict__ v0_out) {
-    const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= (unsigned)nt) return;
+    const unsigned i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= (unsigned)nt) return;
+    const int t = perm[i];
     const uint4 tv = tets[t];
     T p0[3], p1[3], p2[3], p3[3];
     ld3(X, tv.x, p0[0], p0[1], p0[2]);
@@ -75,37 +159,36 @@ __global__ void __launch_bounds__(kTpb) fem_precompute_kernel(int nt, const T* _
     inv[7] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) / d;
     inv[8] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) / d;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) dminv[(size_t)k * nt + t] = inv[k];
+    for (int k = 0; k < 9; ++k) dminv[(size_t)k * nt + i] = inv[k];
     const T vol = fabs(d) / (T)6;
-    v0[t] = vol;
-    w[t] = fabs(vol) * mu[t];
+    v0[i] = vol;
+    w[i] = fabs(vol) * mu[t];
     if (v0_out) v0_out[t] = vol;
 }
 
-// Gradient operator rows of the linear tet: g0 = -(m0+m1+m2), g_{j+1} = m_j,
-// where m_j is row j of Dm^-1 (pdUtil.cu:16, glm mat4x3 {-1,-1,-1,1,0,0,0,1,0,0,0,1}).
 template <class T> __device__ __forceinline__ void load_dminv(const T* __restrict__ dminv, int nt, unsigned t, T (&m)[9]) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = dminv[(size_t)k * nt + t];
 }
 
 // ---------------------------------------------------------------------------
-// Assembly of L_s = sum_t w_t G Dm^-1 Dm^-T G^T (computeSiTSi, pdUtil.cu:9-40):
-// diagonal into diagL (the reference's matrix_diag), off-diagonal entries
-// into the sliced-ELL values found by a search of the (short) row.
+// Assembly of L_s = sum_t w_t G Dm^-1 Dm^-T G^T (computeSiTSi, pdUtil.cu:9-40),
+// G rows (-1,-1,-1),(1,0,0),(0,1,0),(0,0,1) (pdUtil.cu:16): diagonal into
+// diagL (the reference's matrix_diag), off-diagonal entries into the
+// sliced-ELL values found by a search of the (short) row.  One-time prepare.
 template <class T>
 __global__ void __launch_bounds__(kTpb) assemble_kernel(int nt, const uint4* __restrict__ tets,
-                                                        const T* __restrict__ dminv, const T* __restrict__ w,
-                                                        const int* __restrict__ slice_ptr,
+                                                        const int* __restrict__ perm, const T* __restrict__ dminv,
+                                                        const T* __restrict__ w, const int* __restrict__ slice_ptr,
                                                         const int* __restrict__ rowlen,
                                                         const uint32_t* __restrict__ ecol, T* __restrict__ eval,
                                                         T* __restrict__ diagL) {
-    const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= (unsigned)nt) return;
-    const uint4 tv = tets[t];
+    const unsigned i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= (unsigned)nt) return;
+    const uint4 tv = tets[perm[i]];
     const unsigned v[4] = {tv.x, tv.y, tv.z, tv.w};
     T m[9];
-    load_dminv(dminv, nt, t, m);
+    load_dminv(dminv, nt, i, m);
     T g[4][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -114,22 +197,22 @@ __global__ void __launch_bounds__(kTpb) assemble_kernel(int nt, const uint4* __r
         g[2][k] = m[3 + k];
         g[3][k] = m[6 + k];
     }
-    const T ww = w[t];
+    const T ww = w[i];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const unsigned row = v[i];
+    for (int a = 0; a < 4; ++a) {
+        const unsigned row = v[a];
         const size_t base = (size_t)slice_ptr[row / kSlice] + (row % kSlice);
         const int len = rowlen[row];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const T kij = ww * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2]);
-            if (i == j) {
-                atomicAdd(&diagL[row], kij);
+        for (int bb = 0; bb < 4; ++bb) {
+            const T kab = ww * (g[a][0] * g[bb][0] + g[a][1] * g[bb][1] + g[a][2] * g[bb][2]);
+            if (a == bb) {
+                atomicAdd(&diagL[row], kab);
             } else {
                 for (int e = 0; e < len; ++e) {
                     const size_t pos = base + (size_t)e * kSlice;
-                    if (ecol[pos] == v[j]) {
-                        atomicAdd(&eval[pos], kij);
+                    if (ecol[pos] == v[bb]) {
+                        atomicAdd(&eval[pos], kab);
                         break;
                     }
                 }
@@ -154,14 +237,15 @@ __global__ void mass_diag_kernel(int nv, const T* __restrict__ mass, const T* __
 
 // ---------------------------------------------------------------------------
 // Step prologue: gravity force (pdSolver.cu:154-155), setMDt_2MoreDBC
-// (pdUtil.cu:56-71), computeSn (pdUtil.cu:73-95), sn_old copy, prev init
-// (pdSolver.cu:158-162) and the first addM_h2Sn (pdUtil.cu:168-178).
+// (pdUtil.cu:56-71), computeSn (pdUtil.cu:73-95), the sn_old copy and the prev
+// initialisation (pdSolver.cu:158-162; prev = 0 outside Jacobi mode, the
+// reference memsets bytes there).
 template <class T>
 __global__ void begin_step_kernel(int nv, const T* __restrict__ X, const T* __restrict__ V,
                                   const T* __restrict__ mass, const T* __restrict__ dbc,
                                   const T* __restrict__ more_dbc, T dt, T gravity, T* __restrict__ c,
                                   T* __restrict__ sn, T* __restrict__ sn_old, T* __restrict__ prev,
-                                  T* __restrict__ b, int prev_from_sn) {
+                                  int prev_from_sn) {
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= (unsigned)nv) return;
     const T m = mass[v];
@@ -182,93 +266,136 @@ __global__ void begin_step_kernel(int nv, const T* __restrict__ X, const T* __re
     st3(sn_old, v, sx, sy, sz);
     if (prev_from_sn) st3(prev, v, sx, sy, sz);
     else st3(prev, v, (T)0, (T)0, (T)0);
-    st3(b, v, cv * sx, cv * sy, cv * sz);
 }
 
 // ---------------------------------------------------------------------------
-// computeLocal (src/simulation/solver/projective/pdUtil.cu:97-145): one thread
-// per tet, F = Ds Dm^-1, 3x3 SVD in registers, R (or R - F), force columns
-// w R Dm^-T G^T scattered into b with 12 float atomics.
+// computeLocal (src/simulation/solver/projective/pdUtil.cu:97-145), one
+// workgroup per chunk of kChunkTets Morton-ordered tets:
+//   1. the chunk's vertices (a sorted list) are staged from q into LDS;
+//   2. one thread per tet: F = Ds Dm^-1, 3x3 SVD in registers, R (or R - F),
+//      force columns w R Dm^-T G^T added into LDS accumulators (LDS atomics);
+//   3. the chunk writes one partial force per vertex it touches (coalesced,
+//      no global atomics).
 template <class T>
-__global__ void __launch_bounds__(kTpb) local_step_kernel(int nt, const uint4* __restrict__ tets,
-                                                          const T* __restrict__ dminv, const T* __restrict__ w,
-                                                          const T* __restrict__ q, T* __restrict__ b,
-                                                          int residual_form, const int* __restrict__ skip) {
+__global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
+    int nt, const uint2* __restrict__ lidx, const int* __restrict__ chunk_vptr,
+    const uint32_t* __restrict__ chunk_verts, const T* __restrict__ dminv, const T* __restrict__ w,
+    const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= (unsigned)nt) return;
-    const uint4 tv = tets[t];
-    T m[9];
-    load_dminv(dminv, nt, t, m);
-    const T ww = w[t];
-    T p0[3], p1[3], p2[3], p3[3];
-    ld3(q, tv.x, p0[0], p0[1], p0[2]);
-    ld3(q, tv.y, p1[0], p1[1], p1[2]);
-    ld3(q, tv.z, p2[0], p2[1], p2[2]);
-    ld3(q, tv.w, p3[0], p3[1], p3[2]);
-    T F[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        const T d0 = p1[r] - p0[r], d1 = p2[r] - p0[r], d2 = p3[r] - p0[r];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) F[r][c] = d0 * m[c] + d1 * m[3 + c] + d2 * m[6 + c];
+    extern __shared__ unsigned char smem_raw[];
+    const unsigned c = xcd_block(blockIdx.x, gridDim.x);
+    const int e0 = chunk_vptr[c], nu = chunk_vptr[c + 1] - e0;
+    T* qs = reinterpret_cast<T*>(smem_raw);
+    T* as = qs + 3 * (size_t)nu;
+    for (int i = threadIdx.x; i < nu; i += kChunkTets) {
+        const uint32_t g = chunk_verts[e0 + i];
+        T x, y, z;
+        ld3(q, g, x, y, z);
+        qs[3 * i] = x; qs[3 * i + 1] = y; qs[3 * i + 2] = z;
+        as[3 * i] = (T)0; as[3 * i + 1] = (T)0; as[3 * i + 2] = (T)0;
     }
-    T P[3][3];
-    pd_projection(F, residual_form != 0, P);
-    // Q[:, j] = w * P * (row j of Dm^-1)
-    T Q[3][3];
+    __syncthreads();
+    const unsigned t = c * kChunkTets + threadIdx.x;
+    if (t < (unsigned)nt) {
+        const uint2 li = lidx[t];
+        const unsigned l0 = li.x & 0xffffu, l1 = li.x >> 16, l2 = li.y & 0xffffu, l3 = li.y >> 16;
+        T m[9];
+        load_dminv(dminv, nt, t, m);
+        const T ww = w[t];
+        T F[3][3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r) {
+            const T x0 = qs[3 * l0 + r];
+            const T d0 = qs[3 * l1 + r] - x0, d1 = qs[3 * l2 + r] - x0, d2 = qs[3 * l3 + r] - x0;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) Q[r][j] = ww * (P[r][0] * m[3 * j] + P[r][1] * m[3 * j + 1] + P[r][2] * m[3 * j + 2]);
+            for (int k = 0; k < 3; ++k) F[r][k] = d0 * m[k] + d1 * m[3 + k] + d2 * m[6 + k];
+        }
+        T P[3][3];
+        pd_projection(F, residual_form != 0, P);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        atomicAdd(&b[3 * (size_t)tv.x + r], -(Q[r][0] + Q[r][1] + Q[r][2]));
-        atomicAdd(&b[3 * (size_t)tv.y + r], Q[r][0]);
-        atomicAdd(&b[3 * (size_t)tv.z + r], Q[r][1]);
-        atomicAdd(&b[3 * (size_t)tv.w + r], Q[r][2]);
+        for (int r = 0; r < 3; ++r) {
+            // column j of w P Dm^-T is w P (row j of Dm^-1); vertex 0 gets minus their sum
+            const T f1 = ww * (P[r][0] * m[0] + P[r][1] * m[1] + P[r][2] * m[2]);
+            const T f2 = ww * (P[r][0] * m[3] + P[r][1] * m[4] + P[r][2] * m[5]);
+            const T f3 = ww * (P[r][0] * m[6] + P[r][1] * m[7] + P[r][2] * m[8]);
+            atomicAdd(&as[3 * l0 + r], -(f1 + f2 + f3));
+            atomicAdd(&as[3 * l1 + r], f1);
+            atomicAdd(&as[3 * l2 + r], f2);
+            atomicAdd(&as[3 * l3 + r], f3);
+        }
     }
+    __syncthreads();
+    T* out = partial + 3 * (size_t)e0;
+    for (int i = threadIdx.x; i < 3 * nu; i += kChunkTets) out[i] = as[i];
 }
 
-// b entries of pinned vertices are overwritten with DBCX * 1e6/h^2
-// (computeDBCLocal, pdUtil.cu:147-166); used by the kernels below as a
-// substitution on read instead of a separate pass.
-template <class T>
-__device__ __forceinline__ void rhs_at(const T* __restrict__ b, const T* __restrict__ dbc,
-                                       const T* __restrict__ more_dbc, const T* __restrict__ dbcx, T wdbc,
-                                       unsigned v, T& x, T& y, T& z) {
-    const T pin = dbc[v];
-    const T more = more_dbc ? more_dbc[v] : (T)0;
-    if (pin > (T)0) {
-        ld3(dbcx, v, x, y, z);
-        x *= wdbc; y *= wdbc; z *= wdbc;
-    } else if (more > (T)0) {
-        ld3(dbcx, v, x, y, z);
-        x *= more; y *= more; z *= more;
-    } else {
-        ld3(b, v, x, y, z);
+// Right-hand side of one vertex: addM_h2Sn (pdUtil.cu:168-178) plus the
+// local-step partials of every chunk touching it, with the pinned overwrite
+// of computeDBCLocal (pdUtil.cu:147-166).
+template <class T> struct RhsGather {
+    const T* c;
+    const T* sn_old;
+    const int* vent_ptr;
+    const uint32_t* vent;
+    const T* partial;
+    const T* dbc;
+    const T* more_dbc;
+    const T* dbcx;
+    T wdbc;
+
+    __device__ __forceinline__ void at(unsigned v, T& bx, T& by, T& bz) const {
+        const T pin = dbc[v];
+        const T more = more_dbc ? more_dbc[v] : (T)0;
+        if (pin > (T)0) {
+            ld3(dbcx, v, bx, by, bz);
+            bx *= wdbc; by *= wdbc; bz *= wdbc;
+            return;
+        }
+        if (more > (T)0) {
+            ld3(dbcx, v, bx, by, bz);
+            bx *= more; by *= more; bz *= more;
+            return;
+        }
+        T sx, sy, sz;
+        ld3(sn_old, v, sx, sy, sz);
+        const T cv = c[v];
+        bx = cv * sx; by = cv * sy; bz = cv * sz;
+        const int e1 = vent_ptr[v + 1];
+        for (int e = vent_ptr[v]; e < e1; ++e) {
+            const uint32_t k = vent[e];
+            T px, py, pz;
+            ld3(partial, k, px, py, pz);
+            bx += px; by += py; bz += pz;
+        }
     }
+};
+
+// b_eff = gathered right-hand side, written out (direct solve input, tests).
+template <class T>
+__global__ void rhs_kernel(int nv, RhsGather<T> g, T* __restrict__ out, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (v >= (unsigned)nv) return;
+    T x, y, z;
+    g.at(v, x, y, z);
+    st3(out, v, x, y, z);
 }
 
 // Jacobi + Chebyshev global step (getErrorKern pdUtil.cu:195-214 and
-// chebyshevKern pdUtil.cu:216-226 fused per vertex), followed by the next
-// iteration's addM_h2Sn (b = c*sn_old) so no separate pass is needed.
+// chebyshevKern pdUtil.cu:216-226 fused per vertex).
 template <class T>
-__global__ void jacobi_chebyshev_kernel(int nv, T* __restrict__ b, const T* __restrict__ c,
-                                        const T* __restrict__ diagL, const T* __restrict__ dbc,
-                                        const T* __restrict__ more_dbc, const T* __restrict__ dbcx, T wdbc,
-                                        T* __restrict__ sn, T* __restrict__ prev, const T* __restrict__ sn_old,
-                                        T omega) {
+__global__ void jacobi_chebyshev_kernel(int nv, RhsGather<T> g, const T* __restrict__ diagL,
+                                        T* __restrict__ sn, T* __restrict__ prev, T omega) {
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= (unsigned)nv) return;
     T bx, by, bz;
-    rhs_at(b, dbc, more_dbc, dbcx, wdbc, v, bx, by, bz);
+    g.at(v, bx, by, bz);
     T sx, sy, sz, px, py, pz;
     ld3(sn, v, sx, sy, sz);
     ld3(prev, v, px, py, pz);
-    const T cv = c[v];
+    const T cv = g.c[v];
     T nx, ny, nz;
-    if (more_dbc && more_dbc[v] > (T)0) {
+    if (g.more_dbc && g.more_dbc[v] > (T)0) {
         nx = sx; ny = sy; nz = sz;
     } else {
         const T den = cv + diagL[v];
@@ -284,30 +411,12 @@ __global__ void jacobi_chebyshev_kernel(int nv, T* __restrict__ b, const T* __re
     nz = (nz - pz) * omega + pz;
     st3(prev, v, sx, sy, sz);
     st3(sn, v, nx, ny, nz);
-    T ox, oy, oz;
-    ld3(sn_old, v, ox, oy, oz);
-    st3(b, v, cv * ox, cv * oy, cv * oz);
-}
-
-// Chebyshev recurrence applied to a PCG solution held in `next`.
-template <class T>
-__global__ void chebyshev_kernel(int nv, const T* __restrict__ next, T* __restrict__ sn, T* __restrict__ prev,
-                                 T omega, const int* __restrict__ skip) {
-    if (skip && *skip) return;
-    const unsigned i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= 3u * (unsigned)nv) return;
-    const T s = sn[i];
-    T nx = (T)0.9 * (next[i] - s) + s;
-    nx = (nx - prev[i]) * omega + prev[i];
-    prev[i] = s;
-    sn[i] = nx;
 }
 
 // updateVelPos (pdUtil.cu:180-193) and X <- XTilde (pdSolver.cu:227).
 template <class T>
 __global__ void end_step_kernel(int nv, const T* __restrict__ sn, T dt_inv, T* __restrict__ XTilde,
-                                T* __restrict__ V, T* __restrict__ X, const T* __restrict__ more_dbc,
-                                int copy_x) {
+                                T* __restrict__ V, T* __restrict__ X, const T* __restrict__ more_dbc) {
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= (unsigned)nv) return;
     T nx, ny, nz, ox, oy, oz;
@@ -316,341 +425,294 @@ __global__ void end_step_kernel(int nv, const T* __restrict__ sn, T dt_inv, T* _
     if (more_dbc && more_dbc[v] > (T)0) st3(V, v, (T)0, (T)0, (T)0);
     else st3(V, v, (nx - ox) * dt_inv, (ny - oy) * dt_inv, (nz - oz) * dt_inv);
     st3(XTilde, v, nx, ny, nz);
-    if (copy_x) st3(X, v, nx, ny, nz);
+    st3(X, v, nx, ny, nz);
 }
 
-// ---------------------------------------------------------------------------
-// Jacobi-PCG over the scalar matrix with 3 fused right-hand sides.
-// Reductions: each block reduces its rows with wave shuffles + LDS and writes
-// one partial per quantity; the last block to arrive (atomic ticket) sums the
-// partials in double and updates the control block, so no kernel launch or
-// host round trip is spent on a dot product.
-struct CtlBlock {
-    unsigned ticket;
-    int done;        // PCG converged or out of iterations
-    int outer_done;  // local/global loop early exit (direct / PCG modes)
-    int iters;       // PCG iterations of the current solve
-    int total_iters; // PCG iterations since the start of the step
-    int max_iter;    // hard limit per solve
-    int launch_cap;  // iterations the host launched for each solve
-    int cap_hits;    // solves that ran out of launched iterations unconverged
-    int max_used;    // most iterations any solve of this step needed
-    int solves;      // PCG solves started this step
-    int active[3];
-    double rz[3], thr[3], alpha[3], beta[3];
-    double tol2, floor2, outer_tol, err;
-};
-
-__global__ void ctl_init_kernel(CtlBlock* ctl, int max_iter, int launch_cap, double tol2, double floor2,
-                                double outer_tol) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    ctl->ticket = 0;
-    ctl->done = 1;
-    ctl->outer_done = 0;
-    ctl->iters = 0;
-    ctl->total_iters = 0;
-    ctl->max_iter = max_iter;
-    ctl->launch_cap = launch_cap;
-    ctl->cap_hits = 0;
-    ctl->max_used = 0;
-    ctl->solves = 0;
-    for (int k = 0; k < 3; ++k) {
-        ctl->active[k] = 0;
-        ctl->rz[k] = ctl->thr[k] = ctl->alpha[k] = ctl->beta[k] = 0.0;
-    }
-    ctl->tol2 = tol2;
-    ctl->floor2 = floor2;
-    ctl->outer_tol = outer_tol;
-    ctl->err = 1.0;
-}
-
-template <int NQ>
-__device__ __forceinline__ bool block_partials(const double (&vals)[NQ], double* __restrict__ partial,
-                                               CtlBlock* ctl) {
-    __shared__ double red[kTpb / 64][NQ];
-    __shared__ bool last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-        const double s = wave_sum(vals[k]);
-        if (lane == 0) red[wid][k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < NQ; ++k) {
-            double s = 0;
-            for (int w2 = 0; w2 < kTpb / 64; ++w2) s += red[w2][k];
-            partial[(size_t)blockIdx.x * NQ + k] = s;
-        }
-        __threadfence();
-        const unsigned tk = atomicAdd(&ctl->ticket, 1u);
-        last = (tk == gridDim.x - 1);
-    }
-    __syncthreads();
-    return last;
-}
-
-template <int NQ>
-__device__ __forceinline__ void last_block_sum(const double* __restrict__ partial, double (&out)[NQ]) {
-    __shared__ double red[kTpb / 64][NQ];
-    __threadfence();
-    double acc[NQ];
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) acc[k] = 0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) acc[k] += __builtin_nontemporal_load(&partial[(size_t)i * NQ + k]);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-        const double s = wave_sum(acc[k]);
-        if (lane == 0) red[wid][k] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-        double s = 0;
-        for (int w2 = 0; w2 < kTpb / 64; ++w2) s += red[w2][k];
-        out[k] = s;
-    }
-}
-
-// y = A_s x for one row, 3 components (sliced ELL, row-per-lane).
-template <class T>
-__device__ __forceinline__ void spmv_row(unsigned v, const int* __restrict__ slice_ptr, const int* __restrict__ slice_w,
-                                         const uint32_t* __restrict__ ecol, const T* __restrict__ eval,
-                                         const T* __restrict__ diagA, const T* __restrict__ x, T& y0, T& y1, T& y2) {
-    T x0, x1, x2;
-    ld3(x, v, x0, x1, x2);
-    const T d = diagA[v];
-    y0 = d * x0; y1 = d * x1; y2 = d * x2;
-    const unsigned s = v / kSlice, lane = v % kSlice;
-    const size_t base = (size_t)slice_ptr[s] + lane;
-    const int width = slice_w[s];
-    for (int e = 0; e < width; ++e) {
-        const size_t pos = base + (size_t)e * kSlice;
-        const uint32_t u = ecol[pos];
-        const T a = eval[pos];
-        T u0, u1, u2;
-        ld3(x, u, u0, u1, u2);
-        y0 += a * u0; y1 += a * u1; y2 += a * u2;
-    }
-}
-
-// r = b - A x, z = D^-1 r, p = z, and the initial norms.  Also re-initialises
-// b <- c*sn_old for the next outer iteration when reset_b is set, and copies
-// the warm start into x_out (x_out may equal x).
-template <class T>
-__global__ void __launch_bounds__(kTpb) pcg_init_kernel(
-    int nv, T* __restrict__ b, const T* __restrict__ dbc, const T* __restrict__ more_dbc,
-    const T* __restrict__ dbcx, T wdbc, int dbc_override, const T* __restrict__ x, T* __restrict__ x_out,
-    const int* __restrict__ slice_ptr, const int* __restrict__ slice_w, const uint32_t* __restrict__ ecol,
-    const T* __restrict__ eval, const T* __restrict__ diagA, const T* __restrict__ dinv, T* __restrict__ r,
-    T* __restrict__ p, int reset_b, const T* __restrict__ c, const T* __restrict__ sn_old,
-    double* __restrict__ partial, CtlBlock* ctl) {
-    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const bool skip_all = ctl->outer_done != 0;
-    double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // rz[3], zz[3], dd[3]
-    if (v < (unsigned)nv && !skip_all) {
-        T bx, by, bz;
-        if (dbc_override) rhs_at(b, dbc, more_dbc, dbcx, wdbc, v, bx, by, bz);
-        else ld3(b, v, bx, by, bz);
-        T a0, a1, a2;
-        spmv_row(v, slice_ptr, slice_w, ecol, eval, diagA, x, a0, a1, a2);
-        const T di = dinv[v];
-        const T r0 = bx - a0, r1 = by - a1, r2 = bz - a2;
-        const T z0 = di * r0, z1 = di * r1, z2 = di * r2;
-        st3(r, v, r0, r1, r2);
-        st3(p, v, z0, z1, z2);
-        if (x_out != x) {
-            T x0, x1, x2;
-            ld3(x, v, x0, x1, x2);
-            st3(x_out, v, x0, x1, x2);
-        }
-        vals[0] = (double)r0 * z0; vals[1] = (double)r1 * z1; vals[2] = (double)r2 * z2;
-        vals[3] = (double)z0 * z0; vals[4] = (double)z1 * z1; vals[5] = (double)z2 * z2;
-        const double d0 = (double)di * bx, d1 = (double)di * by, d2 = (double)di * bz;
-        vals[6] = d0 * d0; vals[7] = d1 * d1; vals[8] = d2 * d2;
-        if (reset_b) {
-            T ox, oy, oz;
-            ld3(sn_old, v, ox, oy, oz);
-            const T cv = c[v];
-            st3(b, v, cv * ox, cv * oy, cv * oz);
-        }
-    }
-    if (skip_all) return;
-    if (block_partials<9>(vals, partial, ctl)) {
-        double s[9];
-        last_block_sum<9>(partial, s);
-        if (threadIdx.x == 0) {
-            int any = 0;
-            for (int k = 0; k < 3; ++k) {
-                ctl->rz[k] = s[k];
-                const double thr = fmax(ctl->tol2 * s[3 + k], ctl->floor2 * s[6 + k]);
-                ctl->thr[k] = thr;
-                ctl->active[k] = s[3 + k] > thr;
-                any |= ctl->active[k];
-                ctl->beta[k] = 0;
-            }
-            ctl->iters = 0;
-            ctl->done = !any;
-            ctl->solves += 1;
-            ctl->ticket = 0;
-        }
-    }
-}
-
-// q = A p and p.q; last block: alpha = rz / pq for active components.
-template <class T>
-__global__ void __launch_bounds__(kTpb) pcg_spmv_kernel(int nv, const int* __restrict__ slice_ptr,
-                                                        const int* __restrict__ slice_w,
-                                                        const uint32_t* __restrict__ ecol,
-                                                        const T* __restrict__ eval, const T* __restrict__ diagA,
-                                                        const T* __restrict__ p, T* __restrict__ q,
-                                                        double* __restrict__ partial, CtlBlock* ctl) {
-    if (ctl->done || ctl->outer_done) return;
-    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    double vals[3] = {0, 0, 0};
-    if (v < (unsigned)nv) {
-        T a0, a1, a2;
-        spmv_row(v, slice_ptr, slice_w, ecol, eval, diagA, p, a0, a1, a2);
-        st3(q, v, a0, a1, a2);
-        T p0, p1, p2;
-        ld3(p, v, p0, p1, p2);
-        vals[0] = (double)p0 * a0; vals[1] = (double)p1 * a1; vals[2] = (double)p2 * a2;
-    }
-    if (block_partials<3>(vals, partial, ctl)) {
-        double s[3];
-        last_block_sum<3>(partial, s);
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < 3; ++k) ctl->alpha[k] = (ctl->active[k] && s[k] != 0.0) ? ctl->rz[k] / s[k] : 0.0;
-            ctl->ticket = 0;
-        }
-    }
-}
-
-// x += alpha p, r -= alpha q, z = D^-1 r, r.z, z.z; last block: beta, stop test.
-template <class T>
-__global__ void __launch_bounds__(kTpb) pcg_update_kernel(int nv, T* __restrict__ x, T* __restrict__ r,
-                                                          const T* __restrict__ p, const T* __restrict__ q,
-                                                          const T* __restrict__ dinv, double* __restrict__ partial,
-                                                          CtlBlock* ctl) {
-    if (ctl->done || ctl->outer_done) return;
-    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const T al0 = (T)ctl->alpha[0], al1 = (T)ctl->alpha[1], al2 = (T)ctl->alpha[2];
-    double vals[6] = {0, 0, 0, 0, 0, 0};
-    if (v < (unsigned)nv) {
-        T x0, x1, x2, r0, r1, r2, p0, p1, p2, q0, q1, q2;
-        ld3(x, v, x0, x1, x2);
-        ld3(r, v, r0, r1, r2);
-        ld3(p, v, p0, p1, p2);
-        ld3(q, v, q0, q1, q2);
-        x0 += al0 * p0; x1 += al1 * p1; x2 += al2 * p2;
-        r0 -= al0 * q0; r1 -= al1 * q1; r2 -= al2 * q2;
-        st3(x, v, x0, x1, x2);
-        st3(r, v, r0, r1, r2);
-        const T di = dinv[v];
-        const T z0 = di * r0, z1 = di * r1, z2 = di * r2;
-        vals[0] = (double)r0 * z0; vals[1] = (double)r1 * z1; vals[2] = (double)r2 * z2;
-        vals[3] = (double)z0 * z0; vals[4] = (double)z1 * z1; vals[5] = (double)z2 * z2;
-    }
-    if (block_partials<6>(vals, partial, ctl)) {
-        double s[6];
-        last_block_sum<6>(partial, s);
-        if (threadIdx.x == 0) {
-            int any = 0;
-            for (int k = 0; k < 3; ++k) {
-                const bool act = ctl->active[k] != 0;
-                ctl->beta[k] = (act && ctl->rz[k] != 0.0) ? s[k] / ctl->rz[k] : 0.0;
-                ctl->rz[k] = s[k];
-                ctl->active[k] = act && (s[3 + k] > ctl->thr[k]);
-                any |= ctl->active[k];
-            }
-            ctl->iters += 1;
-            ctl->total_iters += 1;
-            ctl->done = (!any) || (ctl->iters >= ctl->max_iter);
-            if (ctl->iters > ctl->max_used) ctl->max_used = ctl->iters;
-            if (!ctl->done && ctl->iters >= ctl->launch_cap) ctl->cap_hits += 1;
-            ctl->ticket = 0;
-        }
-    }
-}
-
-// p = D^-1 r + beta p.
-template <class T>
-__global__ void pcg_direction_kernel(int nv, const T* __restrict__ r, const T* __restrict__ dinv, T* __restrict__ p,
-                                     CtlBlock* ctl) {
-    if (ctl->done || ctl->outer_done) return;
-    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (v >= (unsigned)nv) return;
-    const T b0 = (T)ctl->beta[0], b1 = (T)ctl->beta[1], b2 = (T)ctl->beta[2];
-    T r0, r1, r2, p0, p1, p2;
-    ld3(r, v, r0, r1, r2);
-    ld3(p, v, p0, p1, p2);
-    const T di = dinv[v];
-    st3(p, v, di * r0 + b0 * p0, di * r1 + b1 * p1, di * r2 + b2 * p2);
-}
-
-// computeError (pdSolver.cu:243-253) + prev <- sn, and the early-exit flag of
-// the local/global loop (pdSolver.cu:164): stop once sqrt(mean (prev-sn)^2) < tol.
+// computeError (pdSolver.cu:243-253) + prev <- sn and the early exit of the
+// local/global loop (pdSolver.cu:164): stop once sqrt(mean (prev-sn)^2) < tol.
+// The last block to finish (ticket) turns the sum into the decision.
 template <class T>
 __global__ void __launch_bounds__(kTpb) outer_error_kernel(int nv, const T* __restrict__ sn, T* __restrict__ prev,
-                                                           double* __restrict__ partial, CtlBlock* ctl) {
+                                                           double* __restrict__ partial, CtlBlock* ctl, double tol) {
     if (ctl->outer_done) return;
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    double vals[1] = {0};
+    double e = 0;
     if (v < (unsigned)nv) {
         T s0, s1, s2, p0, p1, p2;
         ld3(sn, v, s0, s1, s2);
         ld3(prev, v, p0, p1, p2);
         const T d0 = p0 - s0, d1 = p1 - s1, d2 = p2 - s2;
-        vals[0] = (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2);
+        e = (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2);
         st3(prev, v, s0, s1, s2);
     }
-    if (block_partials<1>(vals, partial, ctl)) {
-        double s[1];
-        last_block_sum<1>(partial, s);
+    __shared__ double red[kTpb / 64];
+    __shared__ bool last;
+    const double s = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kTpb / 64; ++w) t += red[w];
+        partial[blockIdx.x] = t;
+        __threadfence();
+        last = atomicAdd(&ctl->ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double t = 0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += kTpb)
+            t += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double s2 = wave_sum(t);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
+        __syncthreads();
         if (threadIdx.x == 0) {
-            const double err = s[0] / (3.0 * (double)nv);
+            double tot = 0;
+            for (int w = 0; w < kTpb / 64; ++w) tot += red[w];
+            const double err = tot / (3.0 * (double)nv);
             ctl->err = err;
-            ctl->outer_done = !(sqrt(err) >= ctl->outer_tol);
+            ctl->outer_done = !(sqrt(err) >= tol);
             ctl->ticket = 0;
         }
     }
 }
 
-// b <- c*sn_old (addM_h2Sn) for the direct-solve modes.
+// ---------------------------------------------------------------------------
+// Persistent Jacobi-PCG over the scalar matrix with 3 fused right-hand sides
+// (one cooperative launch per global solve; the stop test runs on the GPU).
+//
+// Each block owns a contiguous range of rows (XCD-aware placement keeps the
+// neighbours it gathers in the same L2).  Per iteration k:
+//   A: p = D^-1 r + beta p_old (own rows; neighbours' p recomputed on the fly
+//      from r and p_old), q = A p, p.q             -> grid barrier -> alpha
+//   B: x += alpha p, r -= alpha q, z = D^-1 r, r.z, z.z -> grid barrier -> beta, stop
+// Dot products are reduced per block (wave shuffles + LDS) and added into
+// double accumulators; a component stops when ||z|| <= max(tol ||z_0||,
+// floor ||D^-1 b||).  The launch ends with the Chebyshev recurrence
+// (finish = 1) or a copy into sn with computeError (finish = 2).
+template <class T> struct PcgArgs {
+    int nv;
+    int rows_per_block;
+    int gathered;             // 1: rhs from the local-step partials (+ pins), 0: plain b
+    RhsGather<T> g;
+    const T* b;
+    const T* x0;              // warm start (null = 0)
+    T* x;                     // solution
+    const int* slice_ptr;
+    const int* slice_w;
+    const uint32_t* ecol;
+    const T* eval;
+    const T* diagA;
+    const T* dinv;
+    T* r;
+    T* p0;
+    T* p1;
+    T* q;
+    int finish;               // 0: none, 1: Chebyshev into sn/prev, 2: sn = x + computeError
+    T* sn;
+    T* prev;
+    T omega;
+    double outer_tol;
+    double* acc;              // this launch's accumulator bank
+    double* acc_other;        // the other bank (zeroed here for the next launch)
+    CtlBlock* ctl;
+    int max_iter;
+    double tol2;
+    double floor2;
+    const int* skip;          // outer loop already converged
+};
+
 template <class T>
-__global__ void rhs_reset_kernel(int nv, T* __restrict__ b, const T* __restrict__ c, const T* __restrict__ sn_old,
-                                 const int* __restrict__ skip) {
-    if (skip && *skip) return;
-    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (v >= (unsigned)nv) return;
-    T ox, oy, oz;
-    ld3(sn_old, v, ox, oy, oz);
-    const T cv = c[v];
-    st3(b, v, cv * ox, cv * oy, cv * oz);
+__device__ __forceinline__ void spmv_gather_row(const PcgArgs<T>& a, unsigned v, const T* __restrict__ xs, T& y0, T& y1,
+                                                T& y2) {
+    T x0, x1, x2;
+    ld3(xs, v, x0, x1, x2);
+    const T d = a.diagA[v];
+    y0 = d * x0; y1 = d * x1; y2 = d * x2;
+    const unsigned s = v / kSlice, lane = v % kSlice;
+    const size_t base = (size_t)a.slice_ptr[s] + lane;
+    const int width = a.slice_w[s];
+    for (int e = 0; e < width; ++e) {
+        const size_t pos = base + (size_t)e * kSlice;
+        const uint32_t u = a.ecol[pos];
+        const T m = a.eval[pos];
+        T u0, u1, u2;
+        ld3(xs, u, u0, u1, u2);
+        y0 += m * u0; y1 += m * u1; y2 += m * u2;
+    }
 }
 
-// b_eff = b with the pinned overwrite applied (for the direct solve).
 template <class T>
-__global__ void rhs_pinned_kernel(int nv, const T* __restrict__ b, const T* __restrict__ dbc,
-                                  const T* __restrict__ more_dbc, const T* __restrict__ dbcx, T wdbc,
-                                  T* __restrict__ out, const int* __restrict__ skip) {
-    if (skip && *skip) return;
-    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (v >= (unsigned)nv) return;
-    T x, y, z;
-    rhs_at(b, dbc, more_dbc, dbcx, wdbc, v, x, y, z);
-    st3(out, v, x, y, z);
-}
+__global__ void __launch_bounds__(kPcgTpb) pcg_persistent_kernel(PcgArgs<T> a) {
+    if (a.skip && *a.skip) return;   // uniform: read once by every block before any barrier
+    cg::grid_group grid = cg::this_grid();
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = (int)lb * a.rows_per_block;
+    const int r1 = min(a.nv, r0 + a.rows_per_block);
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < kAccBank; i += kPcgTpb) acc_store(&a.acc_other[i], 0.0);
 
-template <class T> __global__ void copy_kernel(size_t n, const T* __restrict__ src, T* __restrict__ dst) {
-    const size_t i = (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    // ---- init: r = b - A x0, p_old = 0, norms ----
+    {
+        double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
+            T b0, b1, b2;
+            if (a.gathered) a.g.at(v, b0, b1, b2);
+            else ld3(a.b, v, b0, b1, b2);
+            T ax0 = 0, ax1 = 0, ax2 = 0, x0 = 0, x1 = 0, x2 = 0;
+            if (a.x0) {
+                spmv_gather_row(a, v, a.x0, ax0, ax1, ax2);
+                ld3(a.x0, v, x0, x1, x2);
+            }
+            st3(a.x, v, x0, x1, x2);
+            const T rr0 = b0 - ax0, rr1 = b1 - ax1, rr2 = b2 - ax2;
+            st3(a.r, v, rr0, rr1, rr2);
+            st3(a.p0, v, (T)0, (T)0, (T)0);
+            const T di = a.dinv[v];
+            const T z0 = di * rr0, z1 = di * rr1, z2 = di * rr2;
+            vals[0] += (double)rr0 * z0; vals[1] += (double)rr1 * z1; vals[2] += (double)rr2 * z2;
+            vals[3] += (double)z0 * z0; vals[4] += (double)z1 * z1; vals[5] += (double)z2 * z2;
+            const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
+            vals[6] += d0 * d0; vals[7] += d1 * d1; vals[8] += d2 * d2;
+        }
+        block_accumulate<9, kPcgTpb>(vals, a.acc + kAccInit);
+    }
+    grid_barrier(grid);
+    double rz[3], thr[3], beta[3] = {0, 0, 0};
+    bool active[3];
+    for (int k = 0; k < 3; ++k) {
+        rz[k] = acc_load(&a.acc[kAccInit + k]);
+        const double zz = acc_load(&a.acc[kAccInit + 3 + k]);
+        const double dd = acc_load(&a.acc[kAccInit + 6 + k]);
+        thr[k] = fmax(a.tol2 * zz, a.floor2 * dd);
+        active[k] = zz > thr[k];
+    }
+    int it = 0;
+    while ((active[0] || active[1] || active[2]) && it < a.max_iter) {
+        const int par = it & 1;
+        const T* pin = par ? a.p1 : a.p0;
+        T* pout = par ? a.p0 : a.p1;
+        const T be0 = (T)beta[0], be1 = (T)beta[1], be2 = (T)beta[2];
+        // ---- phase A: p, q = A p, p.q ----
+        {
+            double vals[3] = {0, 0, 0};
+            for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
+                T rv0, rv1, rv2, pv0, pv1, pv2;
+                ld3(a.r, v, rv0, rv1, rv2);
+                ld3(pin, v, pv0, pv1, pv2);
+                const T dv = a.dinv[v];
+                const T p0 = dv * rv0 + be0 * pv0, p1 = dv * rv1 + be1 * pv1, p2 = dv * rv2 + be2 * pv2;
+                st3(pout, v, p0, p1, p2);
+                const T d = a.diagA[v];
+                T y0 = d * p0, y1 = d * p1, y2 = d * p2;
+                const unsigned s = (unsigned)v / kSlice, lane = (unsigned)v % kSlice;
+                const size_t base = (size_t)a.slice_ptr[s] + lane;
+                const int width = a.slice_w[s];
+                for (int e = 0; e < width; ++e) {
+                    const size_t pos = base + (size_t)e * kSlice;
+                    const uint32_t u = a.ecol[pos];
+                    const T m = a.eval[pos];
+                    T ru0, ru1, ru2, pu0, pu1, pu2;
+                    ld3(a.r, u, ru0, ru1, ru2);
+                    ld3(pin, u, pu0, pu1, pu2);
+                    const T du = a.dinv[u];
+                    y0 += m * (du * ru0 + be0 * pu0);
+                    y1 += m * (du * ru1 + be1 * pu1);
+                    y2 += m * (du * ru2 + be2 * pu2);
+                }
+                st3(a.q, v, y0, y1, y2);
+                vals[0] += (double)p0 * y0; vals[1] += (double)p1 * y1; vals[2] += (double)p2 * y2;
+            }
+            block_accumulate<3, kPcgTpb>(vals, a.acc + kAccA + 4 * par);
+        }
+        grid_barrier(grid);
+        if (blockIdx.x == 0 && it > 0)  // B slot of the previous iteration: everyone has read it
+            for (int i = threadIdx.x; i < 8; i += kPcgTpb) acc_store(&a.acc[kAccB + 8 * (1 - par) + i], 0.0);
+        T al[3];
+        for (int k = 0; k < 3; ++k) {
+            const double pq = acc_load(&a.acc[kAccA + 4 * par + k]);
+            al[k] = (T)((active[k] && pq != 0.0) ? rz[k] / pq : 0.0);
+        }
+        // ---- phase B: x, r, z and their norms ----
+        {
+            double vals[6] = {0, 0, 0, 0, 0, 0};
+            for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
+                T x0, x1, x2, rv0, rv1, rv2, p0, p1, p2, q0, q1, q2;
+                ld3(a.x, v, x0, x1, x2);
+                ld3(a.r, v, rv0, rv1, rv2);
+                ld3(pout, v, p0, p1, p2);
+                ld3(a.q, v, q0, q1, q2);
+                x0 += al[0] * p0; x1 += al[1] * p1; x2 += al[2] * p2;
+                rv0 -= al[0] * q0; rv1 -= al[1] * q1; rv2 -= al[2] * q2;
+                st3(a.x, v, x0, x1, x2);
+                st3(a.r, v, rv0, rv1, rv2);
+                const T dv = a.dinv[v];
+                const T z0 = dv * rv0, z1 = dv * rv1, z2 = dv * rv2;
+                vals[0] += (double)rv0 * z0; vals[1] += (double)rv1 * z1; vals[2] += (double)rv2 * z2;
+                vals[3] += (double)z0 * z0; vals[4] += (double)z1 * z1; vals[5] += (double)z2 * z2;
+            }
+            block_accumulate<6, kPcgTpb>(vals, a.acc + kAccB + 8 * par);
+        }
+        grid_barrier(grid);
+        if (blockIdx.x == 0)  // A slot of this iteration: everyone has read it
+            for (int i = threadIdx.x; i < 4; i += kPcgTpb) acc_store(&a.acc[kAccA + 4 * par + i], 0.0);
+        for (int k = 0; k < 3; ++k) {
+            const double rzn = acc_load(&a.acc[kAccB + 8 * par + k]);
+            const double zz = acc_load(&a.acc[kAccB + 8 * par + 3 + k]);
+            beta[k] = (active[k] && rz[k] != 0.0) ? rzn / rz[k] : 0.0;
+            rz[k] = rzn;
+            active[k] = active[k] && zz > thr[k];
+        }
+        ++it;
+    }
+    // ---- finish ----
+    if (a.finish == 1) {
+        for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
+            for (int k = 0; k < 3; ++k) {
+                const size_t i = 3 * (size_t)v + k;
+                const T s = a.sn[i];
+                T nx = (T)0.9 * (a.x[i] - s) + s;
+                nx = (nx - a.prev[i]) * a.omega + a.prev[i];
+                a.prev[i] = s;
+                a.sn[i] = nx;
+            }
+        }
+    } else if (a.finish == 2) {
+        double vals[1] = {0};
+        for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
+            T x0, x1, x2, p0, p1, p2;
+            ld3(a.x, v, x0, x1, x2);
+            ld3(a.prev, v, p0, p1, p2);
+            const T d0 = p0 - x0, d1 = p1 - x1, d2 = p2 - x2;
+            vals[0] += (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2);
+            st3(a.sn, v, x0, x1, x2);
+            st3(a.prev, v, x0, x1, x2);
+        }
+        block_accumulate<1, kPcgTpb>(vals, a.acc + kAccErr);
+        grid_barrier(grid);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const double err = acc_load(&a.acc[kAccErr]) / (3.0 * (double)a.nv);
+            a.ctl->err = err;
+            a.ctl->outer_done = !(sqrt(err) >= a.outer_tol);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CtlBlock* c = a.ctl;
+        c->pcg_iters += it;
+        c->solves += 1;
+        c->last_iters = it;
+        if (it > c->max_used) c->max_used = it;
+        if (active[0] || active[1] || active[2]) c->unconverged += 1;
+    }
 }
 
 }  // namespace
+
+const char* kernel_name(int id) {
+    static const char* names[kKCount] = {"local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve", "rhs_gather"};
+    return (id >= 0 && id < kKCount) ? names[id] : "?";
+}
 
 // ---------------------------------------------------------------------------
 template <class T> PdEngine<T>::PdEngine(int threads_per_block, const PdData<T>& data) : tpb_(threads_per_block) {
@@ -667,23 +729,30 @@ template <class T> PdEngine<T>::PdEngine(int threads_per_block, const PdData<T>&
     host_tets_.resize(4 * (size_t)nt_);
     SBS_HIP(hipMemcpy(host_tets_.data(), data.Tet, host_tets_.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t v : host_tets_) SBS_REQUIRE(v < (uint32_t)nv_, "PdEngine: tet index out of range");
+    std::vector<T> X0t(3 * (size_t)nv_);
+    SBS_HIP(hipMemcpy(X0t.data(), data.X0, X0t.size() * sizeof(T), hipMemcpyDeviceToHost));
+    std::vector<double> X0h(X0t.begin(), X0t.end());
+    build_chunks(X0h);
     dminv_.alloc(9 * (size_t)nt_);
     w_.alloc(nt_);
     v0_.alloc(nt_);
     // FEMSolver ctor: Dm^-1, V0 from the rest positions (femSolver.cu:7-17).
     hipLaunchKernelGGL(fem_precompute_kernel<T>, dim3(ceil_div(nt_, kTpb)), dim3(kTpb), 0, stream_, nt_, data.X0,
-                       (const uint4*)data.Tet, data.mu, dminv_.get(), v0_.get(), w_.get(), data.V0);
+                       (const uint4*)data.Tet, tet_perm_d_.get(), data.mu, dminv_.get(), v0_.get(), w_.get(),
+                       data.V0);
     SBS_HIP(hipGetLastError());
-    SBS_HIP(hipHostMalloc((void**)&host_counters_, 8 * sizeof(int), hipHostMallocDefault));
-    std::memset(host_counters_, 0, 8 * sizeof(int));
-    SBS_HIP(hipEventCreateWithFlags(&step_event_, hipEventDisableTiming));
+    SBS_HIP(hipHostMalloc((void**)&host_counters_, 16 * sizeof(int), hipHostMallocDefault));
+    std::memset(host_counters_, 0, 16 * sizeof(int));
+    for (auto& e : step_event_) SBS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    SBS_HIP(hipStreamSynchronize(stream_));
 }
 
 template <class T> PdEngine<T>::~PdEngine() {
-    if (step_event_) {
-        (void)hipEventSynchronize(step_event_);
-        (void)hipEventDestroy(step_event_);
-    }
+    for (auto& e : step_event_)
+        if (e) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
     if (host_counters_) (void)hipHostFree(host_counters_);
     for (auto& pool : tev_)
         for (auto& e : pool) {
@@ -697,11 +766,92 @@ template <class T> void PdEngine<T>::reset() {
     for (auto& e : perf_data_) e.ms = 0.0;
 }
 
+// Morton order of tets by rest centroid, chunks of kChunkTets, per-chunk
+// sorted vertex lists, u16 local indices and the vertex -> entry lists.
+template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int v = 0; v < nv_; ++v)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], X0h[3 * (size_t)v + k]);
+            hi[k] = std::max(hi[k], X0h[3 * (size_t)v + k]);
+        }
+    auto spread = [](uint32_t x) {  // 10 bits -> every third bit
+        x = (x * 0x00010001u) & 0xFF0000FFu;
+        x = (x * 0x00000101u) & 0x0F00F00Fu;
+        x = (x * 0x00000011u) & 0xC30C30C3u;
+        x = (x * 0x00000005u) & 0x49249249u;
+        return x;
+    };
+    std::vector<uint64_t> key(nt_);
+    for (int t = 0; t < nt_; ++t) {
+        uint32_t cell[3];
+        for (int k = 0; k < 3; ++k) {
+            double cen = 0;
+            for (int j = 0; j < 4; ++j) cen += X0h[3 * (size_t)host_tets_[4 * (size_t)t + j] + k];
+            cen *= 0.25;
+            const double ext = hi[k] - lo[k];
+            double u = ext > 0 ? (cen - lo[k]) / ext : 0.0;
+            u = std::min(std::max(u * 1024.0, 0.0), 1023.0);
+            cell[k] = (uint32_t)u;
+        }
+        const uint32_t code = (spread(cell[0]) << 2) | (spread(cell[1]) << 1) | spread(cell[2]);
+        key[t] = ((uint64_t)code << 32) | (uint32_t)t;
+    }
+    std::sort(key.begin(), key.end());
+    tet_perm_.resize(nt_);
+    for (int i = 0; i < nt_; ++i) tet_perm_[i] = (int)(key[i] & 0xffffffffu);
+    nchunks_ = (nt_ + kChunkTets - 1) / kChunkTets;
+    std::vector<int> cvptr(nchunks_ + 1, 0);
+    std::vector<uint32_t> cverts;
+    cverts.reserve((size_t)nt_ * 2);
+    std::vector<uint2> lidx(nt_);
+    std::vector<uint32_t> tmp;
+    chunk_max_verts_ = 0;
+    for (int c = 0; c < nchunks_; ++c) {
+        const int t0 = c * kChunkTets, t1 = std::min(nt_, t0 + kChunkTets);
+        tmp.clear();
+        for (int i = t0; i < t1; ++i)
+            for (int j = 0; j < 4; ++j) tmp.push_back(host_tets_[4 * (size_t)tet_perm_[i] + j]);
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        chunk_max_verts_ = std::max(chunk_max_verts_, (int)tmp.size());
+        for (int i = t0; i < t1; ++i) {
+            uint32_t l[4];
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t g = host_tets_[4 * (size_t)tet_perm_[i] + j];
+                l[j] = (uint32_t)(std::lower_bound(tmp.begin(), tmp.end(), g) - tmp.begin());
+            }
+            lidx[i] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+        }
+        cverts.insert(cverts.end(), tmp.begin(), tmp.end());
+        cvptr[c + 1] = (int)cverts.size();
+    }
+    nentries_ = cverts.size();
+    // vertex -> entries
+    std::vector<int> vptr(nv_ + 1, 0);
+    for (uint32_t g : cverts) vptr[g + 1]++;
+    for (int v = 0; v < nv_; ++v) vptr[v + 1] += vptr[v];
+    std::vector<uint32_t> vent(nentries_);
+    std::vector<int> fill(vptr.begin(), vptr.end() - 1);
+    for (size_t e = 0; e < nentries_; ++e) vent[fill[cverts[e]]++] = (uint32_t)e;
+    tet_perm_d_.alloc(nt_);
+    tet_lidx_.alloc(nt_);
+    chunk_vptr_.alloc(nchunks_ + 1);
+    chunk_verts_.alloc(nentries_);
+    vent_ptr_.alloc(nv_ + 1);
+    vent_.alloc(nentries_);
+    partial_f_.alloc(3 * nentries_);
+    SBS_HIP(hipMemcpy(tet_perm_d_.get(), tet_perm_.data(), nt_ * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(tet_lidx_.get(), lidx.data(), nt_ * sizeof(uint2), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(chunk_vptr_.get(), cvptr.data(), cvptr.size() * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(chunk_verts_.get(), cverts.data(), nentries_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(vent_ptr_.get(), vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(vent_.get(), vent.data(), nentries_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+}
+
 // Sliced-ELL structure of the off-diagonal scalar pattern, built on the host
 // from the tet list (one-time, like the reference's host-side prepare).
 template <class T> void PdEngine<T>::build_structure() {
-    SBS_REQUIRE(host_tets_.size() == 4 * (size_t)nt_, "PdEngine: tet list not loaded");
-    // edge pairs (both directions), sorted and unique
     std::vector<uint64_t> keys;
     keys.reserve(12 * (size_t)nt_);
     for (size_t t = 0; t < (size_t)nt_; ++t) {
@@ -740,12 +890,11 @@ template <class T> void PdEngine<T>::build_structure() {
     rowlen_.alloc(nv_);
     ecol_.alloc(std::max<size_t>(1, ecol.size()));
     eval_.alloc(std::max<size_t>(1, ecol.size()));
-    SBS_HIP(hipMemcpyAsync(slice_ptr_.get(), sptr.data(), sptr.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
-    SBS_HIP(hipMemcpyAsync(slice_w_.get(), sw.data(), sw.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
-    SBS_HIP(hipMemcpyAsync(rowlen_.get(), rowlen.data(), rowlen.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    SBS_HIP(hipMemcpy(slice_ptr_.get(), sptr.data(), sptr.size() * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(slice_w_.get(), sw.data(), sw.size() * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(rowlen_.get(), rowlen.data(), rowlen.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!ecol.empty())
-        SBS_HIP(hipMemcpyAsync(ecol_.get(), ecol.data(), ecol.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
-    SBS_HIP(hipStreamSynchronize(stream_));
+        SBS_HIP(hipMemcpy(ecol_.get(), ecol.data(), ecol.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 }
 
 template <class T> void PdEngine<T>::assemble(const PdParams& params) {
@@ -753,8 +902,8 @@ template <class T> void PdEngine<T>::assemble(const PdParams& params) {
     SBS_HIP(hipMemsetAsync(eval_.get(), 0, eval_.bytes(), stream_));
     SBS_HIP(hipMemsetAsync(diagL_.get(), 0, diagL_.bytes(), stream_));
     hipLaunchKernelGGL(assemble_kernel<T>, dim3(ceil_div(nt_, kTpb)), dim3(kTpb), 0, stream_, nt_,
-                       (const uint4*)data_.Tet, dminv_.get(), w_.get(), slice_ptr_.get(), rowlen_.get(), ecol_.get(),
-                       eval_.get(), diagL_.get());
+                       (const uint4*)data_.Tet, tet_perm_d_.get(), dminv_.get(), w_.get(), slice_ptr_.get(),
+                       rowlen_.get(), ecol_.get(), eval_.get(), diagL_.get());
     SBS_HIP(hipGetLastError());
     hipLaunchKernelGGL(mass_diag_kernel<T>, dim3(ceil_div(nv_, kTpb)), dim3(kTpb), 0, stream_, nv_, data_.mass,
                        data_.DBC, (T)1e6, dt2, diagL_.get(), c_.get(), diagA_.get(), dinv_.get());
@@ -766,16 +915,26 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
     SBS_REQUIRE(params.dt > 0, "PdSolver: dt must be positive");
     const size_t n3 = 3 * (size_t)nv_;
     c_.alloc(nv_); diagL_.alloc(nv_); diagA_.alloc(nv_); dinv_.alloc(nv_);
-    sn_.alloc(n3); sn_old_.alloc(n3); prev_.alloc(n3); next_.alloc(n3); b_.alloc(n3);
-    r_.alloc(n3); z_.alloc(n3); p_.alloc(n3); q_.alloc(n3);
-    red_blocks_ = (int)ceil_div(nv_, kTpb);
-    partial_.alloc((size_t)red_blocks_ * 9 + 16);
+    sn_.alloc(n3); sn_old_.alloc(n3); prev_.alloc(n3); next_.alloc(n3); rhs_.alloc(n3);
+    r_.alloc(n3); p0_.alloc(n3); p1_.alloc(n3); q_.alloc(n3);
+    acc_.alloc(2 * kAccBank);
     ctl_.alloc(sizeof(CtlBlock));
-    SBS_HIP(hipMemsetAsync(ctl_.get(), 0, sizeof(CtlBlock), stream_));
+    red_partial_.alloc(ceil_div(nv_, kTpb) + 16);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream_, acc_.get(), 2 * kAccBank);
+    hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, reinterpret_cast<CtlBlock*>(ctl_.get()));
+    // persistent PCG grid: every block co-resident (cooperative launch)
+    int dev = 0, cus = 0, per_cu = 0, coop = 0;
+    SBS_HIP(hipGetDevice(&dev));
+    SBS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SBS_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    SBS_REQUIRE(coop, "device does not support cooperative launch (needed by the PCG global solve)");
+    SBS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_persistent_kernel<T>, kPcgTpb, 0));
+    SBS_REQUIRE(per_cu > 0, "PCG kernel cannot be resident");
+    const int want = (int)ceil_div(nv_, kPcgTpb);
+    pcg_grid_ = std::max(1, std::min(want, cus * std::min(per_cu, 2)));
     build_structure();
     assemble(params);
     system_built_ = true;
-    prepared_dt_ = params.dt;
     // DBCX <- X0 (pdSolver.cu:134)
     SBS_HIP(hipMemcpyAsync(data.DBCX, data.X0, n3 * sizeof(T), hipMemcpyDeviceToDevice, stream_));
     direct_.reset();
@@ -793,78 +952,77 @@ template <class T> void PdEngine<T>::direct_factor() {
 
 template <class T> void PdEngine<T>::direct_solve(const T* b, T* x) {
     SBS_REQUIRE(direct_ != nullptr, "direct solve requested before factorisation");
-    direct_->solve(b, x, stream_, (const int*)&reinterpret_cast<CtlBlock*>(ctl_.get())->outer_done);
+    direct_->solve(b, x, stream_, &reinterpret_cast<CtlBlock*>(ctl_.get())->outer_done);
 }
 
-template <class T>
-void PdEngine<T>::pcg_iterations(T* x, int count) {
-    CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
-    const unsigned nb = (unsigned)red_blocks_;
-    for (int it = 0; it < count; ++it) {
-        tick(kKPcgSpmv, true);
-        hipLaunchKernelGGL(pcg_spmv_kernel<T>, dim3(nb), dim3(kTpb), 0, stream_, nv_, slice_ptr_.get(),
-                           slice_w_.get(), ecol_.get(), eval_.get(), diagA_.get(), p_.get(), q_.get(), partial_.get(),
-                           ctl);
-        tick(kKPcgSpmv, false);
-        tick(kKPcgUpdate, true);
-        hipLaunchKernelGGL(pcg_update_kernel<T>, dim3(nb), dim3(kTpb), 0, stream_, nv_, x, r_.get(), p_.get(),
-                           q_.get(), dinv_.get(), partial_.get(), ctl);
-        tick(kKPcgUpdate, false);
-        tick(kKPcgDirection, true);
-        hipLaunchKernelGGL(pcg_direction_kernel<T>, dim3(nb), dim3(kTpb), 0, stream_, nv_, r_.get(), dinv_.get(),
-                           p_.get(), ctl);
-        tick(kKPcgDirection, false);
-    }
+template <class T> void PdEngine<T>::launch_local(const T* q, bool residual_form, const int* skip) {
+    const size_t lds = 6 * (size_t)chunk_max_verts_ * sizeof(T);
+    SBS_REQUIRE(lds <= 64 * 1024, "local step: chunk vertex list exceeds LDS");
+    tick(kKLocal, true);
+    hipLaunchKernelGGL(local_chunk_kernel<T>, dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_, tet_lidx_.get(),
+                       chunk_vptr_.get(), chunk_verts_.get(), dminv_.get(), w_.get(), q, partial_f_.get(),
+                       (int)residual_form, skip);
+    tick(kKLocal, false);
     SBS_HIP(hipGetLastError());
 }
 
-// Launch policy of the PCG loop.  Kernels of a converged solve return at
-// once, but each launch still costs a dispatch, so:
-//   exact    -- launch chunks of kPcgChunk iterations and read the device
-//               `done` flag between chunks (one host sync per chunk);
-//   adaptive -- launch cap_ iterations with no host sync.  The device counts
-//               solves that ran out of launched iterations (cap_hits); the
-//               host reads the counters of the last completed step and raises
-//               cap_ when needed.  cap_ starts from an exact first step.
 template <class T>
-void PdEngine<T>::pcg_solve(const T* b_src, T* x, bool reset_b, const T* sn_old, bool use_guess, const PdParams& params,
-                            bool dbc_override, bool exact, const T* guess_src) {
-    CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
-    const unsigned nb = (unsigned)red_blocks_;
-    const T wdbc = pin_weight<T>(params.dt);
-    if (!use_guess) SBS_HIP(hipMemsetAsync(x, 0, 3 * (size_t)nv_ * sizeof(T), stream_));
-    tick(kKPcgInit, true);
-    hipLaunchKernelGGL(pcg_init_kernel<T>, dim3(nb), dim3(kTpb), 0, stream_, nv_, const_cast<T*>(b_src), data_.DBC,
-                       data_.moreDBC, data_.DBCX, wdbc, (int)dbc_override, guess_src ? guess_src : (const T*)x, x,
-                       slice_ptr_.get(), slice_w_.get(),
-                       ecol_.get(), eval_.get(), diagA_.get(), dinv_.get(), r_.get(), p_.get(), (int)reset_b, c_.get(),
-                       sn_old, partial_.get(), ctl);
-    tick(kKPcgInit, false);
-    SBS_HIP(hipGetLastError());
-    if (!exact) {
-        pcg_iterations(x, std::min(cap_, params.pcg_max_iter));
-        return;
-    }
-    int launched = 0;
-    while (launched < params.pcg_max_iter) {
-        const int n = std::min(kPcgChunk, params.pcg_max_iter - launched);
-        pcg_iterations(x, n);
-        launched += n;
-        SBS_HIP(hipMemcpyAsync(host_counters_ + 4, &ctl->done, sizeof(int), hipMemcpyDeviceToHost, stream_));
-        SBS_HIP(hipStreamSynchronize(stream_));
-        if (host_counters_[4]) break;
-    }
+void PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, int finish, T omega, const PdParams& params,
+                             double outer_tol) {
+    PcgArgs<T> a{};
+    a.nv = nv_;
+    const int rpb = (int)ceil_div(nv_, pcg_grid_);
+    a.rows_per_block = ((rpb + kSlice - 1) / kSlice) * kSlice;
+    a.gathered = gathered;
+    a.g = RhsGather<T>{c_.get(), sn_old_.get(), vent_ptr_.get(), vent_.get(), partial_f_.get(), data_.DBC,
+                       data_.moreDBC, data_.DBCX, pin_weight<T>(params.dt)};
+    a.b = b;
+    a.x0 = x0;
+    a.x = x;
+    a.slice_ptr = slice_ptr_.get();
+    a.slice_w = slice_w_.get();
+    a.ecol = ecol_.get();
+    a.eval = eval_.get();
+    a.diagA = diagA_.get();
+    a.dinv = dinv_.get();
+    a.r = r_.get();
+    a.p0 = p0_.get();
+    a.p1 = p1_.get();
+    a.q = q_.get();
+    a.finish = finish;
+    a.sn = sn_.get();
+    a.prev = prev_.get();
+    a.omega = omega;
+    a.outer_tol = outer_tol;
+    const int bank = pcg_launches_ & 1;
+    a.acc = acc_.get() + bank * kAccBank;
+    a.acc_other = acc_.get() + (1 - bank) * kAccBank;
+    a.ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
+    a.max_iter = params.pcg_max_iter;
+    a.tol2 = params.pcg_tol * params.pcg_tol;
+    a.floor2 = params.pcg_floor * params.pcg_floor;
+    a.skip = finish == 2 ? &a.ctl->outer_done : nullptr;
+    void* args[] = {&a};
+    tick(kKPcg, true);
+    SBS_HIP(hipLaunchCooperativeKernel((const void*)pcg_persistent_kernel<T>, dim3(pcg_grid_), dim3(kPcgTpb), args, 0,
+                                       stream_));
+    tick(kKPcg, false);
+    ++pcg_launches_;
 }
 
-template <class T> void PdEngine<T>::review_counters() {
-    if (!step_pending_) return;
-    if (hipEventQuery(step_event_) != hipSuccess) return;  // previous step still running: keep the current cap
-    step_pending_ = false;
-    const int cap_hits = host_counters_[1], max_used = host_counters_[2];
-    last_pcg_iters_ = host_counters_[0];
-    cap_hits_total_ += cap_hits;
-    if (max_used + kPcgMargin > cap_) cap_ = max_used + kPcgMargin;
-    if (cap_hits > 0) cap_ += std::max(4, cap_ / 2);
+template <class T> void PdEngine<T>::fold_slot(int slot) {
+    if (!step_pending_[slot]) return;
+    SBS_HIP(hipEventSynchronize(step_event_[slot]));
+    step_pending_[slot] = false;
+    last_pcg_iters_ = host_counters_[4 * slot];
+    unconverged_total_ += host_counters_[4 * slot + 1];
+    pcg_iters_total_ += host_counters_[4 * slot];
+    pcg_solves_total_ += host_counters_[4 * slot + 3];
+}
+
+template <class T> void PdEngine<T>::sync_counters() {
+    fold_slot((int)(steps_ & 1));        // older step first
+    fold_slot((int)((steps_ + 1) & 1));
 }
 
 template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& params) {
@@ -874,38 +1032,31 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     SBS_REQUIRE(params.pcg_max_iter >= 1, "PdSolver: pcg_max_iter must be >= 1");
     const bool direct = solver_ == GlobalSolver::CuSolverCholesky || solver_ == GlobalSolver::EigenCholesky;
     if (direct && !direct_) direct_factor();
+    // this step's pinned counter slot was last written two steps ago: fold it in
+    const int slot = (int)(steps_ & 1);
+    fold_slot(slot);
     const T dt = (T)params.dt;
-    const unsigned vb = ceil_div(nv_, kTpb), tb = ceil_div(nt_, kTpb);
-    const T wdbc = pin_weight<T>(params.dt);
+    const unsigned vb = ceil_div(nv_, kTpb);
     CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
     const bool jacobi = solver_ == GlobalSolver::Jacobi;
     const bool pcg = solver_ == GlobalSolver::PCG;
     const bool cheb = jacobi || (pcg && params.pcg_chebyshev);
-    review_counters();
-    const bool exact = pcg && (!params.pcg_adaptive || cap_ <= 0);
-    if (pcg && exact && step_pending_) {  // counters of an in-flight step are needed before an exact step
-        SBS_HIP(hipEventSynchronize(step_event_));
-        review_counters();
-    }
+    const RhsGather<T> g{c_.get(), sn_old_.get(), vent_ptr_.get(), vent_.get(), partial_f_.get(), data.DBC,
+                         data.moreDBC, data.DBCX, pin_weight<T>(params.dt)};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (perf_) for (auto& e : ev) SBS_HIP(hipEventCreate(&e));
 
-    hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl, params.pcg_max_iter,
-                       exact ? params.pcg_max_iter : std::min(cap_, params.pcg_max_iter),
-                       params.pcg_tol * params.pcg_tol, params.pcg_floor * params.pcg_floor, params.tol);
+    hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl);
     hipLaunchKernelGGL(begin_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, data.X, data.V, data.mass,
                        data.DBC, data.moreDBC, dt, (T)params.gravity, c_.get(), sn_.get(), sn_old_.get(), prev_.get(),
-                       b_.get(), (int)cheb);
+                       (int)cheb);
     SBS_HIP(hipGetLastError());
     T omega = 1;
     const T rho = (T)params.rho;
     const int* skip = cheb ? nullptr : &ctl->outer_done;
     for (int i = 0; i < params.num_iterations; ++i) {
         if (perf_) SBS_HIP(hipEventRecord(ev[0], stream_));
-        tick(kKLocal, true);
-        hipLaunchKernelGGL(local_step_kernel<T>, dim3(tb), dim3(kTpb), 0, stream_, nt_, (const uint4*)data.Tet,
-                           dminv_.get(), w_.get(), sn_.get(), b_.get(), (int)jacobi, skip);
-        tick(kKLocal, false);
+        launch_local(sn_.get(), jacobi, skip);
         if (perf_) SBS_HIP(hipEventRecord(ev[1], stream_));
         if (cheb) {
             if (i <= 10) omega = 1;
@@ -914,94 +1065,81 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
         }
         if (jacobi) {
             tick(kKJacobi, true);
-            hipLaunchKernelGGL(jacobi_chebyshev_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, b_.get(), c_.get(),
-                               diagL_.get(), data.DBC, data.moreDBC, data.DBCX, wdbc, sn_.get(), prev_.get(),
-                               sn_old_.get(), omega);
+            hipLaunchKernelGGL(jacobi_chebyshev_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, diagL_.get(),
+                               sn_.get(), prev_.get(), omega);
             tick(kKJacobi, false);
         } else if (pcg) {
-            T* x = cheb ? next_.get() : sn_.get();
-            pcg_solve(b_.get(), x, /*reset_b=*/true, sn_old_.get(), /*use_guess=*/true, params, /*dbc=*/true, exact,
-                      cheb ? sn_.get() : nullptr);
-            if (cheb) {
-                tick(kKChebyshev, true);
-                hipLaunchKernelGGL(chebyshev_kernel<T>, dim3(ceil_div(3 * (size_t)nv_, kTpb)), dim3(kTpb), 0, stream_,
-                                   nv_, (const T*)next_.get(), sn_.get(), prev_.get(), omega, (const int*)nullptr);
-                tick(kKChebyshev, false);
-            } else
-                hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(),
-                                   prev_.get(), partial_.get(), ctl);
+            // PCG warm-started from the current iterate; Chebyshev recurrence
+            // or computeError + early exit at the end of the same launch
+            launch_pcg(1, nullptr, sn_.get(), next_.get(), cheb ? 1 : 2, omega, params, params.tol);
         } else {
-            hipLaunchKernelGGL(rhs_pinned_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)b_.get(),
-                               data.DBC, data.moreDBC, data.DBCX, wdbc, next_.get(), skip);
+            tick(kKRhs, true);
+            hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, rhs_.get(), skip);
+            tick(kKRhs, false);
             tick(kKDirect, true);
-            direct_solve(next_.get(), sn_.get());
+            direct_solve(rhs_.get(), sn_.get());
             tick(kKDirect, false);
             hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(),
-                               prev_.get(), partial_.get(), ctl);
-            hipLaunchKernelGGL(rhs_reset_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, b_.get(), c_.get(),
-                               (const T*)sn_old_.get(), skip);
+                               prev_.get(), red_partial_.get(), ctl, params.tol);
         }
         SBS_HIP(hipGetLastError());
         if (perf_) {
             SBS_HIP(hipEventRecord(ev[2], stream_));
             SBS_HIP(hipEventSynchronize(ev[2]));
-            float a = 0, g = 0;
+            float a = 0, gm = 0;
             SBS_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-            SBS_HIP(hipEventElapsedTime(&g, ev[1], ev[2]));
+            SBS_HIP(hipEventElapsedTime(&gm, ev[1], ev[2]));
             perf_data_[0].ms += a;
-            perf_data_[1].ms += g;
+            perf_data_[1].ms += gm;
         }
     }
     hipLaunchKernelGGL(end_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(), (T)1 / dt,
-                       data.XTilde, data.V, data.X, data.moreDBC, 1);
+                       data.XTilde, data.V, data.X, data.moreDBC);
     SBS_HIP(hipGetLastError());
-    // counters: total PCG iterations, cap hits, max iterations of one solve, solves
-    SBS_HIP(hipMemcpyAsync(host_counters_, &ctl->total_iters, sizeof(int), hipMemcpyDeviceToHost, stream_));
-    SBS_HIP(hipMemcpyAsync(host_counters_ + 1, &ctl->cap_hits, 3 * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    SBS_HIP(hipEventRecord(step_event_, stream_));
-    step_pending_ = true;
+    // counters: PCG iterations of the step, unconverged solves, max per solve, solves
+    SBS_HIP(hipMemcpyAsync(host_counters_ + 4 * slot, &ctl->pcg_iters, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                           stream_));
+    SBS_HIP(hipEventRecord(step_event_[slot], stream_));
+    step_pending_[slot] = true;
+    ++steps_;
     if (perf_) {
         SBS_HIP(hipStreamSynchronize(stream_));
         for (auto& e : ev) (void)hipEventDestroy(e);
-    }
-    if (exact && pcg) {
-        SBS_HIP(hipEventSynchronize(step_event_));
-        review_counters();
     }
     last_iters_ = params.num_iterations;
 }
 
 template <class T> void PdEngine<T>::local_only(const T* q, const T* sn_old, bool residual_form, double dt, T* b_out) {
     SBS_REQUIRE(ready_, "local_only: engine not prepared");
-    const unsigned vb = ceil_div(nv_, kTpb), tb = ceil_div(nt_, kTpb);
-    hipLaunchKernelGGL(rhs_reset_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, b_.get(), c_.get(), sn_old,
-                       (const int*)nullptr);
-    hipLaunchKernelGGL(local_step_kernel<T>, dim3(tb), dim3(kTpb), 0, stream_, nt_, (const uint4*)data_.Tet,
-                       dminv_.get(), w_.get(), q, b_.get(), (int)residual_form, (const int*)nullptr);
-    const T wdbc = pin_weight<T>(dt);
-    hipLaunchKernelGGL(rhs_pinned_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)b_.get(), data_.DBC,
-                       data_.moreDBC, data_.DBCX, wdbc, b_out, (const int*)nullptr);
+    const unsigned vb = ceil_div(nv_, kTpb);
+    launch_local(q, residual_form, nullptr);
+    const RhsGather<T> g{c_.get(), sn_old, vent_ptr_.get(), vent_.get(), partial_f_.get(), data_.DBC, data_.moreDBC,
+                         data_.DBCX, pin_weight<T>(dt)};
+    hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, b_out, (const int*)nullptr);
     SBS_HIP(hipGetLastError());
 }
 
 template <class T> int PdEngine<T>::solve(const T* b, T* x, const PdParams& params, bool use_guess) {
     SBS_REQUIRE(ready_, "solve: engine not prepared");
+    sync_counters();
     CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
-    if (step_pending_) {
-        SBS_HIP(hipEventSynchronize(step_event_));
-        review_counters();
-    }
-    hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl, params.pcg_max_iter, params.pcg_max_iter,
-                       params.pcg_tol * params.pcg_tol, params.pcg_floor * params.pcg_floor, 0.0);
+    hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl);
     if (direct_) {
         direct_solve(b, x);
         SBS_HIP(hipStreamSynchronize(stream_));
         return 0;
     }
-    pcg_solve(b, x, false, nullptr, use_guess, params, false, /*exact=*/true, nullptr);
-    SBS_HIP(hipMemcpyAsync(host_counters_ + 4, &ctl->iters, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    // x0 = x (the caller's guess) or 0; the init phase reads x0 at every row
+    // before the grid barrier that precedes any write to x
+    SBS_REQUIRE(!use_guess || x != nullptr, "solve: guess requested without x");
+    if (use_guess) {
+        SBS_HIP(hipMemcpyAsync(next_.get(), x, 3 * (size_t)nv_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+    }
+    launch_pcg(0, b, use_guess ? next_.get() : nullptr, x, 0, (T)1, params, 0.0);
+    SBS_HIP(hipMemcpyAsync(host_counters_ + 8, &ctl->unconverged, 5 * sizeof(int), hipMemcpyDeviceToHost, stream_));
     SBS_HIP(hipStreamSynchronize(stream_));
-    return host_counters_[4];
+    unconverged_total_ += host_counters_[8];
+    return host_counters_[12];  // last_iters
 }
 
 template <class T> typename PdEngine<T>::Export PdEngine<T>::export_system() const {
@@ -1011,22 +1149,31 @@ template <class T> typename PdEngine<T>::Export PdEngine<T>::export_system() con
         if (n) SBS_HIP(hipMemcpy(dst.data(), src.get(), n * sizeof(T), hipMemcpyDeviceToHost));
     };
     SBS_HIP(hipStreamSynchronize(stream_));
-    d2h(ex.dminv, dminv_, 9 * (size_t)nt_);
-    d2h(ex.V0, v0_, nt_);
-    d2h(ex.w, w_, nt_);
+    std::vector<T> dm, v0, w;
+    d2h(dm, dminv_, 9 * (size_t)nt_);
+    d2h(v0, v0_, nt_);
+    d2h(w, w_, nt_);
+    // internal (Morton) order -> caller order, Dm^-1 row-major per tet
+    ex.V0.resize(nt_);
+    ex.w.resize(nt_);
+    ex.dminv.resize(9 * (size_t)nt_);
+    for (int i = 0; i < nt_; ++i) {
+        const int t = tet_perm_[i];
+        ex.V0[t] = v0[i];
+        ex.w[t] = w[i];
+        for (int k = 0; k < 9; ++k) ex.dminv[9 * (size_t)t + k] = dm[(size_t)k * nt_ + i];
+    }
     if (!system_built_) return ex;
     d2h(ex.c, c_, nv_);
     d2h(ex.diagA, diagA_, nv_);
     d2h(ex.diagL, diagL_, nv_);
-    std::vector<int> sptr(nslices_ + 1), sw(nslices_), rl(nv_);
+    std::vector<int> sptr(nslices_ + 1), rl(nv_);
     SBS_HIP(hipMemcpy(sptr.data(), slice_ptr_.get(), sptr.size() * sizeof(int), hipMemcpyDeviceToHost));
-    SBS_HIP(hipMemcpy(sw.data(), slice_w_.get(), sw.size() * sizeof(int), hipMemcpyDeviceToHost));
     SBS_HIP(hipMemcpy(rl.data(), rowlen_.get(), rl.size() * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<uint32_t> ec(ecol_.n);
-    std::vector<T> ev(eval_.n);
+    std::vector<T> evv(eval_.n);
     SBS_HIP(hipMemcpy(ec.data(), ecol_.get(), ec.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    SBS_HIP(hipMemcpy(ev.data(), eval_.get(), ev.size() * sizeof(T), hipMemcpyDeviceToHost));
-    // full CSR of A_s with sorted columns (diagonal included)
+    SBS_HIP(hipMemcpy(evv.data(), eval_.get(), evv.size() * sizeof(T), hipMemcpyDeviceToHost));
     ex.rowptr.assign(nv_ + 1, 0);
     for (int v = 0; v < nv_; ++v) ex.rowptr[v + 1] = ex.rowptr[v] + rl[v] + 1;
     ex.col.resize(ex.rowptr[nv_]);
@@ -1037,7 +1184,7 @@ template <class T> typename PdEngine<T>::Export PdEngine<T>::export_system() con
         const int s = v / kSlice, lane = v % kSlice;
         for (int e = 0; e < rl[v]; ++e) {
             const size_t pos = (size_t)sptr[s] + (size_t)e * kSlice + lane;
-            row.emplace_back((int)ec[pos], ev[pos]);
+            row.emplace_back((int)ec[pos], evv[pos]);
         }
         std::sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
         for (size_t k = 0; k < row.size(); ++k) {
@@ -1046,12 +1193,6 @@ template <class T> typename PdEngine<T>::Export PdEngine<T>::export_system() con
         }
     }
     return ex;
-}
-
-const char* kernel_name(int id) {
-    static const char* names[kKCount] = {"local_step", "pcg_init", "pcg_spmv", "pcg_update",
-                                         "pcg_direction", "jacobi_chebyshev", "chebyshev", "direct_solve"};
-    return (id >= 0 && id < kKCount) ? names[id] : "?";
 }
 
 template <class T> void PdEngine<T>::set_kernel_timing(unsigned mask) {

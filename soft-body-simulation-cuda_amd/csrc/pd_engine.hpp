@@ -9,13 +9,19 @@
 // Layout in HBM (one body or many bodies concatenated, like the reference):
 //   * vertex vectors are interleaved xyz (3*N_v scalars), the reference's
 //     glm::vec3 AoS layout, so a gather of one neighbour is one 12-byte run;
-//   * tets stay u32[4*N_t] (read as one 16-byte load per tet);
-//   * per-tet constants are SoA: Dm^-1 as 9 planes and w = |V0|*mu as one,
-//     so each plane is read fully coalesced;
+//   * tets are reordered internally along a Morton curve of their rest
+//     centroids and grouped into chunks of kChunkTets; a chunk stores the
+//     sorted list of the vertices it touches, and each tet its 4 vertices as
+//     u16 indices into that list; per-tet constants are SoA in that order
+//     (Dm^-1 as 9 planes, w = |V0| mu as one) so every plane is read coalesced;
+//   * the local step writes one partial force per (chunk, vertex) pair
+//     instead of scattering with global atomics; the consumers of the
+//     right-hand side gather a vertex's partials through a vertex->entry
+//     list and add c*s_n there;
 //   * the system is the scalar matrix A_s = L_s + diag(c) (the reference's
-//     3N x 3N matrix is A_s (x) I_3): its diagonal is kept as one N_v array
-//     and the off-diagonal part in sliced ELL with 64-row slices (one slice
-//     per wavefront, column-major inside the slice).
+//     3N x 3N matrix is A_s (x) I_3): its diagonal is one N_v array and the
+//     off-diagonal part is sliced ELL with 64-row slices (one slice per
+//     wavefront, column-major inside the slice).
 #pragma once
 
 #include <memory>
@@ -44,7 +50,7 @@ struct PdParams {
     double pcg_floor = 1e-8;    // ... or <= pcg_floor ||D^-1 b||
     int pcg_max_iter = 100;
     int pcg_chebyshev = 0;      // apply the Chebyshev recurrence after each PCG solve
-    int pcg_adaptive = 1;       // launch a learned iteration cap instead of syncing per chunk
+    int pcg_adaptive = 1;       // accepted for ABI compatibility; the PCG loop stops on the device
 };
 
 // Caller-owned device arrays: the reference's SolverData<Scalar> (src/def.h:22-56).
@@ -71,13 +77,10 @@ struct PerfEntry {
 // Kernels that can be timed with HIP events on the engine's stream.
 enum KernelId : int {
     kKLocal = 0,
-    kKPcgInit,
-    kKPcgSpmv,
-    kKPcgUpdate,
-    kKPcgDirection,
+    kKPcg,
     kKJacobi,
-    kKChebyshev,
     kKDirect,
+    kKRhs,
     kKCount
 };
 const char* kernel_name(int id);
@@ -86,6 +89,8 @@ struct KernelTime {
     int launches = 0;
     double ms = 0.0;
 };
+
+constexpr int kChunkTets = 256;  // tets per local-step chunk (one workgroup)
 
 template <class T> class PdEngine {
 public:
@@ -102,6 +107,7 @@ public:
 
     // One PdSolver::Update (prepare on first call, then the step).
     void update(PdData<T>& data, const PdParams& params);
+    void prepare(PdData<T>& data, const PdParams& params);
 
     // Local step alone on a given iterate, for kernel-level tests:
     // b = c*sn_old + sum_t w_t P_t Dm^-T G^T, then the pinned overwrite.
@@ -111,9 +117,9 @@ public:
     // matrix (direct when factored, else PCG); exposes the Solve(b, x) call.
     int solve(const T* b, T* x, const PdParams& params, bool use_guess);
 
-    // Prepared system, for tests: c (N_v), diag(A_s) (N_v), sliced-ELL parts.
+    // Prepared system, for tests (caller tet order).
     struct Export {
-        std::vector<T> c, diagA, diagL, V0, w, dminv;
+        std::vector<T> c, diagA, diagL, V0, w, dminv;  // dminv row-major per tet
         std::vector<int> rowptr, col;
         std::vector<T> val;
     };
@@ -121,79 +127,88 @@ public:
 
     int last_iterations() const { return last_iters_; }
     int last_pcg_iterations() const { return last_pcg_iters_; }
-    int pcg_cap() const { return cap_; }
-    // Event pairs around every launch of the kernels in `mask` (bit = KernelId);
-    // kernel_times() waits for the stream and sums the elapsed times.
-    void set_kernel_timing(unsigned mask);
-    KernelTime kernel_time(int id);
-    long long pcg_cap_hits() const { return cap_hits_total_; }
-    // Waits for the last step and folds its counters into the cap statistics.
-    void sync_counters() {
-        if (step_pending_) { SBS_HIP(hipEventSynchronize(step_event_)); review_counters(); }
-    }
     int num_verts() const { return nv_; }
     int num_tets() const { return nt_; }
     size_t nnz_offdiag() const { return nnz_off_; }
     bool prepared() const { return ready_; }
-    void prepare(PdData<T>& data, const PdParams& params);
+    int pcg_cap() const { return pcg_grid_; }
+    long long pcg_cap_hits() const { return unconverged_total_; }
+    long long pcg_iterations_total() const { return pcg_iters_total_; }
+    long long pcg_solves_total() const { return pcg_solves_total_; }
+    // Waits for queued work and folds the last step's counters in.
+    void sync_counters();
+    void set_kernel_timing(unsigned mask);
+    KernelTime kernel_time(int id);
 
 private:
+    void build_chunks(const std::vector<double>& X0h);
     void build_structure();
     void assemble(const PdParams& params);
-    void pcg_solve(const T* b_src, T* x, bool reset_b, const T* sn_old, bool use_guess, const PdParams& params,
-                   bool dbc_override, bool exact, const T* guess_src);
-    void pcg_iterations(T* x, int count);
-    void review_counters();
+    void launch_local(const T* q, bool residual_form, const int* skip);
+    void launch_pcg(int gathered, const T* b, const T* x0, T* x, int finish, T omega, const PdParams& params,
+                    double outer_tol);
     void direct_factor();
     void direct_solve(const T* b, T* x);
+    void tick(int id, bool start);
+    void flush_timing();
 
     int tpb_ = 256;
     int nv_ = 0, nt_ = 0;
     PdData<T> data_{};
     GlobalSolver solver_ = GlobalSolver::Jacobi;
     bool ready_ = false;
-    bool system_built_ = false;   // matrix assembled (set inside prepare)
+    bool system_built_ = false;
     bool perf_ = false;
     hipStream_t stream_ = nullptr;
     std::vector<PerfEntry> perf_data_;
-    double prepared_dt_ = 0.0;
 
-    // per tet
-    DevBuf<T> dminv_;   // 9 planes of N_t
-    DevBuf<T> w_;       // |V0| * mu
-    DevBuf<T> v0_;      // V0
+    // tets in internal (Morton chunk) order
+    std::vector<uint32_t> host_tets_;   // caller order, 4*N_t
+    std::vector<int> tet_perm_;         // internal position -> caller tet index
+    DevBuf<int> tet_perm_d_;
+    DevBuf<uint2> tet_lidx_;            // 4 x u16 chunk-local vertex indices
+    int nchunks_ = 0, chunk_max_verts_ = 0;
+    DevBuf<int> chunk_vptr_;            // nchunks+1: first entry of each chunk
+    DevBuf<uint32_t> chunk_verts_;      // global vertex of each entry
+    DevBuf<int> vent_ptr_;              // N_v+1: entries of each vertex
+    DevBuf<uint32_t> vent_;             // entry ids grouped by vertex
+    size_t nentries_ = 0;
+    DevBuf<T> partial_f_;               // 3 scalars per entry: local-step partial forces
+    DevBuf<T> dminv_;                   // 9 planes of N_t
+    DevBuf<T> w_;                       // |V0| * mu
+    DevBuf<T> v0_;                      // V0
     // per vertex
     DevBuf<T> c_, diagL_, diagA_, dinv_;
-    DevBuf<T> sn_, sn_old_, prev_, next_, b_;
+    DevBuf<T> sn_, sn_old_, prev_, next_, rhs_;
     // matrix: sliced ELL (64-row slices) of the off-diagonal part
     int nslices_ = 0;
     size_t nnz_off_ = 0;
-    DevBuf<int> slice_ptr_;   // nslices+1 (offset of each slice in the ELL arrays)
-    DevBuf<int> slice_w_;     // nslices (width of each slice)
-    DevBuf<int> rowlen_;      // N_v
-    DevBuf<uint32_t> ecol_;   // padded entries
+    DevBuf<int> slice_ptr_, slice_w_, rowlen_;
+    DevBuf<uint32_t> ecol_;
     DevBuf<T> eval_;
-    // PCG workspace
-    DevBuf<T> r_, z_, p_, q_;
-    DevBuf<double> partial_;
+    // PCG workspace and control
+    DevBuf<T> r_, p0_, p1_, q_;
+    DevBuf<double> acc_;                // reduction accumulators (2 banks)
     DevBuf<unsigned char> ctl_;
-    int red_blocks_ = 0;
+    int pcg_grid_ = 0, pcg_launches_ = 0;
+    DevBuf<double> red_partial_;        // per-block partials of the small reductions
     // direct solver (band Cholesky factor, GPU triangular solves) for small bodies
     std::unique_ptr<BandCholesky<T>> direct_;
     int last_iters_ = 0, last_pcg_iters_ = 0;
-    // pinned host mirror of iteration counters: [total, cap_hits, max_used, solves, scratch]
+    // pinned counters of the last two steps (slot = step parity):
+    // [pcg iterations, unconverged solves, max per solve, solves]; [8..15] scratch
     int* host_counters_ = nullptr;
-    hipEvent_t step_event_ = nullptr;
-    bool step_pending_ = false;
-    int cap_ = 0;                 // adaptive PCG launch cap (0 = unknown)
+    hipEvent_t step_event_[2] = {nullptr, nullptr};
+    bool step_pending_[2] = {false, false};
+    long long steps_ = 0;
+    void fold_slot(int slot);
+    long long unconverged_total_ = 0;
+    long long pcg_iters_total_ = 0, pcg_solves_total_ = 0;
+    // kernel timing
     unsigned timing_mask_ = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev_[kKCount];
     size_t tev_used_[kKCount] = {};
     KernelTime ktime_[kKCount];
-    void tick(int id, bool start);
-    void flush_timing();
-    long long cap_hits_total_ = 0;
-    std::vector<uint32_t> host_tets_;
 };
 
 }  // namespace sbs

@@ -20,8 +20,7 @@ SOLVER_CUSOLVER_CHOLESKY = 1
 SOLVER_EIGEN_CHOLESKY = 2
 SOLVER_PCG = 3
 
-KERNELS = ["local_step", "pcg_init", "pcg_spmv", "pcg_update", "pcg_direction", "jacobi_chebyshev", "chebyshev",
-           "direct_solve"]
+KERNELS = ["local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve", "rhs_gather"]
 
 
 class LibraryNotBuilt(RuntimeError):
@@ -51,7 +50,8 @@ class SbsSolverParams(ctypes.Structure):
 
 class SbsPdStats(ctypes.Structure):
     _fields_ = [("pcg_iterations_last_step", ctypes.c_int), ("pcg_cap", ctypes.c_int),
-                ("pcg_cap_hits", ctypes.c_longlong), ("nnz_offdiag", ctypes.c_longlong)]
+                ("pcg_cap_hits", ctypes.c_longlong), ("nnz_offdiag", ctypes.c_longlong),
+                ("pcg_iterations_total", ctypes.c_longlong), ("pcg_solves_total", ctypes.c_longlong)]
 
 
 # Every function the header declares, with its ctypes signature.
