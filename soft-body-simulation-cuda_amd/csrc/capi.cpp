@@ -244,12 +244,7 @@ int sbs_pd_export_system(SbsPd* pd, void* c, void* diagA, void* diagL, void* V0,
             };
             put(V0, ex.V0);
             put(w, ex.w);
-            if (dminv) {  // engine planes -> row-major per tet
-                const size_t nt = ex.V0.size();
-                S* o = static_cast<S*>(dminv);
-                for (size_t t = 0; t < nt; ++t)
-                    for (int k = 0; k < 9; ++k) o[9 * t + k] = ex.dminv[(size_t)k * nt + t];
-            }
+            put(dminv, ex.dminv);  // row-major per tet, caller tet order
             if (e.prepared()) {
                 put(c, ex.c);
                 put(diagA, ex.diagA);

@@ -135,7 +135,7 @@ def test_eigen_cholesky_mode_matches_cusolver_mode(gpu, precision):
     p = params_for(SolverType.CuSolverCholesky, n_iter=10, tol=0.0)
     a, _ = run_gpu(s, SolverType.CuSolverCholesky, p, 20, precision)
     b, _ = run_gpu(s, SolverType.EigenCholesky, p, 20, precision)
-    assert rel_err(a["X"], b["X"]) < (1e-6 if precision == 32 else 1e-13)
+    assert rel_err(a["X"], b["X"]) < (1e-5 if precision == 32 else 1e-13)
 
 
 @pytest.mark.parametrize("precision", [32, 64])
