@@ -47,15 +47,18 @@ def local_step_bytes(nt: int, nv: int) -> int:
     return 56 * nt + 40 * nv
 
 
-def pcg_iteration_bytes(nnz_off: int, nv: int) -> int:
-    # one Jacobi-PCG iteration over the scalar matrix, 3 RHS fused: matrix values
-    # + u32 columns, row pointers, D^-1, and 11 passes over 12-byte vertex
-    # vectors (SpMV p -> q: 2, x += a p: 3, r -= a q: 3, p = z + b p: 3)
-    nnz_s = nnz_off + nv
-    return 8 * nnz_s + 4 * (nv + 1) + 4 * nv + 132 * nv
+def pcg_spmv_bytes(nnz_off: int, nv: int) -> int:
+    # one PCG iteration's first kernel: off-diagonal values + u32 columns, the
+    # diagonal and D^-1, read r and p_old, write p and q (12-byte vertex vectors)
+    return 8 * nnz_off + 56 * nv
 
 
-def run_gpu(scene, steps: int, warmup: int, precision: int, time_kernels: bool):
+def pcg_update_bytes(nv: int) -> int:
+    # second kernel: read x, r, p, q and D^-1, write x and r
+    return 76 * nv
+
+
+def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int):
     data = SolverData(scene, precision)
     pd = PdSolver(128, data)
     pd.SetGlobalSolver(SolverType.PCG)
@@ -63,26 +66,31 @@ def run_gpu(scene, steps: int, warmup: int, precision: int, time_kernels: bool):
     for _ in range(warmup):
         pd.Update(data, p)
     pd.synchronize()
-    stats_w = pd.stats()
-    if time_kernels:
-        pd.set_kernel_timing(["local_step", "pcg_solve"])
     st0 = pd.stats()
-    pd.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         pd.Update(data, p)
     pd.synchronize()
     t1 = time.perf_counter()
-    kt = {}
-    if time_kernels:
-        for k in ("local_step", "pcg_solve"):
-            kt[k] = pd.kernel_time(k)
-    st = pd.stats()
+    st1 = pd.stats()
     X = data.get("X")
-    out = dict(seconds=t1 - t0, X=X, stats=st, stats_warm=stats_w, kernel_times=kt,
-               nnz_off=int(st.nnz_offdiag),
-               pcg_iters=int(st.pcg_iterations_total - st0.pcg_iterations_total),
-               pcg_solves=int(st.pcg_solves_total - st0.pcg_solves_total))
+    kt, tst = {}, None
+    if timing_steps > 0:
+        # kernel durations from HIP events on the solver's stream, in extra
+        # steps after the timed region (events between launches add gaps)
+        names = ("local_step", "pcg_spmv", "pcg_update", "pcg_solve")
+        pd.set_kernel_timing(names)
+        for _ in range(timing_steps):
+            pd.Update(data, p)
+        pd.synchronize()
+        st2 = pd.stats()
+        kt = {k: pd.kernel_time(k) for k in names}
+        tst = dict(steps=timing_steps, pcg_iters=int(st2.pcg_iterations_total - st1.pcg_iterations_total),
+                   pcg_solves=int(st2.pcg_solves_total - st1.pcg_solves_total))
+    out = dict(seconds=t1 - t0, X=X, stats=st1, kernel_times=kt, timing=tst, nnz_off=int(st1.nnz_offdiag),
+               pcg_iters=int(st1.pcg_iterations_total - st0.pcg_iterations_total),
+               pcg_solves=int(st1.pcg_solves_total - st0.pcg_solves_total),
+               unconverged=int(st1.pcg_cap_hits - st0.pcg_cap_hits))
     pd.close()
     data.free()
     return out
@@ -118,6 +126,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--timing-steps", type=int, default=3, help="extra steps timed kernel by kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU (the PD step of one body on one MI355X); "
@@ -126,13 +135,13 @@ def main():
 
     scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
     print(f"# scene: {scene.nv} vertices, {scene.nt} tets", file=sys.stderr, flush=True)
-    g = run_gpu(scene, args.steps, args.warmup, 32, time_kernels=True)
+    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=args.timing_steps)
     print(f"# gpu fp32: {args.steps} steps in {g['seconds']:.3f} s", file=sys.stderr, flush=True)
     sps = args.steps / g["seconds"]
 
     err = None
     if not args.no_fp64:
-        r = run_gpu(scene, args.steps, args.warmup, 64, time_kernels=False)
+        r = run_gpu(scene, args.steps, args.warmup, 64, timing_steps=0)
         print(f"# gpu fp64 reference run: {r['seconds']:.3f} s", file=sys.stderr, flush=True)
         disp = np.abs(r["X"] - scene.X).max()
         err = dict(rel_l2=float(np.linalg.norm(g["X"] - r["X"]) / np.linalg.norm(r["X"])),
@@ -140,29 +149,29 @@ def main():
                    max_abs_over_max_displacement=float(np.abs(g["X"] - r["X"]).max() / disp),
                    reference="FP64 run of the same steps on the GPU (FP64 path parity-tested vs the FP64 oracle)")
 
-    kt = g["kernel_times"]
-    nl, ml = kt["local_step"]
-    npcg, mpcg = kt["pcg_solve"]
-    t_local = ml / max(nl, 1) * 1e-3
-    t_pcg = mpcg / max(npcg, 1) * 1e-3
-    b_local = local_step_bytes(scene.nt, scene.nv)
-    it_per_solve = g["pcg_iters"] / max(g["pcg_solves"], 1)
-    # a PCG launch is priced at its iterations only (the init SpMV and the
-    # Chebyshev pass are not counted): a lower bound on its bytes
-    b_pcg = int(round(it_per_solve * pcg_iteration_bytes(g["nnz_off"], scene.nv)))
-    gbs_local = b_local / t_local / 1e9
-    gbs_pcg = b_pcg / t_pcg / 1e9
-    if mpcg >= ml:
-        dom, ach, bytes_per = "pcg_solve", gbs_pcg, b_pcg
-    else:
-        dom, ach, bytes_per = "local_step", gbs_local, b_local
-    roof = dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, algorithmic_bytes_per_launch=bytes_per,
-                kernels={k: dict(launches=v[0], total_ms=round(v[1], 3),
-                                 avg_us=round(v[1] / max(v[0], 1) * 1e3, 2)) for k, v in kt.items()},
-                local_step=dict(avg_us=round(t_local * 1e6, 2), gbs=round(gbs_local, 1), bytes=b_local),
-                pcg_solve=dict(avg_us=round(t_pcg * 1e6, 2), gbs=round(gbs_pcg, 1), bytes=b_pcg,
-                               iterations_per_launch=round(it_per_solve, 3)))
+    kt, tm = g["kernel_times"], g["timing"]
+    iters = tm["pcg_iters"]
+    per_launch = {
+        "local_step": local_step_bytes(scene.nt, scene.nv),
+        # algorithmic bytes of the effective iterations spread over every launch
+        # (the stop probe and launches after convergence do no vector work)
+        "pcg_spmv": iters * pcg_spmv_bytes(g["nnz_off"], scene.nv) / max(kt["pcg_spmv"][0], 1),
+        "pcg_update": iters * pcg_update_bytes(scene.nv) / max(kt["pcg_update"][0], 1),
+    }
+    kinfo = {}
+    for k, (n, ms) in kt.items():
+        d = dict(launches=n, total_ms=round(ms, 3), avg_us=round(ms / max(n, 1) * 1e3, 2))
+        if k in per_launch:
+            d["bytes_per_launch"] = int(per_launch[k])
+            d["gbs"] = round(per_launch[k] / (ms / max(n, 1) * 1e-3) / 1e9, 1)
+        kinfo[k] = d
+    dom = max(per_launch, key=lambda k: kt[k][1])
+    ach = kinfo[dom]["gbs"]
+    roof = dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                algorithmic_bytes_per_launch=kinfo[dom]["bytes_per_launch"], kernels=kinfo,
+                timing_sample=f"{tm['steps']} extra steps after the timed region, HIP events around every launch, "
+                              f"{iters} PCG iterations in {tm['pcg_solves']} solves")
 
     cpu = None
     if not args.no_cpu:
@@ -181,8 +190,8 @@ def main():
         "cpu_baseline": cpu,
         "error_vs_fp64": err,
         "pcg": {"iterations_per_solve": round(g["pcg_iters"] / max(g["pcg_solves"], 1), 3),
-                "solves": g["pcg_solves"], "persistent_grid_blocks": st.pcg_cap,
-                "unconverged_solves": int(st.pcg_cap_hits)},
+                "solves": g["pcg_solves"], "largest_launch_cap": st.pcg_cap,
+                "unconverged_solves": g["unconverged"]},
     }
     print(json.dumps(line), flush=True)
 

@@ -93,8 +93,8 @@ typedef struct SbsPd SbsPd;
 /* Step statistics of the last completed update (see sbs_pd_stats). */
 typedef struct SbsPdStats {
     int pcg_iterations_last_step; /* PCG iterations summed over the last step */
-    int pcg_cap;                  /* blocks of the persistent PCG launch       */
-    long long pcg_cap_hits;       /* solves stopped by pcg_max_iter before the tolerance */
+    int pcg_cap;                  /* largest learned PCG launch cap (iterations) */
+    long long pcg_cap_hits;       /* solves stopped (pcg_max_iter or launch cap) before the tolerance */
     long long nnz_offdiag;        /* off-diagonal nonzeros of the scalar matrix */
     long long pcg_iterations_total; /* PCG iterations over all completed steps  */
     long long pcg_solves_total;     /* PCG solves over all completed steps      */
@@ -120,12 +120,14 @@ int sbs_pd_stats(SbsPd* pd, SbsPdStats* out);
 /* Kernel timing with HIP events on the handle's stream: set a bit mask of
  * SBS_KERNEL_* ids, run updates, then read launches and summed milliseconds
  * (reading waits for the stream).  Setting the mask clears the sums. */
-#define SBS_KERNEL_LOCAL_STEP 0        /* computeLocal: one launch per iteration   */
-#define SBS_KERNEL_PCG_SOLVE 1         /* one persistent launch per global solve    */
+#define SBS_KERNEL_LOCAL_STEP 0        /* computeLocal: one launch per iteration    */
+#define SBS_KERNEL_PCG_SOLVE 1         /* a whole PCG global solve (all its launches) */
 #define SBS_KERNEL_JACOBI_CHEBYSHEV 2
 #define SBS_KERNEL_DIRECT_SOLVE 3
 #define SBS_KERNEL_RHS_GATHER 4
-#define SBS_KERNEL_COUNT 5
+#define SBS_KERNEL_PCG_SPMV 5          /* p update + A p + p.q: one per PCG iteration */
+#define SBS_KERNEL_PCG_UPDATE 6        /* x, r update + r.z, z.z                     */
+#define SBS_KERNEL_COUNT 7
 int sbs_pd_set_kernel_timing(SbsPd* pd, unsigned mask);
 int sbs_pd_kernel_time(SbsPd* pd, int kernel_id, const char** name, int* launches, double* ms);
 

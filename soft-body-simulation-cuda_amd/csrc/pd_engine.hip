@@ -1,16 +1,13 @@
 // gfx950 kernels and host orchestration of the projective-dynamics step.
 // See pd_engine.hpp for the layout; reference citations are per kernel.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
 #include "svd3.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace sbs {
 
@@ -18,13 +15,16 @@ namespace {
 
 constexpr int kTpb = 256;      // threads per block of the streaming kernels (4 wavefronts)
 constexpr int kSlice = 64;     // sliced-ELL slice height = one wavefront
-constexpr int kPcgTpb = 256;   // threads per block of the persistent PCG kernel
-constexpr int kAccBank = 64;   // doubles per accumulator bank
-// accumulator slots inside a bank
-constexpr int kAccInit = 0;    // 9: r.z, z.z, (D^-1 b)^2 per component
-constexpr int kAccA = 16;      // 2 x 4: p.q per component, double-buffered by iteration parity
-constexpr int kAccB = 24;      // 2 x 8: r.z, z.z per component, double-buffered
-constexpr int kAccErr = 40;    // 1: sum (prev - x)^2
+constexpr int kStripes = 32;   // address stripes of each reduction slot
+constexpr int kSlotQ = 16;     // doubles per stripe
+constexpr int kSlotSize = kStripes * kSlotQ;
+// reduction slots (each kSlotSize doubles)
+constexpr int kSlotInit = 0;   // r.z, z.z, (D^-1 b)^2 per component
+constexpr int kSlotA = 1;      // 2 slots: p.q per component (by iteration parity)
+constexpr int kSlotB = 3;      // 2 slots: r.z, z.z per component (by iteration parity)
+constexpr int kSlotErr = 5;    // sum (prev - x)^2 of computeError
+constexpr int kNumSlots = 6;
+constexpr int kCheckChunk = 8;  // iterations between host checks of an unlearned solve
 
 // Pin weight of computeDBCLocal: positional_weight * dt2Inv with
 // dt2Inv = dtInv * dtInv in the solver precision (pdSolver.cu:145-147, :171;
@@ -46,20 +46,12 @@ template <class T> __device__ __forceinline__ void st3(T* __restrict__ p, size_t
     p[3 * i + 2] = z;
 }
 
-__device__ __forceinline__ double acc_load(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void acc_store(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void acc_add(double* p, double v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sums NQ per-thread values over the block; thread 0 adds them to acc[0..NQ).
-template <int NQ, int TPB>
-__device__ __forceinline__ void block_accumulate(const double (&vals)[NQ], double* acc) {
-    __shared__ double red[TPB / 64][NQ];
+// Sums NQ per-thread values over the block; thread 0 adds them to the
+// block's stripe of `slot` (stripe = block id mod kStripes), so no address
+// sees more than gridDim/kStripes atomics.
+template <int NQ>
+__device__ __forceinline__ void stripe_add(const double (&vals)[NQ], double* slot) {
+    __shared__ double red[kTpb / 64][NQ];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
@@ -67,52 +59,71 @@ __device__ __forceinline__ void block_accumulate(const double (&vals)[NQ], doubl
         if (lane == 0) red[wid][k] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < NQ) {
+        double s = 0;
+        for (int w = 0; w < kTpb / 64; ++w) s += red[w][threadIdx.x];
+        atomicAdd(&slot[(blockIdx.x % kStripes) * kSlotQ + threadIdx.x], s);
+    }
+}
+
+// Totals of the first NQ quantities of a slot (written by an earlier kernel),
+// summed by the block's first wavefront and broadcast through LDS.
+template <int NQ>
+__device__ __forceinline__ void slot_totals(const double* slot, double (&out)[NQ]) {
+    __shared__ double tot[NQ];
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            double s = 0;
-            for (int w = 0; w < TPB / 64; ++w) s += red[w][k];
-            acc_add(&acc[k], s);
+            const double v = lane < kStripes ? slot[lane * kSlotQ + k] : 0.0;
+            const double s = wave_sum(v);
+            if (lane == 0) tot[k] = s;
         }
     }
     __syncthreads();
-}
-
-// Grid-wide barrier with explicit agent-scope release/acquire: the L2 of the
-// 8 XCDs are not coherent with each other, so rows written by one XCD and
-// gathered by another need a write-back before and an invalidate after.
-__device__ __forceinline__ void grid_barrier(cg::grid_group& grid) {
-    __syncthreads();
-    if (threadIdx.x == 0) __threadfence();
-    grid.sync();
-    if (threadIdx.x == 0) __threadfence();
-    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) out[k] = tot[k];
 }
 
 // ---------------------------------------------------------------------------
-// Device control block: per-step counters and the outer-loop early exit.
+// Device control block: per-step counters, the outer-loop early exit and
+// the PCG state (double-buffered by iteration parity: the A kernel of
+// iteration k writes st[k&1] from st[(k-1)&1] and the B slot of k-1).
+struct PcgState {
+    double rz[3];
+    double beta[3];
+    double thr[3];
+    int active[3];
+    int iters;       // completed iterations
+    int done;        // converged (all components) or out of iterations
+    int unconverged; // stopped by max_iter with a component still active
+    int pad;
+};
+
+// The first kCounterInts ints are copied to the host at the end of a step.
 struct CtlBlock {
-    int outer_done;      // local/global loop early exit (direct / PCG modes)
     int pcg_iters;       // PCG iterations summed over the step
-    int unconverged;     // PCG solves that hit max_iter before the tolerance
+    int unconverged;     // PCG solves stopped before the tolerance (max_iter or launch cap)
     int max_used;        // most iterations of one solve this step
     int solves;          // PCG solves this step
     int last_iters;      // iterations of the last solve
-    unsigned ticket;     // block arrival counter of the ticket reductions
-    int pad;
+    int pad0;
+    int log[kMaxSolveLog];  // iterations of solve i of the step (-its-1 when stopped early)
+    int outer_done;      // local/global loop early exit (direct / PCG modes)
+    int pad1;
     double err;          // last computeError value
+    PcgState st[2];
 };
+static_assert(offsetof(CtlBlock, log) + sizeof(int) * kMaxSolveLog == sizeof(int) * kCounterInts,
+              "counter block layout");
 
 __global__ void ctl_init_kernel(CtlBlock* ctl) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    ctl->outer_done = 0;
-    ctl->pcg_iters = 0;
-    ctl->unconverged = 0;
-    ctl->max_used = 0;
-    ctl->solves = 0;
-    ctl->last_iters = 0;
-    ctl->ticket = 0;
-    ctl->err = 1.0;
+    if (blockIdx.x != 0) return;
+    for (int i = threadIdx.x; i < kCounterInts; i += blockDim.x) (&ctl->pcg_iters)[i] = 0;
+    if (threadIdx.x == 0) {
+        ctl->outer_done = 0;
+        ctl->err = 1.0;
+    }
 }
 
 __global__ void zero_kernel(double* p, int n) {
@@ -280,9 +291,12 @@ template <class T>
 __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
     int nt, const uint2* __restrict__ lidx, const int* __restrict__ chunk_vptr,
     const uint32_t* __restrict__ chunk_verts, const T* __restrict__ dminv, const T* __restrict__ w,
-    const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip) {
+    const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip,
+    double* __restrict__ zero_slots, int zero_n) {
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
+    if (blockIdx.x == 0 && zero_slots)  // the PCG solve that follows accumulates into these
+        for (int i = threadIdx.x; i < zero_n; i += kChunkTets) zero_slots[i] = 0.0;
     const unsigned c = xcd_block(blockIdx.x, gridDim.x);
     const int e0 = chunk_vptr[c], nu = chunk_vptr[c + 1] - e0;
     T* qs = reinterpret_cast<T*>(smem_raw);
@@ -428,72 +442,45 @@ __global__ void end_step_kernel(int nv, const T* __restrict__ sn, T dt_inv, T* _
     st3(X, v, nx, ny, nz);
 }
 
-// computeError (pdSolver.cu:243-253) + prev <- sn and the early exit of the
-// local/global loop (pdSolver.cu:164): stop once sqrt(mean (prev-sn)^2) < tol.
-// The last block to finish (ticket) turns the sum into the decision.
+// computeError (pdSolver.cu:243-253): sum (prev - src)^2 into the error slot,
+// then prev <- src (and sn <- src when sn is given).  outer_decide_kernel
+// turns the sum into the early exit of the local/global loop.
 template <class T>
-__global__ void __launch_bounds__(kTpb) outer_error_kernel(int nv, const T* __restrict__ sn, T* __restrict__ prev,
-                                                           double* __restrict__ partial, CtlBlock* ctl, double tol) {
-    if (ctl->outer_done) return;
+__global__ void __launch_bounds__(kTpb) outer_error_kernel(int nv, const T* __restrict__ src, T* __restrict__ sn,
+                                                           T* __restrict__ prev, double* __restrict__ slots,
+                                                           const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    double e = 0;
+    double vals[1] = {0};
     if (v < (unsigned)nv) {
-        T s0, s1, s2, p0, p1, p2;
-        ld3(sn, v, s0, s1, s2);
+        T x0, x1, x2, p0, p1, p2;
+        ld3(src, v, x0, x1, x2);
         ld3(prev, v, p0, p1, p2);
-        const T d0 = p0 - s0, d1 = p1 - s1, d2 = p2 - s2;
-        e = (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2);
-        st3(prev, v, s0, s1, s2);
+        const T d0 = p0 - x0, d1 = p1 - x1, d2 = p2 - x2;
+        vals[0] = (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2);
+        if (sn) st3(sn, v, x0, x1, x2);
+        st3(prev, v, x0, x1, x2);
     }
-    __shared__ double red[kTpb / 64];
-    __shared__ bool last;
-    const double s = wave_sum(e);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0;
-        for (int w = 0; w < kTpb / 64; ++w) t += red[w];
-        partial[blockIdx.x] = t;
-        __threadfence();
-        last = atomicAdd(&ctl->ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double t = 0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += kTpb)
-            t += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double s2 = wave_sum(t);
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double tot = 0;
-            for (int w = 0; w < kTpb / 64; ++w) tot += red[w];
-            const double err = tot / (3.0 * (double)nv);
-            ctl->err = err;
-            ctl->outer_done = !(sqrt(err) >= tol);
-            ctl->ticket = 0;
-        }
-    }
+    stripe_add<1>(vals, slots + kSlotErr * kSlotSize);
 }
 
 // ---------------------------------------------------------------------------
-// Persistent Jacobi-PCG over the scalar matrix with 3 fused right-hand sides
-// (one cooperative launch per global solve; the stop test runs on the GPU).
+// Jacobi-PCG over the scalar matrix with 3 fused right-hand sides.
 //
-// Each block owns a contiguous range of rows (XCD-aware placement keeps the
-// neighbours it gathers in the same L2).  Per iteration k:
-//   A: p = D^-1 r + beta p_old (own rows; neighbours' p recomputed on the fly
-//      from r and p_old), q = A p, p.q             -> grid barrier -> alpha
-//   B: x += alpha p, r -= alpha q, z = D^-1 r, r.z, z.z -> grid barrier -> beta, stop
-// Dot products are reduced per block (wave shuffles + LDS) and added into
-// double accumulators; a component stops when ||z|| <= max(tol ||z_0||,
-// floor ||D^-1 b||).  The launch ends with the Chebyshev recurrence
-// (finish = 1) or a copy into sn with computeError (finish = 2).
+// Kernel boundaries are the grid-wide synchronisation (on MI355X a dependent
+// launch costs ~2.5-3.7 us, less than any in-kernel grid barrier measured).
+// Per solve:  init, then per iteration k
+//   A(k): state from st[(k-1)&1] + slot B[(k-1)&1] (beta, stop test) written
+//         to st[k&1]; p = D^-1 r + beta p_old (own rows; neighbours' p
+//         recomputed on the fly from r and p_old), q = A p, p.q -> slot A[k&1]
+//   B(k): alpha from st[k&1] + slot A[k&1]; x += alpha p, r -= alpha q,
+//         r.z, z.z -> slot B[k&1]; zeroes the slots of iteration k+1
+// then a probe A(L) that only evaluates the stop test, and a finish kernel
+// (Chebyshev recurrence, or copy into sn with computeError).  Launches after
+// convergence return at once.  A component stops when
+// ||z|| <= max(tol ||z_0||, floor ||D^-1 b||).
 template <class T> struct PcgArgs {
     int nv;
-    int rows_per_block;
     int gathered;             // 1: rhs from the local-step partials (+ pins), 0: plain b
     RhsGather<T> g;
     const T* b;
@@ -509,13 +496,10 @@ template <class T> struct PcgArgs {
     T* p0;
     T* p1;
     T* q;
-    int finish;               // 0: none, 1: Chebyshev into sn/prev, 2: sn = x + computeError
     T* sn;
     T* prev;
     T omega;
-    double outer_tol;
-    double* acc;              // this launch's accumulator bank
-    double* acc_other;        // the other bank (zeroed here for the next launch)
+    double* slots;            // kNumSlots x kSlotSize doubles
     CtlBlock* ctl;
     int max_iter;
     double tol2;
@@ -524,193 +508,213 @@ template <class T> struct PcgArgs {
 };
 
 template <class T>
-__device__ __forceinline__ void spmv_gather_row(const PcgArgs<T>& a, unsigned v, const T* __restrict__ xs, T& y0, T& y1,
-                                                T& y2) {
-    T x0, x1, x2;
-    ld3(xs, v, x0, x1, x2);
-    const T d = a.diagA[v];
-    y0 = d * x0; y1 = d * x1; y2 = d * x2;
-    const unsigned s = v / kSlice, lane = v % kSlice;
-    const size_t base = (size_t)a.slice_ptr[s] + lane;
-    const int width = a.slice_w[s];
-    for (int e = 0; e < width; ++e) {
-        const size_t pos = base + (size_t)e * kSlice;
-        const uint32_t u = a.ecol[pos];
-        const T m = a.eval[pos];
-        T u0, u1, u2;
-        ld3(xs, u, u0, u1, u2);
-        y0 += m * u0; y1 += m * u1; y2 += m * u2;
+__global__ void __launch_bounds__(kTpb) pcg_init_kernel(PcgArgs<T> a) {
+    if (a.skip && *a.skip) return;
+    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (v < (unsigned)a.nv) {
+        T b0, b1, b2;
+        if (a.gathered) a.g.at(v, b0, b1, b2);
+        else ld3(a.b, v, b0, b1, b2);
+        T ax0 = 0, ax1 = 0, ax2 = 0, x0 = 0, x1 = 0, x2 = 0;
+        if (a.x0) {
+            ld3(a.x0, v, x0, x1, x2);
+            const T d = a.diagA[v];
+            ax0 = d * x0; ax1 = d * x1; ax2 = d * x2;
+            const unsigned s = v / kSlice, lane = v % kSlice;
+            const size_t base = (size_t)a.slice_ptr[s] + lane;
+            const int width = a.slice_w[s];
+            for (int e = 0; e < width; ++e) {
+                const size_t pos = base + (size_t)e * kSlice;
+                const uint32_t u = a.ecol[pos];
+                const T m = a.eval[pos];
+                T u0, u1, u2;
+                ld3(a.x0, u, u0, u1, u2);
+                ax0 += m * u0; ax1 += m * u1; ax2 += m * u2;
+            }
+        }
+        st3(a.x, v, x0, x1, x2);
+        const T rr0 = b0 - ax0, rr1 = b1 - ax1, rr2 = b2 - ax2;
+        st3(a.r, v, rr0, rr1, rr2);
+        st3(a.p0, v, (T)0, (T)0, (T)0);
+        const T di = a.dinv[v];
+        const T z0 = di * rr0, z1 = di * rr1, z2 = di * rr2;
+        vals[0] = (double)rr0 * z0; vals[1] = (double)rr1 * z1; vals[2] = (double)rr2 * z2;
+        vals[3] = (double)z0 * z0; vals[4] = (double)z1 * z1; vals[5] = (double)z2 * z2;
+        const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
+        vals[6] = d0 * d0; vals[7] = d1 * d1; vals[8] = d2 * d2;
     }
+    stripe_add<9>(vals, a.slots + kSlotInit * kSlotSize);
+}
+
+// Stop test / beta of iteration k, identical in every block; block 0 records it.
+template <class T> __device__ __forceinline__ PcgState pcg_state(const PcgArgs<T>& a, int k) {
+    PcgState st;
+    if (k == 0) {
+        double t[9];
+        slot_totals<9>(a.slots + kSlotInit * kSlotSize, t);
+        bool any = false;
+        for (int c = 0; c < 3; ++c) {
+            st.rz[c] = t[c];
+            st.beta[c] = 0.0;
+            st.thr[c] = fmax(a.tol2 * t[3 + c], a.floor2 * t[6 + c]);
+            st.active[c] = t[3 + c] > st.thr[c];
+            any |= st.active[c] != 0;
+        }
+        st.iters = 0;
+        st.done = !any || a.max_iter <= 0;
+        st.unconverged = any && a.max_iter <= 0;
+    } else {
+        st = a.ctl->st[(k - 1) & 1];
+        if (!st.done) {
+            double t[6];
+            slot_totals<6>(a.slots + (kSlotB + ((k - 1) & 1)) * kSlotSize, t);
+            bool any = false;
+            for (int c = 0; c < 3; ++c) {
+                const bool act = st.active[c] != 0;
+                st.beta[c] = (act && st.rz[c] != 0.0) ? t[c] / st.rz[c] : 0.0;
+                st.rz[c] = t[c];
+                st.active[c] = act && t[3 + c] > st.thr[c];
+                any |= st.active[c] != 0;
+            }
+            st.iters = k;
+            st.done = !any || k >= a.max_iter;
+            st.unconverged = any && k >= a.max_iter;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[k & 1] = st;
+    return st;
+}
+
+// Search direction p = D^-1 r + beta p_old.  One explicit fma so that a row's
+// own p and the copy its neighbours recompute are the same bits.
+template <class T> __device__ __forceinline__ T p_next(T dinv, T r, T beta, T p_old) {
+    return fma(beta, p_old, dinv * r);
 }
 
 template <class T>
-__global__ void __launch_bounds__(kPcgTpb) pcg_persistent_kernel(PcgArgs<T> a) {
-    if (a.skip && *a.skip) return;   // uniform: read once by every block before any barrier
-    cg::grid_group grid = cg::this_grid();
-    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = (int)lb * a.rows_per_block;
-    const int r1 = min(a.nv, r0 + a.rows_per_block);
-    if (blockIdx.x == 0)
-        for (int i = threadIdx.x; i < kAccBank; i += kPcgTpb) acc_store(&a.acc_other[i], 0.0);
+__global__ void __launch_bounds__(kTpb) pcg_a_kernel(PcgArgs<T> a, int k, int probe) {
+    if (a.skip && *a.skip) return;
+    const PcgState st = pcg_state(a, k);
+    if (st.done || probe) return;
+    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const T* pin = (k & 1) ? a.p1 : a.p0;
+    T* pout = (k & 1) ? a.p0 : a.p1;
+    const T be0 = (T)st.beta[0], be1 = (T)st.beta[1], be2 = (T)st.beta[2];
+    double vals[3] = {0, 0, 0};
+    if (v < (unsigned)a.nv) {
+        T rv0, rv1, rv2, pv0, pv1, pv2;
+        ld3(a.r, v, rv0, rv1, rv2);
+        ld3(pin, v, pv0, pv1, pv2);
+        const T dv = a.dinv[v];
+        const T p0 = p_next(dv, rv0, be0, pv0), p1 = p_next(dv, rv1, be1, pv1), p2 = p_next(dv, rv2, be2, pv2);
+        st3(pout, v, p0, p1, p2);
+        const T d = a.diagA[v];
+        T y0 = d * p0, y1 = d * p1, y2 = d * p2;
+        const unsigned s = v / kSlice, lane = v % kSlice;
+        const size_t base = (size_t)a.slice_ptr[s] + lane;
+        const int width = a.slice_w[s];
+        for (int e = 0; e < width; ++e) {
+            const size_t pos = base + (size_t)e * kSlice;
+            const uint32_t u = a.ecol[pos];
+            const T m = a.eval[pos];
+            T ru0, ru1, ru2, pu0, pu1, pu2;
+            ld3(a.r, u, ru0, ru1, ru2);
+            ld3(pin, u, pu0, pu1, pu2);
+            const T du = a.dinv[u];
+            y0 += m * p_next(du, ru0, be0, pu0);
+            y1 += m * p_next(du, ru1, be1, pu1);
+            y2 += m * p_next(du, ru2, be2, pu2);
+        }
+        st3(a.q, v, y0, y1, y2);
+        vals[0] = (double)p0 * y0; vals[1] = (double)p1 * y1; vals[2] = (double)p2 * y2;
+    }
+    stripe_add<3>(vals, a.slots + (kSlotA + (k & 1)) * kSlotSize);
+}
 
-    // ---- init: r = b - A x0, p_old = 0, norms ----
-    {
-        double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
-            T b0, b1, b2;
-            if (a.gathered) a.g.at(v, b0, b1, b2);
-            else ld3(a.b, v, b0, b1, b2);
-            T ax0 = 0, ax1 = 0, ax2 = 0, x0 = 0, x1 = 0, x2 = 0;
-            if (a.x0) {
-                spmv_gather_row(a, v, a.x0, ax0, ax1, ax2);
-                ld3(a.x0, v, x0, x1, x2);
-            }
-            st3(a.x, v, x0, x1, x2);
-            const T rr0 = b0 - ax0, rr1 = b1 - ax1, rr2 = b2 - ax2;
-            st3(a.r, v, rr0, rr1, rr2);
-            st3(a.p0, v, (T)0, (T)0, (T)0);
-            const T di = a.dinv[v];
-            const T z0 = di * rr0, z1 = di * rr1, z2 = di * rr2;
-            vals[0] += (double)rr0 * z0; vals[1] += (double)rr1 * z1; vals[2] += (double)rr2 * z2;
-            vals[3] += (double)z0 * z0; vals[4] += (double)z1 * z1; vals[5] += (double)z2 * z2;
-            const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
-            vals[6] += d0 * d0; vals[7] += d1 * d1; vals[8] += d2 * d2;
-        }
-        block_accumulate<9, kPcgTpb>(vals, a.acc + kAccInit);
+template <class T>
+__global__ void __launch_bounds__(kTpb) pcg_b_kernel(PcgArgs<T> a, int k) {
+    if (a.skip && *a.skip) return;
+    const PcgState st = a.ctl->st[k & 1];
+    if (st.done) return;
+    double pq[3];
+    slot_totals<3>(a.slots + (kSlotA + (k & 1)) * kSlotSize, pq);
+    T al[3];
+    for (int c = 0; c < 3; ++c) al[c] = (T)((st.active[c] && pq[c] != 0.0) ? st.rz[c] / pq[c] : 0.0);
+    if (blockIdx.x == 0) {  // slots of iteration k+1 were last read by kernels that have finished
+        double* za = a.slots + (kSlotA + ((k + 1) & 1)) * kSlotSize;
+        double* zb = a.slots + (kSlotB + ((k + 1) & 1)) * kSlotSize;
+        for (int i = threadIdx.x; i < kSlotSize; i += kTpb) { za[i] = 0.0; zb[i] = 0.0; }
     }
-    grid_barrier(grid);
-    double rz[3], thr[3], beta[3] = {0, 0, 0};
-    bool active[3];
-    for (int k = 0; k < 3; ++k) {
-        rz[k] = acc_load(&a.acc[kAccInit + k]);
-        const double zz = acc_load(&a.acc[kAccInit + 3 + k]);
-        const double dd = acc_load(&a.acc[kAccInit + 6 + k]);
-        thr[k] = fmax(a.tol2 * zz, a.floor2 * dd);
-        active[k] = zz > thr[k];
+    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const T* pcur = (k & 1) ? a.p0 : a.p1;
+    double vals[6] = {0, 0, 0, 0, 0, 0};
+    if (v < (unsigned)a.nv) {
+        T x0, x1, x2, rv0, rv1, rv2, p0, p1, p2, q0, q1, q2;
+        ld3(a.x, v, x0, x1, x2);
+        ld3(a.r, v, rv0, rv1, rv2);
+        ld3(pcur, v, p0, p1, p2);
+        ld3(a.q, v, q0, q1, q2);
+        x0 += al[0] * p0; x1 += al[1] * p1; x2 += al[2] * p2;
+        rv0 -= al[0] * q0; rv1 -= al[1] * q1; rv2 -= al[2] * q2;
+        st3(a.x, v, x0, x1, x2);
+        st3(a.r, v, rv0, rv1, rv2);
+        const T dv = a.dinv[v];
+        const T z0 = dv * rv0, z1 = dv * rv1, z2 = dv * rv2;
+        vals[0] = (double)rv0 * z0; vals[1] = (double)rv1 * z1; vals[2] = (double)rv2 * z2;
+        vals[3] = (double)z0 * z0; vals[4] = (double)z1 * z1; vals[5] = (double)z2 * z2;
     }
-    int it = 0;
-    while ((active[0] || active[1] || active[2]) && it < a.max_iter) {
-        const int par = it & 1;
-        const T* pin = par ? a.p1 : a.p0;
-        T* pout = par ? a.p0 : a.p1;
-        const T be0 = (T)beta[0], be1 = (T)beta[1], be2 = (T)beta[2];
-        // ---- phase A: p, q = A p, p.q ----
-        {
-            double vals[3] = {0, 0, 0};
-            for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
-                T rv0, rv1, rv2, pv0, pv1, pv2;
-                ld3(a.r, v, rv0, rv1, rv2);
-                ld3(pin, v, pv0, pv1, pv2);
-                const T dv = a.dinv[v];
-                const T p0 = dv * rv0 + be0 * pv0, p1 = dv * rv1 + be1 * pv1, p2 = dv * rv2 + be2 * pv2;
-                st3(pout, v, p0, p1, p2);
-                const T d = a.diagA[v];
-                T y0 = d * p0, y1 = d * p1, y2 = d * p2;
-                const unsigned s = (unsigned)v / kSlice, lane = (unsigned)v % kSlice;
-                const size_t base = (size_t)a.slice_ptr[s] + lane;
-                const int width = a.slice_w[s];
-                for (int e = 0; e < width; ++e) {
-                    const size_t pos = base + (size_t)e * kSlice;
-                    const uint32_t u = a.ecol[pos];
-                    const T m = a.eval[pos];
-                    T ru0, ru1, ru2, pu0, pu1, pu2;
-                    ld3(a.r, u, ru0, ru1, ru2);
-                    ld3(pin, u, pu0, pu1, pu2);
-                    const T du = a.dinv[u];
-                    y0 += m * (du * ru0 + be0 * pu0);
-                    y1 += m * (du * ru1 + be1 * pu1);
-                    y2 += m * (du * ru2 + be2 * pu2);
-                }
-                st3(a.q, v, y0, y1, y2);
-                vals[0] += (double)p0 * y0; vals[1] += (double)p1 * y1; vals[2] += (double)p2 * y2;
-            }
-            block_accumulate<3, kPcgTpb>(vals, a.acc + kAccA + 4 * par);
-        }
-        grid_barrier(grid);
-        if (blockIdx.x == 0 && it > 0)  // B slot of the previous iteration: everyone has read it
-            for (int i = threadIdx.x; i < 8; i += kPcgTpb) acc_store(&a.acc[kAccB + 8 * (1 - par) + i], 0.0);
-        T al[3];
-        for (int k = 0; k < 3; ++k) {
-            const double pq = acc_load(&a.acc[kAccA + 4 * par + k]);
-            al[k] = (T)((active[k] && pq != 0.0) ? rz[k] / pq : 0.0);
-        }
-        // ---- phase B: x, r, z and their norms ----
-        {
-            double vals[6] = {0, 0, 0, 0, 0, 0};
-            for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
-                T x0, x1, x2, rv0, rv1, rv2, p0, p1, p2, q0, q1, q2;
-                ld3(a.x, v, x0, x1, x2);
-                ld3(a.r, v, rv0, rv1, rv2);
-                ld3(pout, v, p0, p1, p2);
-                ld3(a.q, v, q0, q1, q2);
-                x0 += al[0] * p0; x1 += al[1] * p1; x2 += al[2] * p2;
-                rv0 -= al[0] * q0; rv1 -= al[1] * q1; rv2 -= al[2] * q2;
-                st3(a.x, v, x0, x1, x2);
-                st3(a.r, v, rv0, rv1, rv2);
-                const T dv = a.dinv[v];
-                const T z0 = dv * rv0, z1 = dv * rv1, z2 = dv * rv2;
-                vals[0] += (double)rv0 * z0; vals[1] += (double)rv1 * z1; vals[2] += (double)rv2 * z2;
-                vals[3] += (double)z0 * z0; vals[4] += (double)z1 * z1; vals[5] += (double)z2 * z2;
-            }
-            block_accumulate<6, kPcgTpb>(vals, a.acc + kAccB + 8 * par);
-        }
-        grid_barrier(grid);
-        if (blockIdx.x == 0)  // A slot of this iteration: everyone has read it
-            for (int i = threadIdx.x; i < 4; i += kPcgTpb) acc_store(&a.acc[kAccA + 4 * par + i], 0.0);
-        for (int k = 0; k < 3; ++k) {
-            const double rzn = acc_load(&a.acc[kAccB + 8 * par + k]);
-            const double zz = acc_load(&a.acc[kAccB + 8 * par + 3 + k]);
-            beta[k] = (active[k] && rz[k] != 0.0) ? rzn / rz[k] : 0.0;
-            rz[k] = rzn;
-            active[k] = active[k] && zz > thr[k];
-        }
-        ++it;
-    }
-    // ---- finish ----
-    if (a.finish == 1) {
-        for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
-            for (int k = 0; k < 3; ++k) {
-                const size_t i = 3 * (size_t)v + k;
-                const T s = a.sn[i];
-                T nx = (T)0.9 * (a.x[i] - s) + s;
-                nx = (nx - a.prev[i]) * a.omega + a.prev[i];
-                a.prev[i] = s;
-                a.sn[i] = nx;
-            }
-        }
-    } else if (a.finish == 2) {
-        double vals[1] = {0};
-        for (int v = r0 + (int)threadIdx.x; v < r1; v += kPcgTpb) {
-            T x0, x1, x2, p0, p1, p2;
-            ld3(a.x, v, x0, x1, x2);
-            ld3(a.prev, v, p0, p1, p2);
-            const T d0 = p0 - x0, d1 = p1 - x1, d2 = p2 - x2;
-            vals[0] += (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2);
-            st3(a.sn, v, x0, x1, x2);
-            st3(a.prev, v, x0, x1, x2);
-        }
-        block_accumulate<1, kPcgTpb>(vals, a.acc + kAccErr);
-        grid_barrier(grid);
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            const double err = acc_load(&a.acc[kAccErr]) / (3.0 * (double)a.nv);
-            a.ctl->err = err;
-            a.ctl->outer_done = !(sqrt(err) >= a.outer_tol);
-        }
-    }
+    stripe_add<6>(vals, a.slots + (kSlotB + (k & 1)) * kSlotSize);
+}
+
+// After the probe A(L): bookkeeping (block 0), and with `cheb` the
+// Chebyshev recurrence of the global step (pdSolver.cu:196-205) into sn/prev.
+template <class T>
+__global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, int cheb) {
+    if (a.skip && *a.skip) return;
+    const PcgState st = a.ctl->st[L & 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         CtlBlock* c = a.ctl;
-        c->pcg_iters += it;
+        const int its = st.done ? st.iters : L;
+        const bool cut = st.unconverged || !st.done;  // max_iter or the launch cap
+        if (c->solves < kMaxSolveLog) c->log[c->solves] = cut ? -its - 1 : its;
+        c->pcg_iters += its;
         c->solves += 1;
-        c->last_iters = it;
-        if (it > c->max_used) c->max_used = it;
-        if (active[0] || active[1] || active[2]) c->unconverged += 1;
+        c->last_iters = its;
+        if (its > c->max_used) c->max_used = its;
+        if (cut) c->unconverged += 1;
+    }
+    if (!cheb) return;
+    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (v < (unsigned)a.nv) {
+        for (int k = 0; k < 3; ++k) {
+            const size_t i = 3 * (size_t)v + k;
+            const T s = a.sn[i];
+            T nx = (T)0.9 * (a.x[i] - s) + s;
+            nx = (nx - a.prev[i]) * a.omega + a.prev[i];
+            a.prev[i] = s;
+            a.sn[i] = nx;
+        }
+    }
+}
+
+// Early exit of the local/global loop from the computeError sum (pdSolver.cu:164).
+__global__ void outer_decide_kernel(const double* slots, int nv, double tol, CtlBlock* ctl, const int* skip) {
+    if (skip && *skip) return;
+    double t[1];
+    slot_totals<1>(slots + kSlotErr * kSlotSize, t);
+    if (threadIdx.x == 0) {
+        const double err = t[0] / (3.0 * (double)nv);
+        ctl->err = err;
+        ctl->outer_done = !(sqrt(err) >= tol);
     }
 }
 
 }  // namespace
 
 const char* kernel_name(int id) {
-    static const char* names[kKCount] = {"local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve", "rhs_gather"};
+    static const char* names[kKCount] = {"local_step", "pcg_solve",  "jacobi_chebyshev", "direct_solve",
+                                         "rhs_gather", "pcg_spmv",   "pcg_update"};
     return (id >= 0 && id < kKCount) ? names[id] : "?";
 }
 
@@ -741,8 +745,10 @@ template <class T> PdEngine<T>::PdEngine(int threads_per_block, const PdData<T>&
                        (const uint4*)data.Tet, tet_perm_d_.get(), data.mu, dminv_.get(), v0_.get(), w_.get(),
                        data.V0);
     SBS_HIP(hipGetLastError());
-    SBS_HIP(hipHostMalloc((void**)&host_counters_, 16 * sizeof(int), hipHostMallocDefault));
-    std::memset(host_counters_, 0, 16 * sizeof(int));
+    SBS_HIP(hipHostMalloc((void**)&host_counters_, 2 * kCounterInts * sizeof(int), hipHostMallocDefault));
+    std::memset(host_counters_, 0, 2 * kCounterInts * sizeof(int));
+    SBS_HIP(hipHostMalloc(&host_ctl_, sizeof(CtlBlock), hipHostMallocDefault));
+    std::memset(host_ctl_, 0, sizeof(CtlBlock));
     for (auto& e : step_event_) SBS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     SBS_HIP(hipStreamSynchronize(stream_));
 }
@@ -754,6 +760,7 @@ template <class T> PdEngine<T>::~PdEngine() {
             (void)hipEventDestroy(e);
         }
     if (host_counters_) (void)hipHostFree(host_counters_);
+    if (host_ctl_) (void)hipHostFree(host_ctl_);
     for (auto& pool : tev_)
         for (auto& e : pool) {
             (void)hipEventDestroy(e.first);
@@ -917,21 +924,11 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
     c_.alloc(nv_); diagL_.alloc(nv_); diagA_.alloc(nv_); dinv_.alloc(nv_);
     sn_.alloc(n3); sn_old_.alloc(n3); prev_.alloc(n3); next_.alloc(n3); rhs_.alloc(n3);
     r_.alloc(n3); p0_.alloc(n3); p1_.alloc(n3); q_.alloc(n3);
-    acc_.alloc(2 * kAccBank);
+    slots_.alloc((size_t)kNumSlots * kSlotSize);
     ctl_.alloc(sizeof(CtlBlock));
-    red_partial_.alloc(ceil_div(nv_, kTpb) + 16);
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream_, acc_.get(), 2 * kAccBank);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, stream_, slots_.get(), kNumSlots * kSlotSize);
     hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, reinterpret_cast<CtlBlock*>(ctl_.get()));
-    // persistent PCG grid: every block co-resident (cooperative launch)
-    int dev = 0, cus = 0, per_cu = 0, coop = 0;
-    SBS_HIP(hipGetDevice(&dev));
-    SBS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    SBS_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-    SBS_REQUIRE(coop, "device does not support cooperative launch (needed by the PCG global solve)");
-    SBS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_persistent_kernel<T>, kPcgTpb, 0));
-    SBS_REQUIRE(per_cu > 0, "PCG kernel cannot be resident");
-    const int want = (int)ceil_div(nv_, kPcgTpb);
-    pcg_grid_ = std::max(1, std::min(want, cus * std::min(per_cu, 2)));
+    SBS_HIP(hipGetLastError());
     build_structure();
     assemble(params);
     system_built_ = true;
@@ -959,20 +956,24 @@ template <class T> void PdEngine<T>::launch_local(const T* q, bool residual_form
     const size_t lds = 6 * (size_t)chunk_max_verts_ * sizeof(T);
     SBS_REQUIRE(lds <= 64 * 1024, "local step: chunk vertex list exceeds LDS");
     tick(kKLocal, true);
+    // block 0 also clears the reduction slots of the global step that follows
     hipLaunchKernelGGL(local_chunk_kernel<T>, dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_, tet_lidx_.get(),
                        chunk_vptr_.get(), chunk_verts_.get(), dminv_.get(), w_.get(), q, partial_f_.get(),
-                       (int)residual_form, skip);
+                       (int)residual_form, skip, slots_.get(), kNumSlots * kSlotSize);
     tick(kKLocal, false);
     SBS_HIP(hipGetLastError());
 }
 
+// One global solve: init, L iterations (A, B), the probe A(L) and the finish
+// kernel.  L is the learned launch cap of this solve index when `exact` is
+// false and a cap is known; otherwise iterations are issued in chunks and the
+// host reads the device state after each chunk until the solve has stopped.
+// Returns L.
 template <class T>
-void PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, int finish, T omega, const PdParams& params,
-                             double outer_tol) {
+int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool cheb, T omega, const PdParams& params,
+                            const int* skip, int solve_index, bool exact) {
     PcgArgs<T> a{};
     a.nv = nv_;
-    const int rpb = (int)ceil_div(nv_, pcg_grid_);
-    a.rows_per_block = ((rpb + kSlice - 1) / kSlice) * kSlice;
     a.gathered = gathered;
     a.g = RhsGather<T>{c_.get(), sn_old_.get(), vent_ptr_.get(), vent_.get(), partial_f_.get(), data_.DBC,
                        data_.moreDBC, data_.DBCX, pin_weight<T>(params.dt)};
@@ -989,35 +990,81 @@ void PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, int fi
     a.p0 = p0_.get();
     a.p1 = p1_.get();
     a.q = q_.get();
-    a.finish = finish;
     a.sn = sn_.get();
     a.prev = prev_.get();
     a.omega = omega;
-    a.outer_tol = outer_tol;
-    const int bank = pcg_launches_ & 1;
-    a.acc = acc_.get() + bank * kAccBank;
-    a.acc_other = acc_.get() + (1 - bank) * kAccBank;
+    a.slots = slots_.get();
     a.ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
     a.max_iter = params.pcg_max_iter;
     a.tol2 = params.pcg_tol * params.pcg_tol;
     a.floor2 = params.pcg_floor * params.pcg_floor;
-    a.skip = finish == 2 ? &a.ctl->outer_done : nullptr;
-    void* args[] = {&a};
+    a.skip = skip;
+    const dim3 grid(ceil_div(nv_, kTpb)), block(kTpb);
+    const int max_iter = params.pcg_max_iter;
     tick(kKPcg, true);
-    SBS_HIP(hipLaunchCooperativeKernel((const void*)pcg_persistent_kernel<T>, dim3(pcg_grid_), dim3(kPcgTpb), args, 0,
-                                       stream_));
+    hipLaunchKernelGGL(pcg_init_kernel<T>, grid, block, 0, stream_, a);
+    auto iteration = [&](int k) {
+        tick(kKPcgA, true);
+        hipLaunchKernelGGL(pcg_a_kernel<T>, grid, block, 0, stream_, a, k, 0);
+        tick(kKPcgA, false);
+        tick(kKPcgB, true);
+        hipLaunchKernelGGL(pcg_b_kernel<T>, grid, block, 0, stream_, a, k);
+        tick(kKPcgB, false);
+    };
+    const bool learned = !exact && solve_index >= 0 && solve_index < (int)cap_.size();
+    int L = 0;
+    if (learned && cap_[solve_index] > 0) {
+        L = std::min(cap_[solve_index], max_iter);
+        for (int k = 0; k < L; ++k) iteration(k);
+    } else {
+        const CtlBlock* hc = reinterpret_cast<const CtlBlock*>(host_ctl_);
+        while (L < max_iter) {
+            const int n = std::min(kCheckChunk, max_iter - L);
+            for (int j = 0; j < n; ++j) iteration(L++);
+            SBS_HIP(hipMemcpyAsync(host_ctl_, a.ctl, sizeof(CtlBlock), hipMemcpyDeviceToHost, stream_));
+            SBS_HIP(hipStreamSynchronize(stream_));
+            // st[(L-1)&1] is the state A(L-1) saw: done there means no later iteration ran
+            if ((skip && hc->outer_done) || hc->st[(L - 1) & 1].done) break;
+        }
+        if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L));
+    }
+    tick(kKPcgA, true);
+    hipLaunchKernelGGL(pcg_a_kernel<T>, grid, block, 0, stream_, a, L, 1);
+    tick(kKPcgA, false);
+    hipLaunchKernelGGL(pcg_finish_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
     tick(kKPcg, false);
-    ++pcg_launches_;
+    SBS_HIP(hipGetLastError());
+    return L;
 }
+
+// Iterations of headroom over the last observed count of a solve.
+template <class T> int PdEngine<T>::cap_margin(int used) { return 2 + used / 8; }
 
 template <class T> void PdEngine<T>::fold_slot(int slot) {
     if (!step_pending_[slot]) return;
     SBS_HIP(hipEventSynchronize(step_event_[slot]));
     step_pending_[slot] = false;
-    last_pcg_iters_ = host_counters_[4 * slot];
-    unconverged_total_ += host_counters_[4 * slot + 1];
-    pcg_iters_total_ += host_counters_[4 * slot];
-    pcg_solves_total_ += host_counters_[4 * slot + 3];
+    const int* c = host_counters_ + kCounterInts * slot;
+    last_pcg_iters_ = c[0];
+    unconverged_total_ += c[1];
+    pcg_iters_total_ += c[0];
+    pcg_solves_total_ += c[3];
+    // learned launch caps: follow the larger of the last two observed counts of
+    // each solve index; a solve cut by its cap doubles that cap
+    if (step_caps_[slot] != cap_epoch_) return;  // counters of other settings
+    const int* log = c + 6;
+    const int nsol = std::min(std::min(c[3], kMaxSolveLog), (int)cap_.size());
+    for (int i = 0; i < nsol; ++i) {
+        const int v = log[i];
+        if (v < 0) {
+            const int its = -v - 1;
+            if (its < cap_max_iter_) cap_[i] = std::min(cap_max_iter_, std::max(2 * cap_[i], its + 8));
+            seen_[i] = std::max(seen_[i], its);
+        } else {
+            cap_[i] = std::min(cap_max_iter_, std::max(v, seen_[i]) + cap_margin(std::max(v, seen_[i])));
+            seen_[i] = v;
+        }
+    }
 }
 
 template <class T> void PdEngine<T>::sync_counters() {
@@ -1041,6 +1088,8 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     const bool jacobi = solver_ == GlobalSolver::Jacobi;
     const bool pcg = solver_ == GlobalSolver::PCG;
     const bool cheb = jacobi || (pcg && params.pcg_chebyshev);
+    if (pcg) reset_caps_if_changed(params);
+    step_caps_[slot] = cap_epoch_;
     const RhsGather<T> g{c_.get(), sn_old_.get(), vent_ptr_.get(), vent_.get(), partial_f_.get(), data.DBC,
                          data.moreDBC, data.DBCX, pin_weight<T>(params.dt)};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -1069,9 +1118,15 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
                                sn_.get(), prev_.get(), omega);
             tick(kKJacobi, false);
         } else if (pcg) {
-            // PCG warm-started from the current iterate; Chebyshev recurrence
-            // or computeError + early exit at the end of the same launch
-            launch_pcg(1, nullptr, sn_.get(), next_.get(), cheb ? 1 : 2, omega, params, params.tol);
+            // PCG warm-started from the current iterate, then the Chebyshev
+            // recurrence, or computeError and the early exit
+            launch_pcg(1, nullptr, sn_.get(), next_.get(), cheb, omega, params, skip, i, !params.pcg_adaptive);
+            if (!cheb) {
+                hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_,
+                                   (const T*)next_.get(), sn_.get(), prev_.get(), slots_.get(), skip);
+                hipLaunchKernelGGL(outer_decide_kernel, dim3(1), dim3(64), 0, stream_, (const double*)slots_.get(),
+                                   nv_, params.tol, ctl, skip);
+            }
         } else {
             tick(kKRhs, true);
             hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, rhs_.get(), skip);
@@ -1080,7 +1135,9 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
             direct_solve(rhs_.get(), sn_.get());
             tick(kKDirect, false);
             hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(),
-                               prev_.get(), red_partial_.get(), ctl, params.tol);
+                               (T*)nullptr, prev_.get(), slots_.get(), skip);
+            hipLaunchKernelGGL(outer_decide_kernel, dim3(1), dim3(64), 0, stream_, (const double*)slots_.get(), nv_,
+                               params.tol, ctl, skip);
         }
         SBS_HIP(hipGetLastError());
         if (perf_) {
@@ -1096,9 +1153,10 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     hipLaunchKernelGGL(end_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(), (T)1 / dt,
                        data.XTilde, data.V, data.X, data.moreDBC);
     SBS_HIP(hipGetLastError());
-    // counters: PCG iterations of the step, unconverged solves, max per solve, solves
-    SBS_HIP(hipMemcpyAsync(host_counters_ + 4 * slot, &ctl->pcg_iters, 4 * sizeof(int), hipMemcpyDeviceToHost,
-                           stream_));
+    // counters of the step (PCG iterations, unconverged solves, max per solve,
+    // solves, last, per-solve log) land in this step's pinned slot
+    SBS_HIP(hipMemcpyAsync(host_counters_ + kCounterInts * slot, &ctl->pcg_iters, kCounterInts * sizeof(int),
+                           hipMemcpyDeviceToHost, stream_));
     SBS_HIP(hipEventRecord(step_event_[slot], stream_));
     step_pending_[slot] = true;
     ++steps_;
@@ -1107,6 +1165,28 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
         for (auto& e : ev) (void)hipEventDestroy(e);
     }
     last_iters_ = params.num_iterations;
+}
+
+// Learned launch caps are per solve index and valid for one set of PCG settings.
+template <class T> void PdEngine<T>::reset_caps_if_changed(const PdParams& p) {
+    const bool same = (int)cap_.size() == p.num_iterations && cap_max_iter_ == p.pcg_max_iter &&
+                      cap_tol_ == p.pcg_tol && cap_floor_ == p.pcg_floor && cap_cheb_ == p.pcg_chebyshev &&
+                      cap_dt_ == p.dt;
+    if (same) return;
+    cap_.assign(p.num_iterations, 0);
+    seen_.assign(p.num_iterations, 0);
+    cap_max_iter_ = p.pcg_max_iter;
+    cap_tol_ = p.pcg_tol;
+    cap_floor_ = p.pcg_floor;
+    cap_cheb_ = p.pcg_chebyshev;
+    cap_dt_ = p.dt;
+    ++cap_epoch_;
+}
+
+template <class T> int PdEngine<T>::pcg_cap() const {
+    int m = 0;
+    for (int c : cap_) m = std::max(m, c);
+    return m;
 }
 
 template <class T> void PdEngine<T>::local_only(const T* q, const T* sn_old, bool residual_form, double dt, T* b_out) {
@@ -1121,6 +1201,7 @@ template <class T> void PdEngine<T>::local_only(const T* q, const T* sn_old, boo
 
 template <class T> int PdEngine<T>::solve(const T* b, T* x, const PdParams& params, bool use_guess) {
     SBS_REQUIRE(ready_, "solve: engine not prepared");
+    SBS_REQUIRE(params.pcg_max_iter >= 1, "solve: pcg_max_iter must be >= 1");
     sync_counters();
     CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
     hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl);
@@ -1129,17 +1210,20 @@ template <class T> int PdEngine<T>::solve(const T* b, T* x, const PdParams& para
         SBS_HIP(hipStreamSynchronize(stream_));
         return 0;
     }
-    // x0 = x (the caller's guess) or 0; the init phase reads x0 at every row
-    // before the grid barrier that precedes any write to x
+    // x0 = the caller's guess (copied: x is written while x0 is still read) or 0
     SBS_REQUIRE(!use_guess || x != nullptr, "solve: guess requested without x");
     if (use_guess) {
         SBS_HIP(hipMemcpyAsync(next_.get(), x, 3 * (size_t)nv_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
     }
-    launch_pcg(0, b, use_guess ? next_.get() : nullptr, x, 0, (T)1, params, 0.0);
-    SBS_HIP(hipMemcpyAsync(host_counters_ + 8, &ctl->unconverged, 5 * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, stream_, slots_.get(), kNumSlots * kSlotSize);
+    launch_pcg(0, b, use_guess ? next_.get() : nullptr, x, false, (T)1, params, nullptr, -1, true);
+    SBS_HIP(hipMemcpyAsync(host_ctl_, ctl, sizeof(CtlBlock), hipMemcpyDeviceToHost, stream_));
     SBS_HIP(hipStreamSynchronize(stream_));
-    unconverged_total_ += host_counters_[8];
-    return host_counters_[12];  // last_iters
+    const CtlBlock* hc = reinterpret_cast<const CtlBlock*>(host_ctl_);
+    unconverged_total_ += hc->unconverged;
+    pcg_iters_total_ += hc->pcg_iters;
+    pcg_solves_total_ += hc->solves;
+    return hc->last_iters;
 }
 
 template <class T> typename PdEngine<T>::Export PdEngine<T>::export_system() const {

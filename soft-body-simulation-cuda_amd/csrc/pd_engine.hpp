@@ -81,6 +81,8 @@ enum KernelId : int {
     kKJacobi,
     kKDirect,
     kKRhs,
+    kKPcgA,     // p update + SpMV + p.q (one launch per PCG iteration, plus the stop probe)
+    kKPcgB,     // x, r update + r.z, z.z
     kKCount
 };
 const char* kernel_name(int id);
@@ -91,6 +93,8 @@ struct KernelTime {
 };
 
 constexpr int kChunkTets = 256;  // tets per local-step chunk (one workgroup)
+constexpr int kMaxSolveLog = 64;  // per-step log of PCG iterations, by solve index
+constexpr int kCounterInts = 6 + kMaxSolveLog;  // step counters copied to the host
 
 template <class T> class PdEngine {
 public:
@@ -131,7 +135,7 @@ public:
     int num_tets() const { return nt_; }
     size_t nnz_offdiag() const { return nnz_off_; }
     bool prepared() const { return ready_; }
-    int pcg_cap() const { return pcg_grid_; }
+    int pcg_cap() const;  // largest learned PCG launch cap (iterations)
     long long pcg_cap_hits() const { return unconverged_total_; }
     long long pcg_iterations_total() const { return pcg_iters_total_; }
     long long pcg_solves_total() const { return pcg_solves_total_; }
@@ -145,8 +149,10 @@ private:
     void build_structure();
     void assemble(const PdParams& params);
     void launch_local(const T* q, bool residual_form, const int* skip);
-    void launch_pcg(int gathered, const T* b, const T* x0, T* x, int finish, T omega, const PdParams& params,
-                    double outer_tol);
+    int launch_pcg(int gathered, const T* b, const T* x0, T* x, bool cheb, T omega, const PdParams& params,
+                   const int* skip, int solve_index, bool exact);
+    void reset_caps_if_changed(const PdParams& p);
+    static int cap_margin(int used);
     void direct_factor();
     void direct_solve(const T* b, T* x);
     void tick(int id, bool start);
@@ -188,15 +194,20 @@ private:
     DevBuf<T> eval_;
     // PCG workspace and control
     DevBuf<T> r_, p0_, p1_, q_;
-    DevBuf<double> acc_;                // reduction accumulators (2 banks)
-    DevBuf<unsigned char> ctl_;
-    int pcg_grid_ = 0, pcg_launches_ = 0;
-    DevBuf<double> red_partial_;        // per-block partials of the small reductions
+    DevBuf<double> slots_;              // address-striped reduction slots
+    DevBuf<unsigned char> ctl_;         // device control block
+    void* host_ctl_ = nullptr;          // pinned copy of the control block (host checks)
+    // learned PCG launch caps per solve index, for one set of PCG settings
+    std::vector<int> cap_, seen_;
+    int cap_max_iter_ = 0, cap_cheb_ = -1;
+    double cap_tol_ = -1, cap_floor_ = -1, cap_dt_ = -1;
+    long long cap_epoch_ = 0;
+    long long step_caps_[2] = {-1, -1};  // cap epoch of each counter slot
     // direct solver (band Cholesky factor, GPU triangular solves) for small bodies
     std::unique_ptr<BandCholesky<T>> direct_;
     int last_iters_ = 0, last_pcg_iters_ = 0;
-    // pinned counters of the last two steps (slot = step parity):
-    // [pcg iterations, unconverged solves, max per solve, solves]; [8..15] scratch
+    // pinned counters of the last two steps (slot = step parity), kCounterInts
+    // each: pcg iterations, unconverged solves, max per solve, solves, last, pad, log
     int* host_counters_ = nullptr;
     hipEvent_t step_event_[2] = {nullptr, nullptr};
     bool step_pending_[2] = {false, false};
