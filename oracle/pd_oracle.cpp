@@ -36,7 +36,7 @@
 // Solver modes: 0 = Jacobi/Chebyshev (reference default), 1 = direct
 // (CuSolverCholesky: exact solve, error-based early exit), 2 = EigenCholesky
 // (same arithmetic on the host in the reference), 3 = Jacobi-PCG (the
-// north-star global step: direct-solve semantics with the solve replaced by
+// global step of the 64^3 workload: direct-solve semantics with the solve replaced by
 // Jacobi-preconditioned CG to a relative residual, optional Chebyshev).
 #include <algorithm>
 #include <cmath>

@@ -1,7 +1,8 @@
 """Pins the CPU oracle of the PD step against mathematics it does not share.
 
 Each check is chosen so a plausible slip in the restatement fails it:
-  * SVD / projection  -> numpy's LAPACK SVD, SO(3) membership, exact rotations;
+  * SVD / projection  -> the SVD known answer held by the reference's tests,
+                         numpy's LAPACK SVD, SO(3) membership, exact rotations;
   * Dm^-1, V0         -> Dm Dm^-1 = I and the volume of the closed surface
                          (divergence theorem over the extracted triangles);
   * assembly          -> an independent FEM assembly from barycentric gradients
@@ -12,6 +13,8 @@ Each check is chosen so a plausible slip in the restatement fails it:
   * global step       -> numpy dense solve; free fall closed form; agreement of
                          all solver modes at convergence of the PD iteration.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -82,6 +85,31 @@ def test_svd_matches_lapack_when_well_separated():
         rerr.append(np.abs(O.local_rotation(A) - _polar(A)).max())
     assert np.median(rerr) < 1e-11 and np.median(serr) < 1e-11
     assert np.percentile(rerr, 95) < 1e-6 and max(rerr) < 2e-3
+
+
+def test_svd_matches_reference_fixture():
+    """Known answer held by the reference's tests (tests/test_aabb.cpp:186-194):
+    F = U diag(sigma) V^T from its printed factors; the TR1690 SVD must give the
+    same sigma (descending) and the same singular-vector pairs up to a common
+    sign per pair, with U, V proper rotations."""
+    import json
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "svd_reference_fixture.json")))
+    U, V, sig = np.array(fx["U"]), np.array(fx["V"]), np.array(fx["sigma"])
+    F = U @ np.diag(sig) @ V.T
+    tol = 2e-4  # the fixture carries 4 decimals
+    for fp64 in (True, False):
+        u, s, v = O.svd3(F, fp64=fp64)
+        np.testing.assert_allclose(s, sig, atol=tol)
+        for i in range(3):
+            sgn = np.sign(u[:, i] @ U[:, i])
+            assert sgn == np.sign(v[:, i] @ V[:, i])
+            np.testing.assert_allclose(u[:, i], sgn * U[:, i], atol=tol)
+            np.testing.assert_allclose(v[:, i], sgn * V[:, i], atol=tol)
+        assert abs(np.linalg.det(u) - 1) < 1e-5 and abs(np.linalg.det(v) - 1) < 1e-5
+        np.testing.assert_allclose(O.local_rotation(F, fp64=fp64), U @ V.T, atol=tol)
+    # the fixture's Dm^-1 is the inverse of the edge matrix of a 0.1-scaled tet
+    Dm = np.linalg.inv(np.array(fx["DmInv"]))
+    np.testing.assert_allclose(Dm @ np.array(fx["DmInv"]), np.eye(3), atol=1e-12)
 
 
 def test_projection_of_rotation_is_itself():
