@@ -149,29 +149,31 @@ def main():
                    max_abs_over_max_displacement=float(np.abs(g["X"] - r["X"]).max() / disp),
                    reference="FP64 run of the same steps on the GPU (FP64 path parity-tested vs the FP64 oracle)")
 
+    roof = None
     kt, tm = g["kernel_times"], g["timing"]
-    iters = tm["pcg_iters"]
-    per_launch = {
-        "local_step": local_step_bytes(scene.nt, scene.nv),
-        # algorithmic bytes of the effective iterations spread over every launch
-        # (the stop probe and launches after convergence do no vector work)
-        "pcg_spmv": iters * pcg_spmv_bytes(g["nnz_off"], scene.nv) / max(kt["pcg_spmv"][0], 1),
-        "pcg_update": iters * pcg_update_bytes(scene.nv) / max(kt["pcg_update"][0], 1),
-    }
-    kinfo = {}
-    for k, (n, ms) in kt.items():
-        d = dict(launches=n, total_ms=round(ms, 3), avg_us=round(ms / max(n, 1) * 1e3, 2))
-        if k in per_launch:
-            d["bytes_per_launch"] = int(per_launch[k])
-            d["gbs"] = round(per_launch[k] / (ms / max(n, 1) * 1e-3) / 1e9, 1)
-        kinfo[k] = d
-    dom = max(per_launch, key=lambda k: kt[k][1])
-    ach = kinfo[dom]["gbs"]
-    roof = dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                algorithmic_bytes_per_launch=kinfo[dom]["bytes_per_launch"], kernels=kinfo,
-                timing_sample=f"{tm['steps']} extra steps after the timed region, HIP events around every launch, "
-                              f"{iters} PCG iterations in {tm['pcg_solves']} solves")
+    if tm is not None:
+        iters = tm["pcg_iters"]
+        per_launch = {
+            "local_step": local_step_bytes(scene.nt, scene.nv),
+            # algorithmic bytes of the effective iterations spread over every
+            # launch (the stop probe and launches after convergence move nothing)
+            "pcg_spmv": iters * pcg_spmv_bytes(g["nnz_off"], scene.nv) / max(kt["pcg_spmv"][0], 1),
+            "pcg_update": iters * pcg_update_bytes(scene.nv) / max(kt["pcg_update"][0], 1),
+        }
+        kinfo = {}
+        for k, (n, ms) in kt.items():
+            d = dict(launches=n, total_ms=round(ms, 3), avg_us=round(ms / max(n, 1) * 1e3, 2))
+            if k in per_launch:
+                d["bytes_per_launch"] = int(per_launch[k])
+                d["gbs"] = round(per_launch[k] / (ms / max(n, 1) * 1e-3) / 1e9, 1)
+            kinfo[k] = d
+        dom = max(per_launch, key=lambda k: kt[k][1])
+        ach = kinfo[dom]["gbs"]
+        roof = dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    algorithmic_bytes_per_launch=kinfo[dom]["bytes_per_launch"], kernels=kinfo,
+                    timing_sample=f"{tm['steps']} extra steps after the timed region, HIP events around every "
+                                  f"launch, {iters} PCG iterations in {tm['pcg_solves']} solves")
 
     cpu = None
     if not args.no_cpu:

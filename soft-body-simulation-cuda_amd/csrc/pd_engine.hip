@@ -346,6 +346,8 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
 // Right-hand side of one vertex: addM_h2Sn (pdUtil.cu:168-178) plus the
 // local-step partials of every chunk touching it, with the pinned overwrite
 // of computeDBCLocal (pdUtil.cu:147-166).
+constexpr int kGatherBatch = 4;  // vertex->entry list entries loaded together
+
 template <class T> struct RhsGather {
     const T* c;
     const T* sn_old;
@@ -357,29 +359,31 @@ template <class T> struct RhsGather {
     const T* dbcx;
     T wdbc;
 
+    // All loads are issued up front (the entry list kGatherBatch at a time,
+    // clamped in bounds) so a vertex costs few dependent memory round trips.
     __device__ __forceinline__ void at(unsigned v, T& bx, T& by, T& bz) const {
         const T pin = dbc[v];
         const T more = more_dbc ? more_dbc[v] : (T)0;
-        if (pin > (T)0) {
-            ld3(dbcx, v, bx, by, bz);
-            bx *= wdbc; by *= wdbc; bz *= wdbc;
-            return;
-        }
-        if (more > (T)0) {
-            ld3(dbcx, v, bx, by, bz);
-            bx *= more; by *= more; bz *= more;
-            return;
-        }
+        const int e0 = vent_ptr[v], e1 = vent_ptr[v + 1];
         T sx, sy, sz;
         ld3(sn_old, v, sx, sy, sz);
         const T cv = c[v];
         bx = cv * sx; by = cv * sy; bz = cv * sz;
-        const int e1 = vent_ptr[v + 1];
-        for (int e = vent_ptr[v]; e < e1; ++e) {
-            const uint32_t k = vent[e];
-            T px, py, pz;
-            ld3(partial, k, px, py, pz);
-            bx += px; by += py; bz += pz;
+        for (int e = e0; e < e1; e += kGatherBatch) {
+            uint32_t k[kGatherBatch];
+#pragma unroll
+            for (int j = 0; j < kGatherBatch; ++j) k[j] = vent[min(e + j, e1 - 1)];
+#pragma unroll
+            for (int j = 0; j < kGatherBatch; ++j) {
+                T px, py, pz;
+                ld3(partial, k[j], px, py, pz);
+                if (e + j < e1) { bx += px; by += py; bz += pz; }
+            }
+        }
+        if (pin > (T)0 || more > (T)0) {
+            const T wgt = pin > (T)0 ? wdbc : more;
+            ld3(dbcx, v, bx, by, bz);
+            bx *= wgt; by *= wgt; bz *= wgt;
         }
     }
 };
@@ -479,6 +483,36 @@ __global__ void __launch_bounds__(kTpb) outer_error_kernel(int nv, const T* __re
 // (Chebyshev recurrence, or copy into sn with computeError).  Launches after
 // convergence return at once.  A component stops when
 // ||z|| <= max(tol ||z_0||, floor ||D^-1 b||).
+// Off-diagonal entries of row v of the sliced-ELL matrix, issued kEllBatch at
+// a time: column and value loads of a batch are independent, then the batch's
+// gathers are independent, so a row costs width/kEllBatch dependent rounds of
+// memory latency instead of width.  Positions past the row's width are
+// clamped to its last entry (in bounds) and given the value 0, so every load
+// is unconditional.  f(u, m) accumulates m * x[u] in entry order.
+constexpr int kEllBatch = 8;
+template <class T, class F>
+__device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const int* __restrict__ slice_w,
+                                        const uint32_t* __restrict__ ecol, const T* __restrict__ eval, unsigned v,
+                                        F&& f) {
+    const unsigned s = v / kSlice, lane = v % kSlice;
+    const size_t base = (size_t)slice_ptr[s] + lane;
+    const int width = slice_w[s];
+    for (int e0 = 0; e0 < width; e0 += kEllBatch) {
+        uint32_t u[kEllBatch];
+        T m[kEllBatch];
+#pragma unroll
+        for (int j = 0; j < kEllBatch; ++j) {
+            const int e = min(e0 + j, width - 1);
+            const size_t pos = base + (size_t)e * kSlice;
+            u[j] = ecol[pos];
+            const T mv = eval[pos];
+            m[j] = e0 + j < width ? mv : (T)0;
+        }
+#pragma unroll
+        for (int j = 0; j < kEllBatch; ++j) f(u[j], m[j]);
+    }
+}
+
 template <class T> struct PcgArgs {
     int nv;
     int gathered;             // 1: rhs from the local-step partials (+ pins), 0: plain b
@@ -521,17 +555,11 @@ __global__ void __launch_bounds__(kTpb) pcg_init_kernel(PcgArgs<T> a) {
             ld3(a.x0, v, x0, x1, x2);
             const T d = a.diagA[v];
             ax0 = d * x0; ax1 = d * x1; ax2 = d * x2;
-            const unsigned s = v / kSlice, lane = v % kSlice;
-            const size_t base = (size_t)a.slice_ptr[s] + lane;
-            const int width = a.slice_w[s];
-            for (int e = 0; e < width; ++e) {
-                const size_t pos = base + (size_t)e * kSlice;
-                const uint32_t u = a.ecol[pos];
-                const T m = a.eval[pos];
+            ell_row(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t u, T m) {
                 T u0, u1, u2;
                 ld3(a.x0, u, u0, u1, u2);
                 ax0 += m * u0; ax1 += m * u1; ax2 += m * u2;
-            }
+            });
         }
         st3(a.x, v, x0, x1, x2);
         const T rr0 = b0 - ax0, rr1 = b1 - ax1, rr2 = b2 - ax2;
@@ -611,13 +639,7 @@ __global__ void __launch_bounds__(kTpb) pcg_a_kernel(PcgArgs<T> a, int k, int pr
         st3(pout, v, p0, p1, p2);
         const T d = a.diagA[v];
         T y0 = d * p0, y1 = d * p1, y2 = d * p2;
-        const unsigned s = v / kSlice, lane = v % kSlice;
-        const size_t base = (size_t)a.slice_ptr[s] + lane;
-        const int width = a.slice_w[s];
-        for (int e = 0; e < width; ++e) {
-            const size_t pos = base + (size_t)e * kSlice;
-            const uint32_t u = a.ecol[pos];
-            const T m = a.eval[pos];
+        ell_row(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t u, T m) {
             T ru0, ru1, ru2, pu0, pu1, pu2;
             ld3(a.r, u, ru0, ru1, ru2);
             ld3(pin, u, pu0, pu1, pu2);
@@ -625,7 +647,7 @@ __global__ void __launch_bounds__(kTpb) pcg_a_kernel(PcgArgs<T> a, int k, int pr
             y0 += m * p_next(du, ru0, be0, pu0);
             y1 += m * p_next(du, ru1, be1, pu1);
             y2 += m * p_next(du, ru2, be2, pu2);
-        }
+        });
         st3(a.q, v, y0, y1, y2);
         vals[0] = (double)p0 * y0; vals[1] = (double)p1 * y1; vals[2] = (double)p2 * y2;
     }
