@@ -282,31 +282,44 @@ __global__ void begin_step_kernel(int nv, const T* __restrict__ X, const T* __re
 // ---------------------------------------------------------------------------
 // computeLocal (src/simulation/solver/projective/pdUtil.cu:97-145), one
 // workgroup per chunk of kChunkTets Morton-ordered tets:
-//   1. the chunk's vertices (a sorted list) are staged from q into LDS;
+//   1. the chunk's vertices (a sorted list) are staged from q into LDS, with
+//      the chunk's incidence list (per vertex, the (tet, corner) pairs that
+//      touch it, in tet order);
 //   2. one thread per tet: F = Ds Dm^-1, 3x3 SVD in registers, R (or R - F),
-//      force columns w R Dm^-T G^T added into LDS accumulators (LDS atomics);
-//   3. the chunk writes one partial force per vertex it touches (coalesced,
-//      no global atomics).
+//      the 4 corner forces of w R Dm^-T G^T written to LDS (plain stores);
+//   3. one thread per (vertex, component) sums its incidences in list order
+//      and writes the chunk's partial force for that vertex (coalesced).
+// No atomics anywhere, so the result is deterministic; float atomics on LDS
+// serialise on the shared corners of neighbouring tets and were the
+// bottleneck of the scatter form.
 template <class T>
 __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
     int nt, const uint2* __restrict__ lidx, const int* __restrict__ chunk_vptr,
-    const uint32_t* __restrict__ chunk_verts, const T* __restrict__ dminv, const T* __restrict__ w,
+    const uint32_t* __restrict__ chunk_verts, const uint16_t* __restrict__ inc_codes,
+    const uint16_t* __restrict__ inc_end, const T* __restrict__ dminv, const T* __restrict__ w,
     const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip,
     double* __restrict__ zero_slots, int zero_n) {
     if (skip && *skip) return;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     if (blockIdx.x == 0 && zero_slots)  // the PCG solve that follows accumulates into these
         for (int i = threadIdx.x; i < zero_n; i += kChunkTets) zero_slots[i] = 0.0;
     const unsigned c = xcd_block(blockIdx.x, gridDim.x);
     const int e0 = chunk_vptr[c], nu = chunk_vptr[c + 1] - e0;
-    T* qs = reinterpret_cast<T*>(smem_raw);
-    T* as = qs + 3 * (size_t)nu;
+    T* fs = reinterpret_cast<T*>(smem_raw);                    // [12][kChunkTets] corner forces
+    T* qs = fs + 12 * kChunkTets;                              // [nu][3] staged q
+    uint16_t* codes = reinterpret_cast<uint16_t*>(qs + 3 * (size_t)nu);  // [4 kChunkTets] tet<<2 | corner
+    uint16_t* ends = codes + 4 * kChunkTets;                   // [nu] end of each vertex's incidences
     for (int i = threadIdx.x; i < nu; i += kChunkTets) {
         const uint32_t g = chunk_verts[e0 + i];
         T x, y, z;
         ld3(q, g, x, y, z);
         qs[3 * i] = x; qs[3 * i + 1] = y; qs[3 * i + 2] = z;
-        as[3 * i] = (T)0; as[3 * i + 1] = (T)0; as[3 * i + 2] = (T)0;
+        ends[i] = inc_end[e0 + i];
+    }
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(inc_codes) + (size_t)c * (2 * kChunkTets);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(codes);
+        for (int k = threadIdx.x; k < 2 * kChunkTets; k += kChunkTets) dst[k] = src[k];
     }
     __syncthreads();
     const unsigned t = c * kChunkTets + threadIdx.x;
@@ -328,19 +341,28 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
         pd_projection(F, residual_form != 0, P);
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            // column j of w P Dm^-T is w P (row j of Dm^-1); vertex 0 gets minus their sum
+            // column j of w P Dm^-T is w P (row j of Dm^-1); corner 0 gets minus their sum
             const T f1 = ww * (P[r][0] * m[0] + P[r][1] * m[1] + P[r][2] * m[2]);
             const T f2 = ww * (P[r][0] * m[3] + P[r][1] * m[4] + P[r][2] * m[5]);
             const T f3 = ww * (P[r][0] * m[6] + P[r][1] * m[7] + P[r][2] * m[8]);
-            atomicAdd(&as[3 * l0 + r], -(f1 + f2 + f3));
-            atomicAdd(&as[3 * l1 + r], f1);
-            atomicAdd(&as[3 * l2 + r], f2);
-            atomicAdd(&as[3 * l3 + r], f3);
+            fs[(0 * 3 + r) * kChunkTets + threadIdx.x] = -(f1 + f2 + f3);
+            fs[(1 * 3 + r) * kChunkTets + threadIdx.x] = f1;
+            fs[(2 * 3 + r) * kChunkTets + threadIdx.x] = f2;
+            fs[(3 * 3 + r) * kChunkTets + threadIdx.x] = f3;
         }
     }
     __syncthreads();
     T* out = partial + 3 * (size_t)e0;
-    for (int i = threadIdx.x; i < 3 * nu; i += kChunkTets) out[i] = as[i];
+    for (int wi = threadIdx.x; wi < 3 * nu; wi += kChunkTets) {
+        const int i = wi / 3, r = wi - 3 * i;
+        const int k1 = ends[i];
+        T acc = 0;
+        for (int k = i ? ends[i - 1] : 0; k < k1; ++k) {
+            const unsigned code = codes[k];
+            acc += fs[((code & 3u) * 3 + r) * kChunkTets + (code >> 2)];
+        }
+        out[wi] = acc;
+    }
 }
 
 // Right-hand side of one vertex: addM_h2Sn (pdUtil.cu:168-178) plus the
@@ -835,6 +857,10 @@ template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h
     cverts.reserve((size_t)nt_ * 2);
     std::vector<uint2> lidx(nt_);
     std::vector<uint32_t> tmp;
+    std::vector<int> cnt(4 * kChunkTets);
+    std::vector<uint16_t> inc_end;
+    inc_end.reserve((size_t)nt_ * 2);
+    std::vector<uint16_t> inc_codes((size_t)((nt_ + kChunkTets - 1) / kChunkTets) * 4 * kChunkTets, 0);
     chunk_max_verts_ = 0;
     for (int c = 0; c < nchunks_; ++c) {
         const int t0 = c * kChunkTets, t1 = std::min(nt_, t0 + kChunkTets);
@@ -844,14 +870,28 @@ template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h
         std::sort(tmp.begin(), tmp.end());
         tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
         chunk_max_verts_ = std::max(chunk_max_verts_, (int)tmp.size());
+        std::fill(cnt.begin(), cnt.end(), 0);
         for (int i = t0; i < t1; ++i) {
             uint32_t l[4];
             for (int j = 0; j < 4; ++j) {
                 const uint32_t g = host_tets_[4 * (size_t)tet_perm_[i] + j];
                 l[j] = (uint32_t)(std::lower_bound(tmp.begin(), tmp.end(), g) - tmp.begin());
+                cnt[l[j]]++;
             }
             lidx[i] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
         }
+        // incidence list of the chunk: per local vertex, (tet << 2 | corner) in tet order
+        const size_t nloc = tmp.size();
+        std::vector<int> pos(nloc + 1, 0);
+        for (size_t k = 0; k < nloc; ++k) pos[k + 1] = pos[k] + cnt[k];
+        for (size_t k = 0; k < nloc; ++k) inc_end.push_back((uint16_t)pos[k + 1]);
+        uint16_t* codes = &inc_codes[(size_t)c * 4 * kChunkTets];
+        for (int i = t0; i < t1; ++i)
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t word = j < 2 ? lidx[i].x : lidx[i].y;
+                const uint32_t lv = (j & 1) ? word >> 16 : word & 0xffffu;
+                codes[pos[lv]++] = (uint16_t)(((i - t0) << 2) | j);
+            }
         cverts.insert(cverts.end(), tmp.begin(), tmp.end());
         cvptr[c + 1] = (int)cverts.size();
     }
@@ -870,6 +910,10 @@ template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h
     vent_ptr_.alloc(nv_ + 1);
     vent_.alloc(nentries_);
     partial_f_.alloc(3 * nentries_);
+    inc_codes_.alloc(inc_codes.size());
+    inc_end_.alloc(std::max<size_t>(1, inc_end.size()));
+    SBS_HIP(hipMemcpy(inc_codes_.get(), inc_codes.data(), inc_codes.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(inc_end_.get(), inc_end.data(), inc_end.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(tet_perm_d_.get(), tet_perm_.data(), nt_ * sizeof(int), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(tet_lidx_.get(), lidx.data(), nt_ * sizeof(uint2), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(chunk_vptr_.get(), cvptr.data(), cvptr.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -974,14 +1018,21 @@ template <class T> void PdEngine<T>::direct_solve(const T* b, T* x) {
     direct_->solve(b, x, stream_, &reinterpret_cast<CtlBlock*>(ctl_.get())->outer_done);
 }
 
+// LDS of one local-step workgroup: corner forces, staged q, incidence list.
+template <class T> size_t PdEngine<T>::local_lds_bytes() const {
+    const size_t b = (12 * (size_t)kChunkTets + 3 * (size_t)chunk_max_verts_) * sizeof(T) +
+                     (4 * (size_t)kChunkTets + (size_t)chunk_max_verts_) * sizeof(uint16_t);
+    return (b + 15) & ~(size_t)15;
+}
+
 template <class T> void PdEngine<T>::launch_local(const T* q, bool residual_form, const int* skip) {
-    const size_t lds = 6 * (size_t)chunk_max_verts_ * sizeof(T);
+    const size_t lds = local_lds_bytes();
     SBS_REQUIRE(lds <= 64 * 1024, "local step: chunk vertex list exceeds LDS");
     tick(kKLocal, true);
     // block 0 also clears the reduction slots of the global step that follows
     hipLaunchKernelGGL(local_chunk_kernel<T>, dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_, tet_lidx_.get(),
-                       chunk_vptr_.get(), chunk_verts_.get(), dminv_.get(), w_.get(), q, partial_f_.get(),
-                       (int)residual_form, skip, slots_.get(), kNumSlots * kSlotSize);
+                       chunk_vptr_.get(), chunk_verts_.get(), inc_codes_.get(), inc_end_.get(), dminv_.get(), w_.get(),
+                       q, partial_f_.get(), (int)residual_form, skip, slots_.get(), kNumSlots * kSlotSize);
     tick(kKLocal, false);
     SBS_HIP(hipGetLastError());
 }

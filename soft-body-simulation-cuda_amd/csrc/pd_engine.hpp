@@ -149,6 +149,7 @@ private:
     void build_structure();
     void assemble(const PdParams& params);
     void launch_local(const T* q, bool residual_form, const int* skip);
+    size_t local_lds_bytes() const;
     int launch_pcg(int gathered, const T* b, const T* x0, T* x, bool cheb, T omega, const PdParams& params,
                    const int* skip, int solve_index, bool exact);
     void reset_caps_if_changed(const PdParams& p);
@@ -180,6 +181,8 @@ private:
     DevBuf<uint32_t> vent_;             // entry ids grouped by vertex
     size_t nentries_ = 0;
     DevBuf<T> partial_f_;               // 3 scalars per entry: local-step partial forces
+    DevBuf<uint16_t> inc_codes_;        // per chunk 4*kChunkTets: (tet << 2 | corner) grouped by vertex
+    DevBuf<uint16_t> inc_end_;          // per entry: end of its incidences in the chunk's list
     DevBuf<T> dminv_;                   // 9 planes of N_t
     DevBuf<T> w_;                       // |V0| * mu
     DevBuf<T> v0_;                      // V0
