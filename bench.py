@@ -47,15 +47,10 @@ def local_step_bytes(nt: int, nv: int) -> int:
     return 56 * nt + 40 * nv
 
 
-def pcg_spmv_bytes(nnz_off: int, nv: int) -> int:
-    # one PCG iteration's first kernel: off-diagonal values + u32 columns, the
-    # diagonal and D^-1, read r and p_old, write p and q (12-byte vertex vectors)
-    return 8 * nnz_off + 56 * nv
-
-
-def pcg_update_bytes(nv: int) -> int:
-    # second kernel: read x, r, p, q and D^-1, write x and r
-    return 76 * nv
+def pcg_iteration_bytes(nnz_off: int, nv: int) -> int:
+    # one PCG iteration (one launch): off-diagonal values + u32 columns, the
+    # diagonal and D^-1, read and write x, p, r, w, s (12-byte vertex vectors)
+    return 8 * nnz_off + 128 * nv
 
 
 def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int):
@@ -78,7 +73,7 @@ def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int):
     if timing_steps > 0:
         # kernel durations from HIP events on the solver's stream, in extra
         # steps after the timed region (events between launches add gaps)
-        names = ("local_step", "pcg_spmv", "pcg_update", "pcg_solve")
+        names = ("local_step", "pcg_iteration", "pcg_solve")
         pd.set_kernel_timing(names)
         for _ in range(timing_steps):
             pd.Update(data, p)
@@ -157,8 +152,7 @@ def main():
             "local_step": local_step_bytes(scene.nt, scene.nv),
             # algorithmic bytes of the effective iterations spread over every
             # launch (the stop probe and launches after convergence move nothing)
-            "pcg_spmv": iters * pcg_spmv_bytes(g["nnz_off"], scene.nv) / max(kt["pcg_spmv"][0], 1),
-            "pcg_update": iters * pcg_update_bytes(scene.nv) / max(kt["pcg_update"][0], 1),
+            "pcg_iteration": iters * pcg_iteration_bytes(g["nnz_off"], scene.nv) / max(kt["pcg_iteration"][0], 1),
         }
         kinfo = {}
         for k, (n, ms) in kt.items():

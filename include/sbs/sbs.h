@@ -125,9 +125,8 @@ int sbs_pd_stats(SbsPd* pd, SbsPdStats* out);
 #define SBS_KERNEL_JACOBI_CHEBYSHEV 2
 #define SBS_KERNEL_DIRECT_SOLVE 3
 #define SBS_KERNEL_RHS_GATHER 4
-#define SBS_KERNEL_PCG_SPMV 5          /* p update + A p + p.q: one per PCG iteration */
-#define SBS_KERNEL_PCG_UPDATE 6        /* x, r update + r.z, z.z                     */
-#define SBS_KERNEL_COUNT 7
+#define SBS_KERNEL_PCG_ITERATION 5     /* one launch per PCG iteration (+ stop probe) */
+#define SBS_KERNEL_COUNT 6
 int sbs_pd_set_kernel_timing(SbsPd* pd, unsigned mask);
 int sbs_pd_kernel_time(SbsPd* pd, int kernel_id, const char** name, int* launches, double* ms);
 

@@ -19,11 +19,10 @@ constexpr int kStripes = 32;   // address stripes of each reduction slot
 constexpr int kSlotQ = 16;     // doubles per stripe
 constexpr int kSlotSize = kStripes * kSlotQ;
 // reduction slots (each kSlotSize doubles)
-constexpr int kSlotInit = 0;   // r.z, z.z, (D^-1 b)^2 per component
-constexpr int kSlotA = 1;      // 2 slots: p.q per component (by iteration parity)
-constexpr int kSlotB = 3;      // 2 slots: r.z, z.z per component (by iteration parity)
-constexpr int kSlotErr = 5;    // sum (prev - x)^2 of computeError
-constexpr int kNumSlots = 6;
+constexpr int kSlotInit = 0;   // (D^-1 b)^2 per component
+constexpr int kSlotIter = 1;   // 3 rotating slots: r.u, w.u, u.u per component
+constexpr int kSlotErr = 4;    // sum (prev - x)^2 of computeError
+constexpr int kNumSlots = 5;
 constexpr int kCheckChunk = 8;  // iterations between host checks of an unlearned solve
 
 // Pin weight of computeDBCLocal: positional_weight * dt2Inv with
@@ -87,10 +86,11 @@ __device__ __forceinline__ void slot_totals(const double* slot, double (&out)[NQ
 
 // ---------------------------------------------------------------------------
 // Device control block: per-step counters, the outer-loop early exit and
-// the PCG state (double-buffered by iteration parity: the A kernel of
-// iteration k writes st[k&1] from st[(k-1)&1] and the B slot of k-1).
+// the PCG state (double-buffered by iteration parity: launch i of a solve
+// writes st[i&1] from st[(i-1)&1] and the reduction of launch i-1).
 struct PcgState {
-    double rz[3];
+    double gamma[3];  // r.u of the current iterate
+    double alpha[3];
     double beta[3];
     double thr[3];
     int active[3];
@@ -490,21 +490,6 @@ __global__ void __launch_bounds__(kTpb) outer_error_kernel(int nv, const T* __re
     stripe_add<1>(vals, slots + kSlotErr * kSlotSize);
 }
 
-// ---------------------------------------------------------------------------
-// Jacobi-PCG over the scalar matrix with 3 fused right-hand sides.
-//
-// Kernel boundaries are the grid-wide synchronisation (on MI355X a dependent
-// launch costs ~2.5-3.7 us, less than any in-kernel grid barrier measured).
-// Per solve:  init, then per iteration k
-//   A(k): state from st[(k-1)&1] + slot B[(k-1)&1] (beta, stop test) written
-//         to st[k&1]; p = D^-1 r + beta p_old (own rows; neighbours' p
-//         recomputed on the fly from r and p_old), q = A p, p.q -> slot A[k&1]
-//   B(k): alpha from st[k&1] + slot A[k&1]; x += alpha p, r -= alpha q,
-//         r.z, z.z -> slot B[k&1]; zeroes the slots of iteration k+1
-// then a probe A(L) that only evaluates the stop test, and a finish kernel
-// (Chebyshev recurrence, or copy into sn with computeError).  Launches after
-// convergence return at once.  A component stops when
-// ||z|| <= max(tol ||z_0||, floor ||D^-1 b||).
 // Off-diagonal entries of row v of the sliced-ELL matrix, issued kEllBatch at
 // a time: column and value loads of a batch are independent, then the batch's
 // gathers are independent, so a row costs width/kEllBatch dependent rounds of
@@ -512,18 +497,18 @@ __global__ void __launch_bounds__(kTpb) outer_error_kernel(int nv, const T* __re
 // clamped to its last entry (in bounds) and given the value 0, so every load
 // is unconditional.  f(u, m) accumulates m * x[u] in entry order.
 constexpr int kEllBatch = 8;
-template <class T, class F>
+template <int B = kEllBatch, class T, class F>
 __device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const int* __restrict__ slice_w,
                                         const uint32_t* __restrict__ ecol, const T* __restrict__ eval, unsigned v,
                                         F&& f) {
     const unsigned s = v / kSlice, lane = v % kSlice;
     const size_t base = (size_t)slice_ptr[s] + lane;
     const int width = slice_w[s];
-    for (int e0 = 0; e0 < width; e0 += kEllBatch) {
-        uint32_t u[kEllBatch];
-        T m[kEllBatch];
+    for (int e0 = 0; e0 < width; e0 += B) {
+        uint32_t u[B];
+        T m[B];
 #pragma unroll
-        for (int j = 0; j < kEllBatch; ++j) {
+        for (int j = 0; j < B; ++j) {
             const int e = min(e0 + j, width - 1);
             const size_t pos = base + (size_t)e * kSlice;
             u[j] = ecol[pos];
@@ -531,10 +516,35 @@ __device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const
             m[j] = e0 + j < width ? mv : (T)0;
         }
 #pragma unroll
-        for (int j = 0; j < kEllBatch; ++j) f(u[j], m[j]);
+        for (int j = 0; j < B; ++j) f(u[j], m[j]);
     }
 }
 
+// ---------------------------------------------------------------------------
+// Jacobi-PCG over the scalar matrix with 3 fused right-hand sides, in the
+// Chronopoulos-Gear form: both inner products of an iteration are taken in
+// one pass, so an iteration is ONE kernel launch (a kernel boundary is the
+// cheapest grid-wide synchronisation on MI355X: a dependent launch costs
+// ~2.5-3.7 us, less than any in-kernel grid barrier measured).
+//
+//   init1:  r0 = b - A x0, x = x0, (D^-1 b)^2
+//   init2:  u0 = D^-1 r0, w0 = A u0, gamma0 = r0.u0, delta0 = w0.u0, u0.u0
+//   K(i):   stop test on ||u_i|| and scalars (every block computes them from
+//           the reduction stripes of the previous launch; block 0 records them)
+//             beta_i  = gamma_i / gamma_{i-1}                     (beta_0 = 0)
+//             alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})
+//           then per row
+//             p_i = u_i + beta_i p_{i-1},  s_i = w_i + beta_i s_{i-1}
+//             x_{i+1} = x_i + alpha_i p_i, r_{i+1} = r_i - alpha_i s_i
+//             u_{i+1} = D^-1 r_{i+1},      w_{i+1} = A u_{i+1}
+//           and gamma_{i+1}, delta_{i+1}, u.u into the stripes.
+// The neighbours' u_{i+1} in A u_{i+1} are recomputed from their r_i, w_i,
+// s_{i-1} (double-buffered, read-only in K(i)) with the same operations the
+// owning row uses, so they are the same bits.  This is the reference's
+// Jacobi-PCG (pcgJacobi.cu:89-173) reorganised; in exact arithmetic the
+// iterates are identical.  A component stops when
+// ||u_k|| <= max(tol ||u_0||, floor ||D^-1 b||); launches after the stop
+// return at once.
 template <class T> struct PcgArgs {
     int nv;
     int gathered;             // 1: rhs from the local-step partials (+ pins), 0: plain b
@@ -548,10 +558,10 @@ template <class T> struct PcgArgs {
     const T* eval;
     const T* diagA;
     const T* dinv;
-    T* r;
-    T* p0;
-    T* p1;
-    T* q;
+    T* r[2];                  // r_i in r[i&1]
+    T* w[2];                  // w_i in w[i&1]
+    T* s[2];                  // s_{i-1} in s[i&1]
+    T* p;
     T* sn;
     T* prev;
     T omega;
@@ -563,11 +573,22 @@ template <class T> struct PcgArgs {
     const int* skip;          // outer loop already converged
 };
 
+__device__ __forceinline__ double* iter_slot(double* slots, int i) {
+    return slots + (kSlotIter + ((i % 3) + 3) % 3) * kSlotSize;
+}
+
+// s_i = w_i + beta s_{i-1} (w_i at i = 0), r_{i+1} = r_i - alpha s_i: one
+// explicit fma each, shared by the owning row and its neighbours.
+template <class T> __device__ __forceinline__ T cg_s(bool first, T beta, T s_old, T w) {
+    return first ? w : fma(beta, s_old, w);
+}
+template <class T> __device__ __forceinline__ T cg_r(T alpha, T s, T r) { return fma(-alpha, s, r); }
+
 template <class T>
-__global__ void __launch_bounds__(kTpb) pcg_init_kernel(PcgArgs<T> a) {
+__global__ void __launch_bounds__(kTpb) pcg_init1_kernel(PcgArgs<T> a) {
     if (a.skip && *a.skip) return;
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double vals[3] = {0, 0, 0};
     if (v < (unsigned)a.nv) {
         T b0, b1, b2;
         if (a.gathered) a.g.at(v, b0, b1, b2);
@@ -584,134 +605,155 @@ __global__ void __launch_bounds__(kTpb) pcg_init_kernel(PcgArgs<T> a) {
             });
         }
         st3(a.x, v, x0, x1, x2);
-        const T rr0 = b0 - ax0, rr1 = b1 - ax1, rr2 = b2 - ax2;
-        st3(a.r, v, rr0, rr1, rr2);
-        st3(a.p0, v, (T)0, (T)0, (T)0);
+        st3(a.r[0], v, b0 - ax0, b1 - ax1, b2 - ax2);
         const T di = a.dinv[v];
-        const T z0 = di * rr0, z1 = di * rr1, z2 = di * rr2;
-        vals[0] = (double)rr0 * z0; vals[1] = (double)rr1 * z1; vals[2] = (double)rr2 * z2;
-        vals[3] = (double)z0 * z0; vals[4] = (double)z1 * z1; vals[5] = (double)z2 * z2;
         const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
-        vals[6] = d0 * d0; vals[7] = d1 * d1; vals[8] = d2 * d2;
+        vals[0] = d0 * d0; vals[1] = d1 * d1; vals[2] = d2 * d2;
     }
-    stripe_add<9>(vals, a.slots + kSlotInit * kSlotSize);
+    stripe_add<3>(vals, a.slots + kSlotInit * kSlotSize);
 }
 
-// Stop test / beta of iteration k, identical in every block; block 0 records it.
-template <class T> __device__ __forceinline__ PcgState pcg_state(const PcgArgs<T>& a, int k) {
+template <class T>
+__global__ void __launch_bounds__(kTpb) pcg_init2_kernel(PcgArgs<T> a) {
+    if (a.skip && *a.skip) return;
+    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (v < (unsigned)a.nv) {
+        T r0, r1, r2;
+        ld3(a.r[0], v, r0, r1, r2);
+        const T dv = a.dinv[v], d = a.diagA[v];
+        const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
+        T w0 = d * u0, w1 = d * u1, w2 = d * u2;
+        ell_row(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t n, T m) {
+            T q0, q1, q2;
+            ld3(a.r[0], n, q0, q1, q2);
+            const T dn = a.dinv[n];
+            w0 += m * (dn * q0); w1 += m * (dn * q1); w2 += m * (dn * q2);
+        });
+        st3(a.w[0], v, w0, w1, w2);
+        vals[0] = (double)r0 * u0; vals[1] = (double)r1 * u1; vals[2] = (double)r2 * u2;
+        vals[3] = (double)w0 * u0; vals[4] = (double)w1 * u1; vals[5] = (double)w2 * u2;
+        vals[6] = (double)u0 * u0; vals[7] = (double)u1 * u1; vals[8] = (double)u2 * u2;
+    }
+    stripe_add<9>(vals, iter_slot(a.slots, -1));
+}
+
+// Stop test and scalars of iteration i, identical in every block; block 0
+// records them for the next launch.
+template <class T> __device__ __forceinline__ PcgState pcg_state(const PcgArgs<T>& a, int i) {
     PcgState st;
-    if (k == 0) {
-        double t[9];
-        slot_totals<9>(a.slots + kSlotInit * kSlotSize, t);
+    if (i == 0) {
+        double t[9], dd[3];
+        slot_totals<9>(iter_slot(a.slots, -1), t);
+        slot_totals<3>(a.slots + kSlotInit * kSlotSize, dd);
         bool any = false;
         for (int c = 0; c < 3; ++c) {
-            st.rz[c] = t[c];
+            st.thr[c] = fmax(a.tol2 * t[6 + c], a.floor2 * dd[c]);
+            const bool act = t[6 + c] > st.thr[c];
+            st.active[c] = act;
+            st.gamma[c] = t[c];
             st.beta[c] = 0.0;
-            st.thr[c] = fmax(a.tol2 * t[3 + c], a.floor2 * t[6 + c]);
-            st.active[c] = t[3 + c] > st.thr[c];
-            any |= st.active[c] != 0;
+            st.alpha[c] = (act && t[3 + c] != 0.0) ? t[c] / t[3 + c] : 0.0;
+            any |= act;
         }
         st.iters = 0;
         st.done = !any || a.max_iter <= 0;
         st.unconverged = any && a.max_iter <= 0;
     } else {
-        st = a.ctl->st[(k - 1) & 1];
+        st = a.ctl->st[(i - 1) & 1];
         if (!st.done) {
-            double t[6];
-            slot_totals<6>(a.slots + (kSlotB + ((k - 1) & 1)) * kSlotSize, t);
+            double t[9];
+            slot_totals<9>(iter_slot(a.slots, i - 1), t);
             bool any = false;
             for (int c = 0; c < 3; ++c) {
-                const bool act = st.active[c] != 0;
-                st.beta[c] = (act && st.rz[c] != 0.0) ? t[c] / st.rz[c] : 0.0;
-                st.rz[c] = t[c];
-                st.active[c] = act && t[3 + c] > st.thr[c];
-                any |= st.active[c] != 0;
+                const bool act = st.active[c] && t[6 + c] > st.thr[c];
+                double beta = 0.0, alpha = 0.0;
+                if (act && st.gamma[c] != 0.0) {
+                    beta = t[c] / st.gamma[c];
+                    const double den = t[3 + c] - beta * t[c] / st.alpha[c];
+                    alpha = den != 0.0 ? t[c] / den : 0.0;
+                }
+                st.active[c] = act;
+                st.beta[c] = beta;
+                st.alpha[c] = alpha;
+                st.gamma[c] = t[c];
+                any |= act;
             }
-            st.iters = k;
-            st.done = !any || k >= a.max_iter;
-            st.unconverged = any && k >= a.max_iter;
+            st.iters = i;
+            st.done = !any || i >= a.max_iter;
+            st.unconverged = any && i >= a.max_iter;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[k & 1] = st;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[i & 1] = st;
     return st;
 }
 
-// Search direction p = D^-1 r + beta p_old.  One explicit fma so that a row's
-// own p and the copy its neighbours recompute are the same bits.
-template <class T> __device__ __forceinline__ T p_next(T dinv, T r, T beta, T p_old) {
-    return fma(beta, p_old, dinv * r);
-}
+#ifndef SBS_ITER_BATCH
+#define SBS_ITER_BATCH 4
+#endif
+constexpr int kIterBatch = SBS_ITER_BATCH;  // neighbours in flight per row in the iteration kernel
 
 template <class T>
-__global__ void __launch_bounds__(kTpb) pcg_a_kernel(PcgArgs<T> a, int k, int probe) {
+__global__ void __launch_bounds__(kTpb) pcg_iter_kernel(PcgArgs<T> a, int i, int probe) {
     if (a.skip && *a.skip) return;
-    const PcgState st = pcg_state(a, k);
+    const PcgState st = pcg_state(a, i);
     if (st.done || probe) return;
+    if (blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
+        double* z = iter_slot(a.slots, i + 1);
+        for (int k = threadIdx.x; k < kSlotSize; k += kTpb) z[k] = 0.0;
+    }
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const T* pin = (k & 1) ? a.p1 : a.p0;
-    T* pout = (k & 1) ? a.p0 : a.p1;
+    const bool first = i == 0;
+    const int rd = i & 1, wr = rd ^ 1;
+    const T* __restrict__ r_in = a.r[rd];
+    const T* __restrict__ w_in = a.w[rd];
+    const T* __restrict__ s_in = a.s[rd];
+    const T al0 = (T)st.alpha[0], al1 = (T)st.alpha[1], al2 = (T)st.alpha[2];
     const T be0 = (T)st.beta[0], be1 = (T)st.beta[1], be2 = (T)st.beta[2];
-    double vals[3] = {0, 0, 0};
+    double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (v < (unsigned)a.nv) {
-        T rv0, rv1, rv2, pv0, pv1, pv2;
-        ld3(a.r, v, rv0, rv1, rv2);
-        ld3(pin, v, pv0, pv1, pv2);
-        const T dv = a.dinv[v];
-        const T p0 = p_next(dv, rv0, be0, pv0), p1 = p_next(dv, rv1, be1, pv1), p2 = p_next(dv, rv2, be2, pv2);
-        st3(pout, v, p0, p1, p2);
-        const T d = a.diagA[v];
-        T y0 = d * p0, y1 = d * p1, y2 = d * p2;
-        ell_row(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t u, T m) {
-            T ru0, ru1, ru2, pu0, pu1, pu2;
-            ld3(a.r, u, ru0, ru1, ru2);
-            ld3(pin, u, pu0, pu1, pu2);
-            const T du = a.dinv[u];
-            y0 += m * p_next(du, ru0, be0, pu0);
-            y1 += m * p_next(du, ru1, be1, pu1);
-            y2 += m * p_next(du, ru2, be2, pu2);
-        });
-        st3(a.q, v, y0, y1, y2);
-        vals[0] = (double)p0 * y0; vals[1] = (double)p1 * y1; vals[2] = (double)p2 * y2;
-    }
-    stripe_add<3>(vals, a.slots + (kSlotA + (k & 1)) * kSlotSize);
-}
-
-template <class T>
-__global__ void __launch_bounds__(kTpb) pcg_b_kernel(PcgArgs<T> a, int k) {
-    if (a.skip && *a.skip) return;
-    const PcgState st = a.ctl->st[k & 1];
-    if (st.done) return;
-    double pq[3];
-    slot_totals<3>(a.slots + (kSlotA + (k & 1)) * kSlotSize, pq);
-    T al[3];
-    for (int c = 0; c < 3; ++c) al[c] = (T)((st.active[c] && pq[c] != 0.0) ? st.rz[c] / pq[c] : 0.0);
-    if (blockIdx.x == 0) {  // slots of iteration k+1 were last read by kernels that have finished
-        double* za = a.slots + (kSlotA + ((k + 1) & 1)) * kSlotSize;
-        double* zb = a.slots + (kSlotB + ((k + 1) & 1)) * kSlotSize;
-        for (int i = threadIdx.x; i < kSlotSize; i += kTpb) { za[i] = 0.0; zb[i] = 0.0; }
-    }
-    const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const T* pcur = (k & 1) ? a.p0 : a.p1;
-    double vals[6] = {0, 0, 0, 0, 0, 0};
-    if (v < (unsigned)a.nv) {
-        T x0, x1, x2, rv0, rv1, rv2, p0, p1, p2, q0, q1, q2;
+        T r0, r1, r2, w0, w1, w2, s0 = 0, s1 = 0, s2 = 0, x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
+        ld3(r_in, v, r0, r1, r2);
+        ld3(w_in, v, w0, w1, w2);
+        if (!first) {
+            ld3(s_in, v, s0, s1, s2);
+            ld3(a.p, v, p0, p1, p2);
+        }
         ld3(a.x, v, x0, x1, x2);
-        ld3(a.r, v, rv0, rv1, rv2);
-        ld3(pcur, v, p0, p1, p2);
-        ld3(a.q, v, q0, q1, q2);
-        x0 += al[0] * p0; x1 += al[1] * p1; x2 += al[2] * p2;
-        rv0 -= al[0] * q0; rv1 -= al[1] * q1; rv2 -= al[2] * q2;
+        const T dv = a.dinv[v], d = a.diagA[v];
+        // p_i = u_i + beta p_{i-1}, s_i, x_{i+1}, r_{i+1}
+        const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
+        p0 = first ? u0 : fma(be0, p0, u0);
+        p1 = first ? u1 : fma(be1, p1, u1);
+        p2 = first ? u2 : fma(be2, p2, u2);
+        s0 = cg_s(first, be0, s0, w0); s1 = cg_s(first, be1, s1, w1); s2 = cg_s(first, be2, s2, w2);
+        x0 = fma(al0, p0, x0); x1 = fma(al1, p1, x1); x2 = fma(al2, p2, x2);
+        const T rn0 = cg_r(al0, s0, r0), rn1 = cg_r(al1, s1, r1), rn2 = cg_r(al2, s2, r2);
+        const T un0 = dv * rn0, un1 = dv * rn1, un2 = dv * rn2;
+        T y0 = d * un0, y1 = d * un1, y2 = d * un2;
+        ell_row<kIterBatch>(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t n, T m) {
+            T nr0, nr1, nr2, nw0, nw1, nw2, ns0 = 0, ns1 = 0, ns2 = 0;
+            ld3(r_in, n, nr0, nr1, nr2);
+            ld3(w_in, n, nw0, nw1, nw2);
+            if (!first) ld3(s_in, n, ns0, ns1, ns2);
+            const T dn = a.dinv[n];
+            y0 += m * (dn * cg_r(al0, cg_s(first, be0, ns0, nw0), nr0));
+            y1 += m * (dn * cg_r(al1, cg_s(first, be1, ns1, nw1), nr1));
+            y2 += m * (dn * cg_r(al2, cg_s(first, be2, ns2, nw2), nr2));
+        });
+        st3(a.p, v, p0, p1, p2);
         st3(a.x, v, x0, x1, x2);
-        st3(a.r, v, rv0, rv1, rv2);
-        const T dv = a.dinv[v];
-        const T z0 = dv * rv0, z1 = dv * rv1, z2 = dv * rv2;
-        vals[0] = (double)rv0 * z0; vals[1] = (double)rv1 * z1; vals[2] = (double)rv2 * z2;
-        vals[3] = (double)z0 * z0; vals[4] = (double)z1 * z1; vals[5] = (double)z2 * z2;
+        st3(a.s[wr], v, s0, s1, s2);
+        st3(a.r[wr], v, rn0, rn1, rn2);
+        st3(a.w[wr], v, y0, y1, y2);
+        vals[0] = (double)rn0 * un0; vals[1] = (double)rn1 * un1; vals[2] = (double)rn2 * un2;
+        vals[3] = (double)y0 * un0; vals[4] = (double)y1 * un1; vals[5] = (double)y2 * un2;
+        vals[6] = (double)un0 * un0; vals[7] = (double)un1 * un1; vals[8] = (double)un2 * un2;
     }
-    stripe_add<6>(vals, a.slots + (kSlotB + (k & 1)) * kSlotSize);
+    stripe_add<9>(vals, iter_slot(a.slots, i));
 }
 
-// After the probe A(L): bookkeeping (block 0), and with `cheb` the
+// After the probe K(L): bookkeeping (block 0), and with `cheb` the
 // Chebyshev recurrence of the global step (pdSolver.cu:196-205) into sn/prev.
 template <class T>
 __global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, int cheb) {
@@ -757,8 +799,8 @@ __global__ void outer_decide_kernel(const double* slots, int nv, double tol, Ctl
 }  // namespace
 
 const char* kernel_name(int id) {
-    static const char* names[kKCount] = {"local_step", "pcg_solve",  "jacobi_chebyshev", "direct_solve",
-                                         "rhs_gather", "pcg_spmv",   "pcg_update"};
+    static const char* names[kKCount] = {"local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve",
+                                         "rhs_gather", "pcg_iteration"};
     return (id >= 0 && id < kKCount) ? names[id] : "?";
 }
 
@@ -989,7 +1031,8 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
     const size_t n3 = 3 * (size_t)nv_;
     c_.alloc(nv_); diagL_.alloc(nv_); diagA_.alloc(nv_); dinv_.alloc(nv_);
     sn_.alloc(n3); sn_old_.alloc(n3); prev_.alloc(n3); next_.alloc(n3); rhs_.alloc(n3);
-    r_.alloc(n3); p0_.alloc(n3); p1_.alloc(n3); q_.alloc(n3);
+    for (int k = 0; k < 2; ++k) { cg_r_[k].alloc(n3); cg_w_[k].alloc(n3); cg_s_[k].alloc(n3); }
+    cg_p_.alloc(n3);
     slots_.alloc((size_t)kNumSlots * kSlotSize);
     ctl_.alloc(sizeof(CtlBlock));
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, stream_, slots_.get(), kNumSlots * kSlotSize);
@@ -1059,10 +1102,12 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
     a.eval = eval_.get();
     a.diagA = diagA_.get();
     a.dinv = dinv_.get();
-    a.r = r_.get();
-    a.p0 = p0_.get();
-    a.p1 = p1_.get();
-    a.q = q_.get();
+    for (int k = 0; k < 2; ++k) {
+        a.r[k] = cg_r_[k].get();
+        a.w[k] = cg_w_[k].get();
+        a.s[k] = cg_s_[k].get();
+    }
+    a.p = cg_p_.get();
     a.sn = sn_.get();
     a.prev = prev_.get();
     a.omega = omega;
@@ -1075,35 +1120,31 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
     const dim3 grid(ceil_div(nv_, kTpb)), block(kTpb);
     const int max_iter = params.pcg_max_iter;
     tick(kKPcg, true);
-    hipLaunchKernelGGL(pcg_init_kernel<T>, grid, block, 0, stream_, a);
-    auto iteration = [&](int k) {
-        tick(kKPcgA, true);
-        hipLaunchKernelGGL(pcg_a_kernel<T>, grid, block, 0, stream_, a, k, 0);
-        tick(kKPcgA, false);
-        tick(kKPcgB, true);
-        hipLaunchKernelGGL(pcg_b_kernel<T>, grid, block, 0, stream_, a, k);
-        tick(kKPcgB, false);
+    hipLaunchKernelGGL(pcg_init1_kernel<T>, grid, block, 0, stream_, a);
+    hipLaunchKernelGGL(pcg_init2_kernel<T>, grid, block, 0, stream_, a);
+    auto iteration = [&](int i, int probe) {
+        tick(kKPcgIter, true);
+        hipLaunchKernelGGL(pcg_iter_kernel<T>, grid, block, 0, stream_, a, i, probe);
+        tick(kKPcgIter, false);
     };
     const bool learned = !exact && solve_index >= 0 && solve_index < (int)cap_.size();
     int L = 0;
     if (learned && cap_[solve_index] > 0) {
         L = std::min(cap_[solve_index], max_iter);
-        for (int k = 0; k < L; ++k) iteration(k);
+        for (int i = 0; i < L; ++i) iteration(i, 0);
     } else {
         const CtlBlock* hc = reinterpret_cast<const CtlBlock*>(host_ctl_);
         while (L < max_iter) {
             const int n = std::min(kCheckChunk, max_iter - L);
-            for (int j = 0; j < n; ++j) iteration(L++);
+            for (int j = 0; j < n; ++j) iteration(L++, 0);
             SBS_HIP(hipMemcpyAsync(host_ctl_, a.ctl, sizeof(CtlBlock), hipMemcpyDeviceToHost, stream_));
             SBS_HIP(hipStreamSynchronize(stream_));
-            // st[(L-1)&1] is the state A(L-1) saw: done there means no later iteration ran
+            // st[(L-1)&1] is the state launch L-1 saw: done there means no later iteration ran
             if ((skip && hc->outer_done) || hc->st[(L - 1) & 1].done) break;
         }
         if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L));
     }
-    tick(kKPcgA, true);
-    hipLaunchKernelGGL(pcg_a_kernel<T>, grid, block, 0, stream_, a, L, 1);
-    tick(kKPcgA, false);
+    iteration(L, 1);
     hipLaunchKernelGGL(pcg_finish_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
     tick(kKPcg, false);
     SBS_HIP(hipGetLastError());

@@ -81,8 +81,7 @@ enum KernelId : int {
     kKJacobi,
     kKDirect,
     kKRhs,
-    kKPcgA,     // p update + SpMV + p.q (one launch per PCG iteration, plus the stop probe)
-    kKPcgB,     // x, r update + r.z, z.z
+    kKPcgIter,  // one PCG iteration (and the stop probe after the last one)
     kKCount
 };
 const char* kernel_name(int id);
@@ -196,7 +195,7 @@ private:
     DevBuf<uint32_t> ecol_;
     DevBuf<T> eval_;
     // PCG workspace and control
-    DevBuf<T> r_, p0_, p1_, q_;
+    DevBuf<T> cg_r_[2], cg_w_[2], cg_s_[2], cg_p_;
     DevBuf<double> slots_;              // address-striped reduction slots
     DevBuf<unsigned char> ctl_;         // device control block
     void* host_ctl_ = nullptr;          // pinned copy of the control block (host checks)

@@ -20,7 +20,7 @@ SOLVER_CUSOLVER_CHOLESKY = 1
 SOLVER_EIGEN_CHOLESKY = 2
 SOLVER_PCG = 3
 
-KERNELS = ["local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve", "rhs_gather", "pcg_spmv", "pcg_update"]
+KERNELS = ["local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve", "rhs_gather", "pcg_iteration"]
 
 
 class LibraryNotBuilt(RuntimeError):
