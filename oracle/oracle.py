@@ -72,6 +72,20 @@ def lib(bench: bool = False) -> ctypes.CDLL:
     L.orc_pd_local.argtypes = [ctypes.c_int, ctypes.c_void_p, dp, dp, ctypes.c_int, ctypes.c_double,
                                ctypes.c_double, dp]
     L.orc_num_threads.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    L.orc_col_detect.restype = vp
+    L.orc_col_detect.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, up, up, vp, vp, ctypes.c_int, vp, fp]
+    L.orc_col_destroy.argtypes = [ctypes.c_int, vp]
+    L.orc_col_num_queries.restype = ctypes.c_longlong
+    L.orc_col_num_queries.argtypes = [ctypes.c_int, vp]
+    L.orc_col_num_hits.restype = ctypes.c_int
+    L.orc_col_num_hits.argtypes = [ctypes.c_int, vp]
+    L.orc_col_get.argtypes = [ctypes.c_int, vp, up, dp, dp]
+    L.orc_ccd_test.restype = ctypes.c_double
+    L.orc_ccd_test.argtypes = [ctypes.c_int, ctypes.c_int, up, vp, vp, dp]
+    L.orc_solve_cubic.restype = ctypes.c_int
+    L.orc_solve_cubic.argtypes = [ctypes.c_double] * 4 + [dp]
+    L.orc_ccd_response.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, fp]
     _LIBS[name] = L
     return L
 
@@ -165,3 +179,73 @@ class PdOracle:
 
 def num_threads(bench: bool = False) -> int:
     return int(lib(bench).orc_num_threads())
+
+
+# ------------------------------------------------------------- collision --
+QUERY_VF = 1
+QUERY_EE = 2
+
+
+def _vec(a, fp64):
+    return np.ascontiguousarray(a, dtype=np.float64 if fp64 else np.float32).reshape(-1)
+
+
+def collision_detect(X, XT, tri, fathers, ignore_self=True, fp64=False, resolve=True):
+    """DetectCollision on the CPU: sorted unique queries (n, 5) [type, v0..v3],
+    their toi and normals, and (resolve=True) tI per vertex and per-vertex
+    normals (float32) after storeTi."""
+    L = lib()
+    prec = 64 if fp64 else 32
+    X = _vec(X, fp64)
+    XT = _vec(XT, fp64)
+    nv = X.size // 3
+    tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1)
+    fathers = np.ascontiguousarray(fathers, dtype=np.uint32).reshape(-1)
+    tI = np.ones(nv, dtype=X.dtype) if resolve else None
+    nrm_v = np.zeros(3 * nv, dtype=np.float32) if resolve else None
+    h = L.orc_col_detect(prec, nv, tri.size // 3, _p(tri, ctypes.c_uint32), _p(fathers, ctypes.c_uint32),
+                         X.ctypes.data, XT.ctypes.data, int(ignore_self),
+                         tI.ctypes.data if resolve else None, _p(nrm_v, ctypes.c_float) if resolve else None)
+    n = L.orc_col_num_queries(prec, h)
+    q = np.zeros((n, 5), dtype=np.uint32)
+    toi = np.zeros(n)
+    nrm = np.zeros((n, 3))
+    if n:
+        L.orc_col_get(prec, h, _p(q, ctypes.c_uint32), _p(toi, ctypes.c_double), _p(nrm, ctypes.c_double))
+    hits = L.orc_col_num_hits(prec, h)
+    L.orc_col_destroy(prec, h)
+    out = dict(queries=q, toi=toi, normals=nrm, hits=hits)
+    if resolve:
+        out["tI"] = tI
+        out["vertex_normals"] = nrm_v.reshape(-1, 3)
+    return out
+
+
+def ccd_test(qtype, v, X, XT, fp64=True):
+    """ccdCollisionTest of one query: (toi, normal)."""
+    L = lib()
+    X = _vec(X, fp64)
+    XT = _vec(XT, fp64)
+    v = np.ascontiguousarray(v, dtype=np.uint32)
+    n = np.zeros(3)
+    t = L.orc_ccd_test(64 if fp64 else 32, int(qtype), _p(v, ctypes.c_uint32), X.ctypes.data, XT.ctypes.data,
+                       _p(n, ctypes.c_double))
+    return t, n
+
+
+def solve_cubic(a, b, c, d):
+    x = np.zeros(3)
+    k = lib().orc_solve_cubic(a, b, c, d, _p(x, ctypes.c_double))
+    return x[:k]
+
+
+def ccd_response(X, XT, V, tI, vertex_normals, fp64=False):
+    """CCDKernel on the CPU; returns the new X and V."""
+    X = _vec(X, fp64).copy()
+    XT = _vec(XT, fp64)
+    V = _vec(V, fp64).copy()
+    tI = np.ascontiguousarray(tI, dtype=X.dtype)
+    nrm = np.ascontiguousarray(vertex_normals, dtype=np.float32).reshape(-1)
+    lib().orc_ccd_response(64 if fp64 else 32, X.size // 3, X.ctypes.data, XT.ctypes.data, V.ctypes.data,
+                           tI.ctypes.data, _p(nrm, ctypes.c_float))
+    return X.reshape(-1, 3), V.reshape(-1, 3)

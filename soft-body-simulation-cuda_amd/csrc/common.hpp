@@ -98,4 +98,32 @@ template <class T> __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// One DPP move of a double (both 32-bit halves); lanes without a source, or
+// outside ROW_MASK, read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum of a double over the 64 lanes of a wavefront with DPP moves (VALU only,
+// no LDS round trips), returned to every lane.  The steps: pairs and quads
+// (quad_perm), 8 (row_half_mirror), 16 (row_mirror), then row 1 and 3 take
+// row 0 and 2's total (row_bcast:15) and row 3 takes rows 0-1 (row_bcast:31);
+// lane 63 holds the total.  Every lane of the wavefront must be active.
+__device__ __forceinline__ double wave_total(double v) {
+    v += dpp_f64<0xB1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);        // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);       // row_half_mirror
+    v += dpp_f64<0x140>(v);       // row_mirror
+    v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15
+    v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 }  // namespace sbs

@@ -47,14 +47,14 @@ template <class T> __device__ __forceinline__ void st3(T* __restrict__ p, size_t
 
 // Sums NQ per-thread values over the block; thread 0 adds them to the
 // block's stripe of `slot` (stripe = block id mod kStripes), so no address
-// sees more than gridDim/kStripes atomics.
+// sees more than gridDim/kStripes atomics.  Every thread of the block calls it.
 template <int NQ>
 __device__ __forceinline__ void stripe_add(const double (&vals)[NQ], double* slot) {
     __shared__ double red[kTpb / 64][NQ];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-        const double s = wave_sum(vals[k]);
+        const double s = wave_total(vals[k]);
         if (lane == 0) red[wid][k] = s;
     }
     __syncthreads();
@@ -66,22 +66,15 @@ __device__ __forceinline__ void stripe_add(const double (&vals)[NQ], double* slo
 }
 
 // Totals of the first NQ quantities of a slot (written by an earlier kernel),
-// summed by the block's first wavefront and broadcast through LDS.
+// computed by every wavefront on its own (no LDS, no barrier).
 template <int NQ>
 __device__ __forceinline__ void slot_totals(const double* slot, double (&out)[NQ]) {
-    __shared__ double tot[NQ];
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    double v[NQ];
 #pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            const double v = lane < kStripes ? slot[lane * kSlotQ + k] : 0.0;
-            const double s = wave_sum(v);
-            if (lane == 0) tot[k] = s;
-        }
-    }
-    __syncthreads();
+    for (int k = 0; k < NQ; ++k) v[k] = lane < kStripes ? slot[lane * kSlotQ + k] : 0.0;
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) out[k] = tot[k];
+    for (int k = 0; k < NQ; ++k) out[k] = wave_total(v[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,14 +281,16 @@ __global__ void begin_step_kernel(int nv, const T* __restrict__ X, const T* __re
 //   2. one thread per tet: F = Ds Dm^-1, 3x3 SVD in registers, R (or R - F),
 //      the 4 corner forces of w R Dm^-T G^T written to LDS (plain stores);
 //   3. one thread per (vertex, component) sums its incidences in list order
-//      and writes the chunk's partial force for that vertex (coalesced).
+//      and writes the chunk's partial force for that vertex at the vertex's
+//      slot of the vertex-major partial array.
 // No atomics anywhere, so the result is deterministic; float atomics on LDS
 // serialise on the shared corners of neighbouring tets and were the
 // bottleneck of the scatter form.
 template <class T>
 __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
     int nt, const uint2* __restrict__ lidx, const int* __restrict__ chunk_vptr,
-    const uint32_t* __restrict__ chunk_verts, const uint16_t* __restrict__ inc_codes,
+    const uint32_t* __restrict__ chunk_verts, const uint32_t* __restrict__ entry_pos,
+    const uint16_t* __restrict__ inc_codes,
     const uint16_t* __restrict__ inc_end, const T* __restrict__ dminv, const T* __restrict__ w,
     const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip,
     double* __restrict__ zero_slots, int zero_n) {
@@ -352,7 +347,6 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
         }
     }
     __syncthreads();
-    T* out = partial + 3 * (size_t)e0;
     for (int wi = threadIdx.x; wi < 3 * nu; wi += kChunkTets) {
         const int i = wi / 3, r = wi - 3 * i;
         const int k1 = ends[i];
@@ -361,7 +355,7 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
             const unsigned code = codes[k];
             acc += fs[((code & 3u) * 3 + r) * kChunkTets + (code >> 2)];
         }
-        out[wi] = acc;
+        partial[3 * (size_t)entry_pos[e0 + i] + r] = acc;  // vertex-major: the consumers read contiguously
     }
 }
 
@@ -373,16 +367,15 @@ constexpr int kGatherBatch = 4;  // vertex->entry list entries loaded together
 template <class T> struct RhsGather {
     const T* c;
     const T* sn_old;
-    const int* vent_ptr;
-    const uint32_t* vent;
+    const int* vent_ptr;      // partials of vertex v: [vent_ptr[v], vent_ptr[v+1])
     const T* partial;
     const T* dbc;
     const T* more_dbc;
     const T* dbcx;
     T wdbc;
 
-    // All loads are issued up front (the entry list kGatherBatch at a time,
-    // clamped in bounds) so a vertex costs few dependent memory round trips.
+    // A vertex's partials are contiguous (the local step writes them in vertex
+    // order); they are loaded kGatherBatch at a time, clamped in bounds.
     __device__ __forceinline__ void at(unsigned v, T& bx, T& by, T& bz) const {
         const T pin = dbc[v];
         const T more = more_dbc ? more_dbc[v] : (T)0;
@@ -392,13 +385,10 @@ template <class T> struct RhsGather {
         const T cv = c[v];
         bx = cv * sx; by = cv * sy; bz = cv * sz;
         for (int e = e0; e < e1; e += kGatherBatch) {
-            uint32_t k[kGatherBatch];
-#pragma unroll
-            for (int j = 0; j < kGatherBatch; ++j) k[j] = vent[min(e + j, e1 - 1)];
 #pragma unroll
             for (int j = 0; j < kGatherBatch; ++j) {
                 T px, py, pz;
-                ld3(partial, k[j], px, py, pz);
+                ld3(partial, min(e + j, e1 - 1), px, py, pz);
                 if (e + j < e1) { bx += px; by += py; bz += pz; }
             }
         }
@@ -538,9 +528,9 @@ __device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const
 //             x_{i+1} = x_i + alpha_i p_i, r_{i+1} = r_i - alpha_i s_i
 //             u_{i+1} = D^-1 r_{i+1},      w_{i+1} = A u_{i+1}
 //           and gamma_{i+1}, delta_{i+1}, u.u into the stripes.
-// The neighbours' u_{i+1} in A u_{i+1} are recomputed from their r_i, w_i,
-// s_{i-1} (double-buffered, read-only in K(i)) with the same operations the
-// owning row uses, so they are the same bits.  This is the reference's
+// The neighbours' part of A u_{i+1} is formed from their r_i, w_i, s_{i-1}
+// (double-buffered, read-only in K(i)) as three sums taken before alpha_i and
+// beta_i are known (see pcg_iter_kernel).  This is the reference's
 // Jacobi-PCG (pcgJacobi.cu:89-173) reorganised; in exact arithmetic the
 // iterates are identical.  A component stops when
 // ||u_k|| <= max(tol ||u_0||, floor ||D^-1 b||); launches after the stop
@@ -576,13 +566,6 @@ template <class T> struct PcgArgs {
 __device__ __forceinline__ double* iter_slot(double* slots, int i) {
     return slots + (kSlotIter + ((i % 3) + 3) % 3) * kSlotSize;
 }
-
-// s_i = w_i + beta s_{i-1} (w_i at i = 0), r_{i+1} = r_i - alpha s_i: one
-// explicit fma each, shared by the owning row and its neighbours.
-template <class T> __device__ __forceinline__ T cg_s(bool first, T beta, T s_old, T w) {
-    return first ? w : fma(beta, s_old, w);
-}
-template <class T> __device__ __forceinline__ T cg_r(T alpha, T s, T r) { return fma(-alpha, s, r); }
 
 template <class T>
 __global__ void __launch_bounds__(kTpb) pcg_init1_kernel(PcgArgs<T> a) {
@@ -693,54 +676,82 @@ template <class T> __device__ __forceinline__ PcgState pcg_state(const PcgArgs<T
 #endif
 constexpr int kIterBatch = SBS_ITER_BATCH;  // neighbours in flight per row in the iteration kernel
 
-template <class T>
-__global__ void __launch_bounds__(kTpb) pcg_iter_kernel(PcgArgs<T> a, int i, int probe) {
+// One PCG iteration.  Everything that does not depend on alpha_i, beta_i is
+// issued first: the row's own values and the three neighbour sums
+//   Yr = sum m D^-1 r_i,  Yw = sum m D^-1 w_i,  Ys = sum m D^-1 s_{i-1}
+// so that A u_{i+1} (off-diagonal part) = Yr - alpha (Yw + beta Ys) follows
+// once the scalars are known; the reduction stripes of the previous launch
+// are read while those loads are in flight.
+// 5 wavefronts per SIMD keep every row of the 64^3 cube resident at once
+// (4292 wavefronts on 1024 SIMDs).
+template <class T, bool kFirst>
+__global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) pcg_iter_kernel(PcgArgs<T> a, int i,
+                                                                                               int probe) {
     if (a.skip && *a.skip) return;
-    const PcgState st = pcg_state(a, i);
-    if (st.done || probe) return;
-    if (blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
-        double* z = iter_slot(a.slots, i + 1);
-        for (int k = threadIdx.x; k < kSlotSize; k += kTpb) z[k] = 0.0;
+    if (!kFirst && a.ctl->st[(i - 1) & 1].done) {  // stopped earlier: carry the state forward
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[i & 1] = a.ctl->st[(i - 1) & 1];
+        return;
+    }
+    if (probe) {
+        (void)pcg_state(a, i);
+        return;
     }
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const bool first = i == 0;
+    const bool row = v < (unsigned)a.nv;
+    const unsigned vr = row ? v : (unsigned)a.nv - 1;  // loads stay in bounds; results unused
     const int rd = i & 1, wr = rd ^ 1;
     const T* __restrict__ r_in = a.r[rd];
     const T* __restrict__ w_in = a.w[rd];
     const T* __restrict__ s_in = a.s[rd];
+    T r0, r1, r2, w0, w1, w2, s0 = 0, s1 = 0, s2 = 0, x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
+    ld3(r_in, vr, r0, r1, r2);
+    ld3(w_in, vr, w0, w1, w2);
+    if (!kFirst) {
+        ld3(s_in, vr, s0, s1, s2);
+        ld3(a.p, vr, p0, p1, p2);
+    }
+    ld3(a.x, vr, x0, x1, x2);
+    const T dv = a.dinv[vr], d = a.diagA[vr];
+    T yr0 = 0, yr1 = 0, yr2 = 0, yw0 = 0, yw1 = 0, yw2 = 0, ys0 = 0, ys1 = 0, ys2 = 0;
+    ell_row<kIterBatch>(a.slice_ptr, a.slice_w, a.ecol, a.eval, vr, [&](uint32_t n, T m) {
+        T nr0, nr1, nr2, nw0, nw1, nw2;
+        ld3(r_in, n, nr0, nr1, nr2);
+        ld3(w_in, n, nw0, nw1, nw2);
+        const T md = m * a.dinv[n];
+        yr0 += md * nr0; yr1 += md * nr1; yr2 += md * nr2;
+        yw0 += md * nw0; yw1 += md * nw1; yw2 += md * nw2;
+        if (!kFirst) {
+            T ns0, ns1, ns2;
+            ld3(s_in, n, ns0, ns1, ns2);
+            ys0 += md * ns0; ys1 += md * ns1; ys2 += md * ns2;
+        }
+    });
+    const PcgState st = pcg_state(a, i);
+    if (st.done) return;
+    if (blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
+        double* z = iter_slot(a.slots, i + 1);
+        for (int k = threadIdx.x; k < kSlotSize; k += kTpb) z[k] = 0.0;
+    }
     const T al0 = (T)st.alpha[0], al1 = (T)st.alpha[1], al2 = (T)st.alpha[2];
     const T be0 = (T)st.beta[0], be1 = (T)st.beta[1], be2 = (T)st.beta[2];
     double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (v < (unsigned)a.nv) {
-        T r0, r1, r2, w0, w1, w2, s0 = 0, s1 = 0, s2 = 0, x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
-        ld3(r_in, v, r0, r1, r2);
-        ld3(w_in, v, w0, w1, w2);
-        if (!first) {
-            ld3(s_in, v, s0, s1, s2);
-            ld3(a.p, v, p0, p1, p2);
-        }
-        ld3(a.x, v, x0, x1, x2);
-        const T dv = a.dinv[v], d = a.diagA[v];
-        // p_i = u_i + beta p_{i-1}, s_i, x_{i+1}, r_{i+1}
+    if (row) {
+        // p_i = u_i + beta p_{i-1}, s_i = w_i + beta s_{i-1}, x_{i+1}, r_{i+1}, u_{i+1}
         const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
-        p0 = first ? u0 : fma(be0, p0, u0);
-        p1 = first ? u1 : fma(be1, p1, u1);
-        p2 = first ? u2 : fma(be2, p2, u2);
-        s0 = cg_s(first, be0, s0, w0); s1 = cg_s(first, be1, s1, w1); s2 = cg_s(first, be2, s2, w2);
+        if (!kFirst) {
+            p0 = fma(be0, p0, u0); p1 = fma(be1, p1, u1); p2 = fma(be2, p2, u2);
+            s0 = fma(be0, s0, w0); s1 = fma(be1, s1, w1); s2 = fma(be2, s2, w2);
+        } else {
+            p0 = u0; p1 = u1; p2 = u2;
+            s0 = w0; s1 = w1; s2 = w2;
+        }
         x0 = fma(al0, p0, x0); x1 = fma(al1, p1, x1); x2 = fma(al2, p2, x2);
-        const T rn0 = cg_r(al0, s0, r0), rn1 = cg_r(al1, s1, r1), rn2 = cg_r(al2, s2, r2);
+        const T rn0 = fma(-al0, s0, r0), rn1 = fma(-al1, s1, r1), rn2 = fma(-al2, s2, r2);
         const T un0 = dv * rn0, un1 = dv * rn1, un2 = dv * rn2;
-        T y0 = d * un0, y1 = d * un1, y2 = d * un2;
-        ell_row<kIterBatch>(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t n, T m) {
-            T nr0, nr1, nr2, nw0, nw1, nw2, ns0 = 0, ns1 = 0, ns2 = 0;
-            ld3(r_in, n, nr0, nr1, nr2);
-            ld3(w_in, n, nw0, nw1, nw2);
-            if (!first) ld3(s_in, n, ns0, ns1, ns2);
-            const T dn = a.dinv[n];
-            y0 += m * (dn * cg_r(al0, cg_s(first, be0, ns0, nw0), nr0));
-            y1 += m * (dn * cg_r(al1, cg_s(first, be1, ns1, nw1), nr1));
-            y2 += m * (dn * cg_r(al2, cg_s(first, be2, ns2, nw2), nr2));
-        });
+        // w_{i+1} = diag u_{i+1} + Yr - alpha (Yw + beta Ys)
+        const T y0 = d * un0 + (yr0 - al0 * (kFirst ? yw0 : fma(be0, ys0, yw0)));
+        const T y1 = d * un1 + (yr1 - al1 * (kFirst ? yw1 : fma(be1, ys1, yw1)));
+        const T y2 = d * un2 + (yr2 - al2 * (kFirst ? yw2 : fma(be2, ys2, yw2)));
         st3(a.p, v, p0, p1, p2);
         st3(a.x, v, x0, x1, x2);
         st3(a.s[wr], v, s0, s1, s2);
@@ -945,12 +956,14 @@ template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h
     std::vector<uint32_t> vent(nentries_);
     std::vector<int> fill(vptr.begin(), vptr.end() - 1);
     for (size_t e = 0; e < nentries_; ++e) vent[fill[cverts[e]]++] = (uint32_t)e;
+    std::vector<uint32_t> epos(nentries_);  // entry -> its slot in vertex-major order
+    for (size_t j = 0; j < nentries_; ++j) epos[vent[j]] = (uint32_t)j;
     tet_perm_d_.alloc(nt_);
     tet_lidx_.alloc(nt_);
     chunk_vptr_.alloc(nchunks_ + 1);
     chunk_verts_.alloc(nentries_);
     vent_ptr_.alloc(nv_ + 1);
-    vent_.alloc(nentries_);
+    entry_pos_.alloc(nentries_);
     partial_f_.alloc(3 * nentries_);
     inc_codes_.alloc(inc_codes.size());
     inc_end_.alloc(std::max<size_t>(1, inc_end.size()));
@@ -961,7 +974,7 @@ template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h
     SBS_HIP(hipMemcpy(chunk_vptr_.get(), cvptr.data(), cvptr.size() * sizeof(int), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(chunk_verts_.get(), cverts.data(), nentries_ * sizeof(uint32_t), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(vent_ptr_.get(), vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice));
-    SBS_HIP(hipMemcpy(vent_.get(), vent.data(), nentries_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(entry_pos_.get(), epos.data(), nentries_ * sizeof(uint32_t), hipMemcpyHostToDevice));
 }
 
 // Sliced-ELL structure of the off-diagonal scalar pattern, built on the host
@@ -1074,7 +1087,8 @@ template <class T> void PdEngine<T>::launch_local(const T* q, bool residual_form
     tick(kKLocal, true);
     // block 0 also clears the reduction slots of the global step that follows
     hipLaunchKernelGGL(local_chunk_kernel<T>, dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_, tet_lidx_.get(),
-                       chunk_vptr_.get(), chunk_verts_.get(), inc_codes_.get(), inc_end_.get(), dminv_.get(), w_.get(),
+                       chunk_vptr_.get(), chunk_verts_.get(), entry_pos_.get(), inc_codes_.get(), inc_end_.get(),
+                       dminv_.get(), w_.get(),
                        q, partial_f_.get(), (int)residual_form, skip, slots_.get(), kNumSlots * kSlotSize);
     tick(kKLocal, false);
     SBS_HIP(hipGetLastError());
@@ -1091,7 +1105,7 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
     PcgArgs<T> a{};
     a.nv = nv_;
     a.gathered = gathered;
-    a.g = RhsGather<T>{c_.get(), sn_old_.get(), vent_ptr_.get(), vent_.get(), partial_f_.get(), data_.DBC,
+    a.g = RhsGather<T>{c_.get(), sn_old_.get(), vent_ptr_.get(), partial_f_.get(), data_.DBC,
                        data_.moreDBC, data_.DBCX, pin_weight<T>(params.dt)};
     a.b = b;
     a.x0 = x0;
@@ -1124,7 +1138,8 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
     hipLaunchKernelGGL(pcg_init2_kernel<T>, grid, block, 0, stream_, a);
     auto iteration = [&](int i, int probe) {
         tick(kKPcgIter, true);
-        hipLaunchKernelGGL(pcg_iter_kernel<T>, grid, block, 0, stream_, a, i, probe);
+        if (i == 0) hipLaunchKernelGGL((pcg_iter_kernel<T, true>), grid, block, 0, stream_, a, i, probe);
+        else hipLaunchKernelGGL((pcg_iter_kernel<T, false>), grid, block, 0, stream_, a, i, probe);
         tick(kKPcgIter, false);
     };
     const bool learned = !exact && solve_index >= 0 && solve_index < (int)cap_.size();
@@ -1204,7 +1219,7 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     const bool cheb = jacobi || (pcg && params.pcg_chebyshev);
     if (pcg) reset_caps_if_changed(params);
     step_caps_[slot] = cap_epoch_;
-    const RhsGather<T> g{c_.get(), sn_old_.get(), vent_ptr_.get(), vent_.get(), partial_f_.get(), data.DBC,
+    const RhsGather<T> g{c_.get(), sn_old_.get(), vent_ptr_.get(), partial_f_.get(), data.DBC,
                          data.moreDBC, data.DBCX, pin_weight<T>(params.dt)};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (perf_) for (auto& e : ev) SBS_HIP(hipEventCreate(&e));
@@ -1307,7 +1322,7 @@ template <class T> void PdEngine<T>::local_only(const T* q, const T* sn_old, boo
     SBS_REQUIRE(ready_, "local_only: engine not prepared");
     const unsigned vb = ceil_div(nv_, kTpb);
     launch_local(q, residual_form, nullptr);
-    const RhsGather<T> g{c_.get(), sn_old, vent_ptr_.get(), vent_.get(), partial_f_.get(), data_.DBC, data_.moreDBC,
+    const RhsGather<T> g{c_.get(), sn_old, vent_ptr_.get(), partial_f_.get(), data_.DBC, data_.moreDBC,
                          data_.DBCX, pin_weight<T>(dt)};
     hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, b_out, (const int*)nullptr);
     SBS_HIP(hipGetLastError());

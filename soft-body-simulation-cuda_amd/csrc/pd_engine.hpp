@@ -177,7 +177,7 @@ private:
     DevBuf<int> chunk_vptr_;            // nchunks+1: first entry of each chunk
     DevBuf<uint32_t> chunk_verts_;      // global vertex of each entry
     DevBuf<int> vent_ptr_;              // N_v+1: entries of each vertex
-    DevBuf<uint32_t> vent_;             // entry ids grouped by vertex
+    DevBuf<uint32_t> entry_pos_;        // entry -> slot of its partial in vertex-major order
     size_t nentries_ = 0;
     DevBuf<T> partial_f_;               // 3 scalars per entry: local-step partial forces
     DevBuf<uint16_t> inc_codes_;        // per chunk 4*kChunkTets: (tet << 2 | corner) grouped by vertex
