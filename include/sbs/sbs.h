@@ -71,6 +71,12 @@ typedef struct SbsSolverData {
     const void* DBC;      /* num_verts: 1 for pinned vertices, else 0         */
     const void* moreDBC;  /* num_verts or NULL: extra soft pins (0 = none)    */
     void* V0;             /* num_tets or NULL: rest volumes, written at create */
+    /* Surface for collision handling (used when handle_collision is set):   */
+    int num_tris;
+    const uint32_t* Tri;        /* 3*num_tris                                   */
+    const uint32_t* TriFathers; /* num_tris: body of each triangle (dev_TriFathers) */
+    void* tIs;                  /* num_verts: 1, or 0.5 on hit vertices (dev_tIs) */
+    float* Normals;             /* 3*num_verts: contact normals (dev_Normals)   */
 } SbsSolverData;
 
 /* Mirrors SolverParams<Scalar> (src/def.h:83-100) for the PD step. */
@@ -80,7 +86,7 @@ typedef struct SbsSolverParams {
     double gravity;       /* magnitude along -y                               */
     double tol;           /* early exit of the direct/PCG loop on the iterate */
     double rho;           /* Chebyshev spectral-radius estimate               */
-    int handle_collision; /* reserved: mesh contact after the solve           */
+    int handle_collision; /* CCD between bodies after the solve + CCD response */
     double pcg_tol;       /* PCG: ||D^-1 r_k|| <= pcg_tol * ||D^-1 r_0||       */
     double pcg_floor;     /*  ... or <= pcg_floor * ||D^-1 b||                 */
     int pcg_max_iter;     /* PCG iteration limit per solve                     */
@@ -146,6 +152,40 @@ int sbs_pd_local_step(SbsPd* pd, const void* d_q, const void* d_sn_old, int resi
 int sbs_pd_system_nnz(SbsPd* pd, long long* nnz);
 int sbs_pd_export_system(SbsPd* pd, void* c, void* diagA, void* diagL, void* V0, void* w, void* dminv, int* rowptr,
                          int* col, void* val);
+
+/* CollisionDetection<Scalar> (src/collision/bvh.h:44-89): LBVH broad phase,
+ * vertex-face / edge-edge CCD narrow phase and hit resolution over surface
+ * triangles.  The triangle arrays are bound at creation (device, caller-owned,
+ * read at every call); positions are passed per call.
+ *
+ *   DetectCollision(numVerts, numTris, Tri, X, XTilde, TriFathers, tI, nors,
+ *                   ignoreSelf)              bvh.h:48, bvh.cu:171-181 -> sbs_collision_detect
+ *   ComputeMinStepSize(...)                  bvh.h:50, bvh.cu:183-190 -> sbs_collision_min_step
+ *   GetNumQueries / GetQueries               bvh.h:57-63     -> sbs_collision_num_queries / _get_queries
+ *
+ * tI: num_verts scalars (1, or 0.5 on the vertices of a hit); normals:
+ * 3*num_verts floats, written on hit vertices.  Where the reference leaves
+ * the result to thread scheduling, the order is fixed: equal times of impact
+ * keep the query first in (v1, v2, v3) order, and a vertex touched by several
+ * hits takes the normal of the last one in the sorted query list. */
+typedef struct SbsCollision SbsCollision;
+int sbs_collision_create(int precision, int num_verts, int num_tris, const uint32_t* d_tri,
+                         const uint32_t* d_tri_fathers, SbsCollision** out);
+int sbs_collision_destroy(SbsCollision* c);
+int sbs_collision_set_stream(SbsCollision* c, void* hip_stream);
+int sbs_collision_detect(SbsCollision* c, const void* d_X, const void* d_XTilde, int ignore_self, void* d_tI,
+                         float* d_normals);
+int sbs_collision_min_step(SbsCollision* c, const void* d_X, const void* d_XTilde, int ignore_self, double* toi);
+/* Queries of the last call: sorted, unique, canonical (type 1 = vertex v0 vs
+ * triangle v1 v2 v3; type 2 = edge v0 v1 vs edge v2 v3), 5 u32 each, with
+ * their time of impact and normal (host buffers; any may be NULL). */
+int sbs_collision_num_queries(SbsCollision* c, long long* n);
+int sbs_collision_get_queries(SbsCollision* c, uint32_t* queries, double* toi, double* normals);
+int sbs_collision_stats(SbsCollision* c, long long* candidate_pairs, int* hits);
+/* CCDKernel (src/simulation/collisionUtil.cu:50-70): vertices with tI < 1 get
+ * V = -((XTilde - X).n) n and keep X; the others take X = XTilde. */
+int sbs_ccd_response(int precision, int num_verts, void* d_X, const void* d_XTilde, void* d_V, const void* d_tI,
+                     const float* d_normals);
 
 /* Device memory and copies for callers without their own allocator. */
 int sbs_malloc(void** ptr, size_t bytes);

@@ -4,8 +4,10 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 
 #include "../../include/sbs/sbs.h"
+#include "collision.hpp"
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
 
@@ -24,6 +26,12 @@ struct SbsPd {
     int precision = 32;
     std::unique_ptr<sbs::PdEngine<float>> f32;
     std::unique_ptr<sbs::PdEngine<double>> f64;
+};
+
+struct SbsCollision {
+    int precision = 32;
+    std::unique_ptr<sbs::CollisionDetector<float>> f32;
+    std::unique_ptr<sbs::CollisionDetector<double>> f64;
 };
 
 namespace {
@@ -61,6 +69,11 @@ template <class T> sbs::PdData<T> to_data(const SbsSolverData* d) {
     o.DBC = static_cast<const T*>(d->DBC);
     o.moreDBC = static_cast<const T*>(d->moreDBC);
     o.V0 = static_cast<T*>(d->V0);
+    o.num_tris = d->num_tris;
+    o.Tri = d->Tri;
+    o.TriFathers = d->TriFathers;
+    o.tIs = static_cast<T*>(d->tIs);
+    o.Normals = d->Normals;
     return o;
 }
 
@@ -78,6 +91,12 @@ sbs::PdParams to_params(const SbsSolverParams* p) {
     o.pcg_chebyshev = p->pcg_chebyshev;
     o.pcg_adaptive = p->pcg_adaptive;
     return o;
+}
+
+template <class F> void with_detector(SbsCollision* c, F&& f) {
+    SBS_REQUIRE(c != nullptr, "null SbsCollision handle");
+    if (c->precision == 64) f(*c->f64);
+    else f(*c->f32);
 }
 
 // Calls f with the engine of the handle's precision.
@@ -284,6 +303,89 @@ int sbs_memset(void* dst, int value, size_t bytes) {
 int sbs_device_synchronize(void) {
     return guarded([&] { SBS_HIP(hipDeviceSynchronize()); });
 }
+int sbs_collision_create(int precision, int num_verts, int num_tris, const uint32_t* d_tri,
+                         const uint32_t* d_tri_fathers, SbsCollision** out) {
+    return guarded([&] {
+        SBS_REQUIRE(out != nullptr, "sbs_collision_create: null argument");
+        SBS_REQUIRE(precision == 32 || precision == 64, "sbs_collision_create: precision must be 32 or 64");
+        *out = nullptr;
+        auto h = std::make_unique<SbsCollision>();
+        h->precision = precision;
+        if (precision == 64)
+            h->f64 = std::make_unique<sbs::CollisionDetector<double>>(num_verts, num_tris, d_tri, d_tri_fathers, nullptr);
+        else
+            h->f32 = std::make_unique<sbs::CollisionDetector<float>>(num_verts, num_tris, d_tri, d_tri_fathers, nullptr);
+        *out = h.release();
+    });
+}
+
+int sbs_collision_destroy(SbsCollision* c) {
+    return guarded([&] { delete c; });
+}
+
+int sbs_collision_set_stream(SbsCollision* c, void* hip_stream) {
+    return guarded([&] { with_detector(c, [&](auto& d) { d.set_stream(static_cast<hipStream_t>(hip_stream)); }); });
+}
+
+int sbs_collision_detect(SbsCollision* c, const void* d_X, const void* d_XTilde, int ignore_self, void* d_tI,
+                         float* d_normals) {
+    return guarded([&] {
+        with_detector(c, [&](auto& d) {
+            using T = std::remove_reference_t<decltype(d)>;
+            using S = std::conditional_t<std::is_same_v<T, sbs::CollisionDetector<double>>, double, float>;
+            d.detect(static_cast<const S*>(d_X), static_cast<const S*>(d_XTilde), ignore_self != 0,
+                     static_cast<S*>(d_tI), d_normals);
+        });
+    });
+}
+
+int sbs_collision_min_step(SbsCollision* c, const void* d_X, const void* d_XTilde, int ignore_self, double* toi) {
+    return guarded([&] {
+        SBS_REQUIRE(toi != nullptr, "sbs_collision_min_step: null output");
+        with_detector(c, [&](auto& d) {
+            using T = std::remove_reference_t<decltype(d)>;
+            using S = std::conditional_t<std::is_same_v<T, sbs::CollisionDetector<double>>, double, float>;
+            *toi = d.min_step(static_cast<const S*>(d_X), static_cast<const S*>(d_XTilde), ignore_self != 0);
+        });
+    });
+}
+
+int sbs_collision_num_queries(SbsCollision* c, long long* n) {
+    return guarded([&] {
+        SBS_REQUIRE(n != nullptr, "null output");
+        with_detector(c, [&](auto& d) { *n = d.num_queries(); });
+    });
+}
+
+int sbs_collision_get_queries(SbsCollision* c, uint32_t* queries, double* toi, double* normals) {
+    return guarded([&] { with_detector(c, [&](auto& d) { d.get_queries(queries, toi, normals); }); });
+}
+
+int sbs_collision_stats(SbsCollision* c, long long* candidate_pairs, int* hits) {
+    return guarded([&] {
+        with_detector(c, [&](auto& d) {
+            if (candidate_pairs) *candidate_pairs = d.num_pairs();
+            if (hits) *hits = d.num_hits();
+        });
+    });
+}
+
+int sbs_ccd_response(int precision, int num_verts, void* d_X, const void* d_XTilde, void* d_V, const void* d_tI,
+                     const float* d_normals) {
+    return guarded([&] {
+        SBS_REQUIRE(num_verts >= 0, "sbs_ccd_response: negative size");
+        SBS_REQUIRE(num_verts == 0 || (d_X && d_XTilde && d_V && d_tI && d_normals), "sbs_ccd_response: null array");
+        if (num_verts == 0) return;
+        if (precision == 64)
+            sbs::ccd_response<double>(num_verts, static_cast<double*>(d_X), static_cast<const double*>(d_XTilde),
+                                      static_cast<double*>(d_V), static_cast<const double*>(d_tI), d_normals, nullptr);
+        else
+            sbs::ccd_response<float>(num_verts, static_cast<float*>(d_X), static_cast<const float*>(d_XTilde),
+                                     static_cast<float*>(d_V), static_cast<const float*>(d_tI), d_normals, nullptr);
+        SBS_HIP(hipStreamSynchronize(nullptr));
+    });
+}
+
 int sbs_device_count(int* count) {
     return guarded([&] {
         SBS_REQUIRE(count != nullptr, "null count");

@@ -1,0 +1,90 @@
+// Continuous collision detection between surface triangles on gfx950:
+// LBVH broad phase, vertex-face / edge-edge CCD narrow phase, hit resolution
+// and the velocity response of the PD step.
+//
+// Reference behaviour reproduced:
+//   CollisionDetection::DetectCollision      src/collision/bvh.cu:171-181
+//   CollisionDetection::ComputeMinStepSize   src/collision/bvh.cu:183-190
+//   BroadPhaseCCD (leaf boxes, pair filters, 12 queries per pair, canonical
+//   form, unique)                            src/collision/broadphase.cu:540-554
+//   NarrowPhase + storeTi                    src/collision/narrowphase.cu:115-132
+//   ccdCollisionTest                         src/collision/intersections.cu:286-337
+//   CCDKernel                                src/simulation/collisionUtil.cu:50-70
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sbs {
+
+// Canonical query: type 1 = vertex v0 against triangle (v1, v2, v3), type 2 =
+// edge (v0, v1) against edge (v2, v3); the reference's Query (aabb.h:67-77).
+struct QueryKey {
+    uint32_t type, v0, v1, v2, v3;
+};
+
+template <class T> class CollisionDetector {
+public:
+    // tri: 3*num_tris u32 and fathers: num_tris u32 (body of each triangle),
+    // device arrays owned by the caller and read at every detect.
+    CollisionDetector(int num_verts, int num_tris, const uint32_t* tri, const uint32_t* fathers, hipStream_t stream);
+    ~CollisionDetector();
+
+    // DetectCollision: tI[v] = 1 for every vertex, 0.5 for the vertices of
+    // every resolved hit; normals (3 floats per vertex) written for hit
+    // vertices only.  Synchronises the stream (query counts come to the host).
+    void detect(const T* X, const T* XT, bool ignore_self, T* tI, float* normals);
+    // ComputeMinStepSize: the smallest time of impact over all queries (1 when
+    // there is none).
+    double min_step(const T* X, const T* XT, bool ignore_self);
+
+    long long num_queries() const { return nq_; }
+    int num_hits() const { return hits_; }
+    long long num_pairs() const { return npairs_; }
+    // Host copy of the last query list (sorted, unique) with toi and normals.
+    void get_queries(uint32_t* q5, double* toi, double* n3) const;
+    void set_stream(hipStream_t s) { stream_ = s; }
+
+private:
+    void broad_phase(const T* X, const T* XT, bool ignore_self);
+    void narrow_phase(const T* X, const T* XT);
+
+    int nv_ = 0, nt_ = 0;
+    const uint32_t* tri_ = nullptr;
+    const uint32_t* fathers_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    // tree
+    DevBuf<T> leaf_box_;                 // 6 per triangle (caller order)
+    DevBuf<unsigned long long> keys_, keys_sorted_;
+    DevBuf<uint32_t> order_in_, order_;  // sorted position -> triangle
+    DevBuf<int> left_, right_, parent_, flags_;
+    DevBuf<T> node_box_;                 // 6 per node: internal 0..n-2, leaves n-1..2n-2
+    DevBuf<T> scene_;                    // per-block partial bounds, then the scene box
+    DevBuf<unsigned char> sort_tmp_;
+    size_t sort_tmp_bytes_ = 0;
+    // pairs and queries
+    DevBuf<uint2> pairs_;
+    size_t pair_cap_ = 0;
+    DevBuf<unsigned long long> counters_;  // [0] pairs, [1] stack overflow flag, [2] hits
+    DevBuf<QueryKey> qraw_, qsorted_, qunique_;
+    DevBuf<int> qflag_, qscan_;
+    DevBuf<unsigned char> qtmp_;
+    size_t qtmp_bytes_ = 0;
+    size_t qcap_ = 0;
+    DevBuf<T> toi_;
+    DevBuf<T> qnormal_;                  // 3 per unique query
+    DevBuf<int> owner_;                  // per vertex: winning query of its normal
+    DevBuf<T> red_;                      // min-toi partials
+    long long nq_ = 0, npairs_ = 0;
+    int hits_ = 0;
+    unsigned long long* host_ = nullptr;  // pinned scratch for counts
+};
+
+// CCDKernel (collisionUtil.cu:50-70): hit vertices (tI < 1) get
+// V = -(d.n) n with d = XTilde - X and keep X; the others take X = XTilde.
+template <class T>
+void ccd_response(int nv, T* X, const T* XT, T* V, const T* tI, const float* normals, hipStream_t stream);
+
+}  // namespace sbs

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstring>
 
+#include "collision.hpp"
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
 #include "svd3.hpp"
@@ -455,7 +456,7 @@ __global__ void end_step_kernel(int nv, const T* __restrict__ sn, T dt_inv, T* _
     if (more_dbc && more_dbc[v] > (T)0) st3(V, v, (T)0, (T)0, (T)0);
     else st3(V, v, (nx - ox) * dt_inv, (ny - oy) * dt_inv, (nz - oz) * dt_inv);
     st3(XTilde, v, nx, ny, nz);
-    st3(X, v, nx, ny, nz);
+    if (X) st3(X, v, nx, ny, nz);  // X <- XTilde (pdSolver.cu:227) unless collisions decide it
 }
 
 // computeError (pdSolver.cu:243-253): sum (prev - src)^2 into the error slot,
@@ -1279,9 +1280,22 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
             perf_data_[1].ms += gm;
         }
     }
+    const bool collide = params.handle_collision != 0;
     hipLaunchKernelGGL(end_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(), (T)1 / dt,
-                       data.XTilde, data.V, data.X, data.moreDBC);
+                       data.XTilde, data.V, collide ? (T*)nullptr : data.X, data.moreDBC);
     SBS_HIP(hipGetLastError());
+    if (collide) {
+        // DetectCollision between bodies on the step's motion X -> XTilde, then
+        // CCDKernel (pdSolver.cu:218-224)
+        SBS_REQUIRE(data.num_tris >= 0 && (data.num_tris == 0 || (data.Tri && data.TriFathers)),
+                    "PdSolver: handleCollision needs Tri and TriFathers");
+        SBS_REQUIRE(data.tIs && data.Normals, "PdSolver: handleCollision needs tIs and Normals");
+        if (!collision_)
+            collision_ = std::make_unique<CollisionDetector<T>>(nv_, data.num_tris, data.Tri, data.TriFathers, stream_);
+        collision_->set_stream(stream_);
+        collision_->detect(data.X, data.XTilde, true, data.tIs, data.Normals);
+        ccd_response<T>(nv_, data.X, data.XTilde, data.V, data.tIs, data.Normals, stream_);
+    }
     // counters of the step (PCG iterations, unconverged solves, max per solve,
     // solves, last, per-solve log) land in this step's pinned slot
     SBS_HIP(hipMemcpyAsync(host_counters_ + kCounterInts * slot, &ctl->pcg_iters, kCounterInts * sizeof(int),

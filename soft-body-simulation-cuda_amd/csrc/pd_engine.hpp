@@ -34,6 +34,7 @@
 namespace sbs {
 
 template <class T> class BandCholesky;
+template <class T> class CollisionDetector;
 
 enum class GlobalSolver : int { Jacobi = 0, CuSolverCholesky = 1, EigenCholesky = 2, PCG = 3 };
 
@@ -67,6 +68,11 @@ template <class T> struct PdData {
     const T* DBC = nullptr;         // N_v, 0/1 pin flags
     const T* moreDBC = nullptr;     // N_v, extra soft pins (may be null: all zero)
     T* V0 = nullptr;       // N_t, filled at construction when non-null
+    int num_tris = 0;                   // surface, for collision handling
+    const uint32_t* Tri = nullptr;      // 3*num_tris
+    const uint32_t* TriFathers = nullptr;  // num_tris: body of each triangle
+    T* tIs = nullptr;                   // N_v: 1, or 0.5 on hit vertices
+    float* Normals = nullptr;           // 3*N_v contact normals
 };
 
 struct PerfEntry {
@@ -207,6 +213,8 @@ private:
     long long step_caps_[2] = {-1, -1};  // cap epoch of each counter slot
     // direct solver (band Cholesky factor, GPU triangular solves) for small bodies
     std::unique_ptr<BandCholesky<T>> direct_;
+    // collision handling after the step (built on first use)
+    std::unique_ptr<CollisionDetector<T>> collision_;
     int last_iters_ = 0, last_pcg_iters_ = 0;
     // pinned counters of the last two steps (slot = step parity), kCounterInts
     // each: pcg iterations, unconverged solves, max per solve, solves, last, pad, log

@@ -38,7 +38,9 @@ class SbsSolverData(ctypes.Structure):
                 ("X", ctypes.c_void_p), ("V", ctypes.c_void_p), ("X0", ctypes.c_void_p),
                 ("XTilde", ctypes.c_void_p), ("DBCX", ctypes.c_void_p), ("Tet", ctypes.c_void_p),
                 ("mass", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("DBC", ctypes.c_void_p),
-                ("moreDBC", ctypes.c_void_p), ("V0", ctypes.c_void_p)]
+                ("moreDBC", ctypes.c_void_p), ("V0", ctypes.c_void_p), ("num_tris", ctypes.c_int),
+                ("Tri", ctypes.c_void_p), ("TriFathers", ctypes.c_void_p), ("tIs", ctypes.c_void_p),
+                ("Normals", ctypes.c_void_p)]
 
 
 class SbsSolverParams(ctypes.Structure):
@@ -75,6 +77,15 @@ SIGNATURES = {
     "sbs_pd_local_step": (_I, [_VP, _VP, _VP, _I, _D, _VP]),
     "sbs_pd_system_nnz": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
     "sbs_pd_export_system": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "sbs_collision_create": (_I, [_I, _I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
+    "sbs_collision_destroy": (_I, [_VP]),
+    "sbs_collision_set_stream": (_I, [_VP, _VP]),
+    "sbs_collision_detect": (_I, [_VP, _VP, _VP, _I, _VP, _VP]),
+    "sbs_collision_min_step": (_I, [_VP, _VP, _VP, _I, ctypes.POINTER(_D)]),
+    "sbs_collision_num_queries": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "sbs_collision_get_queries": (_I, [_VP, _VP, _VP, _VP]),
+    "sbs_collision_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)]),
+    "sbs_ccd_response": (_I, [_I, _I, _VP, _VP, _VP, _VP, _VP]),
     "sbs_malloc": (_I, [ctypes.POINTER(_VP), _SZ]),
     "sbs_free": (_I, [_VP]),
     "sbs_memcpy_htod": (_I, [_VP, _VP, _SZ]),

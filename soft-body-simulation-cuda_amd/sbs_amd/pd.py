@@ -82,12 +82,22 @@ class SolverData:
         self.c.DBC = put("DBC", np.ascontiguousarray(scene.dbc, dtype=dt))
         self.c.moreDBC = None
         self.c.V0 = put("V0", np.zeros(self.nt, dtype=dt))
+        # surface for collision handling (dev_TriFathers, dev_tIs, dev_Normals)
+        tri = getattr(scene, "tri", None)
+        if tri is not None and len(tri):
+            self.c.num_tris = int(np.asarray(tri).reshape(-1, 3).shape[0])
+            self.c.Tri = put("Tri", np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1))
+            self.c.TriFathers = put("TriFathers", np.ascontiguousarray(scene.tri_fathers, dtype=np.uint32))
+        self.c.tIs = put("tIs", np.ones(self.nv, dtype=dt))
+        self.c.Normals = put("Normals", np.zeros(3 * self.nv, dtype=np.float32))
 
     def get(self, name: str) -> np.ndarray:
         if name in ("X", "V", "X0", "XTilde", "DBCX"):
             return self._bufs[name].download(self.dtype, (self.nv, 3))
-        if name in ("mass", "DBC"):
+        if name in ("mass", "DBC", "tIs"):
             return self._bufs[name].download(self.dtype, (self.nv,))
+        if name == "Normals":
+            return self._bufs[name].download(np.float32, (self.nv, 3))
         if name in ("mu", "V0"):
             return self._bufs[name].download(self.dtype, (self.nt,))
         raise KeyError(name)
@@ -193,3 +203,90 @@ class PdSolver:
 
     def synchronize(self) -> None:
         capi.check(self._L.sbs_device_synchronize())
+
+
+class CollisionDetection:
+    """CollisionDetection<Scalar> (src/collision/bvh.h:44-89) over the gfx950
+    library: DetectCollision, ComputeMinStepSize, GetNumQueries/GetQueries.
+
+    The triangle arrays are uploaded once; positions are numpy arrays (n, 3)
+    uploaded per call, or device pointers (ints) used as they are."""
+
+    def __init__(self, num_verts: int, tri, tri_fathers, precision: int = 32):
+        self._L = capi.lib()
+        self.precision = precision
+        self.dtype = np.float32 if precision == 32 else np.float64
+        self.nv = int(num_verts)
+        tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1)
+        self.ntri = tri.size // 3
+        self._tri = capi.DeviceArray.from_numpy(tri)
+        self._fat = capi.DeviceArray.from_numpy(np.ascontiguousarray(tri_fathers, dtype=np.uint32))
+        self._x = capi.DeviceArray(3 * self.nv * np.dtype(self.dtype).itemsize)
+        self._xt = capi.DeviceArray(3 * self.nv * np.dtype(self.dtype).itemsize)
+        self._ti = capi.DeviceArray(self.nv * np.dtype(self.dtype).itemsize)
+        self._n = capi.DeviceArray(3 * self.nv * 4)
+        self._n.upload(np.zeros(3 * self.nv, np.float32))
+        h = ctypes.c_void_p()
+        capi.check(self._L.sbs_collision_create(precision, self.nv, self.ntri, self._tri.ptr, self._fat.ptr,
+                                                ctypes.byref(h)))
+        self.h = h.value
+
+    def _pos(self, X, XT):
+        self._x.upload(np.ascontiguousarray(X, dtype=self.dtype).reshape(-1))
+        self._xt.upload(np.ascontiguousarray(XT, dtype=self.dtype).reshape(-1))
+
+    def DetectCollision(self, X, XTilde, ignoreSelfCollision: bool = True):
+        """Returns (tI (nv,), normals (nv, 3) float32)."""
+        self._pos(X, XTilde)
+        capi.check(self._L.sbs_collision_detect(self.h, self._x.ptr, self._xt.ptr, int(ignoreSelfCollision),
+                                                self._ti.ptr, self._n.ptr))
+        return self._ti.download(self.dtype, (self.nv,)), self._n.download(np.float32, (self.nv, 3))
+
+    def ComputeMinStepSize(self, X, XTilde, ignoreSelfCollision: bool = False) -> float:
+        self._pos(X, XTilde)
+        t = ctypes.c_double()
+        capi.check(self._L.sbs_collision_min_step(self.h, self._x.ptr, self._xt.ptr, int(ignoreSelfCollision),
+                                                  ctypes.byref(t)))
+        return t.value
+
+    def GetNumQueries(self) -> int:
+        n = ctypes.c_longlong()
+        capi.check(self._L.sbs_collision_num_queries(self.h, ctypes.byref(n)))
+        return n.value
+
+    def GetQueries(self):
+        """(queries (n, 5) u32 [type, v0..v3], toi (n,), normals (n, 3))."""
+        n = self.GetNumQueries()
+        q = np.zeros((n, 5), np.uint32)
+        toi = np.zeros(n)
+        nrm = np.zeros((n, 3))
+        if n:
+            capi.check(self._L.sbs_collision_get_queries(self.h, q.ctypes.data, toi.ctypes.data, nrm.ctypes.data))
+        return q, toi, nrm
+
+    def stats(self):
+        p, h = ctypes.c_longlong(), ctypes.c_int()
+        capi.check(self._L.sbs_collision_stats(self.h, ctypes.byref(p), ctypes.byref(h)))
+        return p.value, h.value
+
+    def close(self):
+        if self.h:
+            capi.check(self._L.sbs_collision_destroy(self.h))
+            self.h = None
+        for b in (self._tri, self._fat, self._x, self._xt, self._ti, self._n):
+            b.free()
+
+
+def ccd_response(X, XTilde, V, tI, normals, precision: int = 32):
+    """CCDKernel (collisionUtil.cu:50-70) on the GPU; returns new (X, V)."""
+    dt = np.float32 if precision == 32 else np.float64
+    L = capi.lib()
+    bufs = [capi.DeviceArray.from_numpy(np.ascontiguousarray(a, dtype=t).reshape(-1))
+            for a, t in ((X, dt), (XTilde, dt), (V, dt), (tI, dt), (normals, np.float32))]
+    nv = np.asarray(X).reshape(-1, 3).shape[0]
+    capi.check(L.sbs_ccd_response(precision, nv, *(b.ptr for b in bufs)))
+    Xn = bufs[0].download(dt, (nv, 3))
+    Vn = bufs[2].download(dt, (nv, 3))
+    for b in bufs:
+        b.free()
+    return Xn, Vn
