@@ -294,7 +294,7 @@ __global__ void refit_kernel(int n, const uint32_t* order, const T* leaf_box, co
     int node = k + n - 1;
     const uint32_t tri = order[k];
     for (int c = 0; c < 6; ++c) node_box[6 * (size_t)node + c] = leaf_box[6 * (size_t)tri + c];
-    while (true) {
+    for (int step = 0; step < n; ++step) {  // a path to the root has fewer than n edges
         const int p = parent[node];
         if (p < 0) break;
         const int prev = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -335,7 +335,11 @@ __global__ void traverse_kernel(int n, const uint32_t* order, const uint32_t* tr
     int stack[kStack];
     int sp = 0;
     stack[sp++] = 0;
-    while (sp > 0) {
+    for (int visits = 0; sp > 0; ++visits) {
+        if (visits >= n) {  // a tree has n - 1 internal nodes: more visits means a malformed tree
+            atomicOr(&counters[1], 2ull);
+            break;
+        }
         const int node = stack[--sp];
         const int ch[2] = {left[node], right[node]};
         for (int c = 0; c < 2; ++c) {
@@ -610,7 +614,7 @@ template <class T> void CollisionDetector<T>::broad_phase(const T* X, const T* X
         SBS_HIP(hipGetLastError());
         SBS_HIP(hipMemcpyAsync(host_, counters_.get(), 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
         SBS_HIP(hipStreamSynchronize(stream_));
-        SBS_REQUIRE(host_[1] == 0, "CollisionDetection: BVH traversal stack overflow (degenerate tree)");
+        SBS_REQUIRE(host_[1] == 0, "CollisionDetection: BVH traversal stack overflow or malformed tree");
         if (host_[0] <= pair_cap_) break;
         pair_cap_ = (size_t)host_[0] + host_[0] / 4;  // grow and run the traversal again
         pairs_.alloc(pair_cap_);
