@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <type_traits>
 
 #include "collision.hpp"
 #include "direct_solver.hpp"
@@ -44,6 +45,46 @@ template <class T> __device__ __forceinline__ void st3(T* __restrict__ p, size_t
     p[3 * i] = x;
     p[3 * i + 1] = y;
     p[3 * i + 2] = z;
+}
+
+// The PCG vectors a row's neighbours read (r, w, s and D^-1) live in one
+// 12-scalar record per vertex: [r xyz, w xyz, s xyz, D^-1, 0, 0], so a
+// neighbour costs three aligned 16-byte (float) loads from one or two cache
+// lines instead of four gathers from four arrays.
+constexpr int kRec = 12;
+template <class T> struct Rec {
+    T v[kRec];
+};
+template <class T> __device__ __forceinline__ Rec<T> ld_rec(const T* __restrict__ base, size_t i) {
+    Rec<T> r;
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    constexpr int per = 16 / sizeof(T);
+    const V* p = reinterpret_cast<const V*>(base + kRec * i);
+#pragma unroll
+    for (int k = 0; k < kRec / per; ++k) {
+        const V q = p[k];
+        if constexpr (per == 4) {
+            r.v[4 * k] = q.x; r.v[4 * k + 1] = q.y; r.v[4 * k + 2] = q.z; r.v[4 * k + 3] = q.w;
+        } else {
+            r.v[2 * k] = q.x; r.v[2 * k + 1] = q.y;
+        }
+    }
+    return r;
+}
+template <class T> __device__ __forceinline__ void st_rec(T* __restrict__ base, size_t i, const Rec<T>& r) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    constexpr int per = 16 / sizeof(T);
+    V* p = reinterpret_cast<V*>(base + kRec * i);
+#pragma unroll
+    for (int k = 0; k < kRec / per; ++k) {
+        V q;
+        if constexpr (per == 4) {
+            q.x = r.v[4 * k]; q.y = r.v[4 * k + 1]; q.z = r.v[4 * k + 2]; q.w = r.v[4 * k + 3];
+        } else {
+            q.x = r.v[2 * k]; q.y = r.v[2 * k + 1];
+        }
+        p[k] = q;
+    }
 }
 
 // Sums NQ per-thread values over the block; thread 0 adds them to the
@@ -549,9 +590,7 @@ template <class T> struct PcgArgs {
     const T* eval;
     const T* diagA;
     const T* dinv;
-    T* r[2];                  // r_i in r[i&1]
-    T* w[2];                  // w_i in w[i&1]
-    T* s[2];                  // s_{i-1} in s[i&1]
+    T* rec[2];                // records of iteration i in rec[i&1]: r_i, w_i, s_{i-1}, D^-1
     T* p;
     T* sn;
     T* prev;
@@ -589,8 +628,12 @@ __global__ void __launch_bounds__(kTpb) pcg_init1_kernel(PcgArgs<T> a) {
             });
         }
         st3(a.x, v, x0, x1, x2);
-        st3(a.r[0], v, b0 - ax0, b1 - ax1, b2 - ax2);
         const T di = a.dinv[v];
+        Rec<T> rc;
+        rc.v[0] = b0 - ax0; rc.v[1] = b1 - ax1; rc.v[2] = b2 - ax2;
+        for (int k = 3; k < kRec; ++k) rc.v[k] = (T)0;
+        rc.v[9] = di;
+        st_rec(a.rec[0], v, rc);
         const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
         vals[0] = d0 * d0; vals[1] = d1 * d1; vals[2] = d2 * d2;
     }
@@ -603,18 +646,18 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_kernel(PcgArgs<T> a) {
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (v < (unsigned)a.nv) {
-        T r0, r1, r2;
-        ld3(a.r[0], v, r0, r1, r2);
-        const T dv = a.dinv[v], d = a.diagA[v];
+        T* rec = a.rec[0];
+        const T r0 = rec[kRec * (size_t)v], r1 = rec[kRec * (size_t)v + 1], r2 = rec[kRec * (size_t)v + 2];
+        const T dv = rec[kRec * (size_t)v + 9], d = a.diagA[v];
         const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
         T w0 = d * u0, w1 = d * u1, w2 = d * u2;
         ell_row(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t n, T m) {
-            T q0, q1, q2;
-            ld3(a.r[0], n, q0, q1, q2);
-            const T dn = a.dinv[n];
+            const T q0 = rec[kRec * (size_t)n], q1 = rec[kRec * (size_t)n + 1], q2 = rec[kRec * (size_t)n + 2];
+            const T dn = rec[kRec * (size_t)n + 9];
             w0 += m * (dn * q0); w1 += m * (dn * q1); w2 += m * (dn * q2);
         });
-        st3(a.w[0], v, w0, w1, w2);
+        // w0 into the record (neighbours read only r and D^-1 of it in this launch)
+        rec[kRec * (size_t)v + 3] = w0; rec[kRec * (size_t)v + 4] = w1; rec[kRec * (size_t)v + 5] = w2;
         vals[0] = (double)r0 * u0; vals[1] = (double)r1 * u1; vals[2] = (double)r2 * u2;
         vals[3] = (double)w0 * u0; vals[4] = (double)w1 * u1; vals[5] = (double)w2 * u2;
         vals[6] = (double)u0 * u0; vals[7] = (double)u1 * u1; vals[8] = (double)u2 * u2;
@@ -701,31 +744,22 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
     const bool row = v < (unsigned)a.nv;
     const unsigned vr = row ? v : (unsigned)a.nv - 1;  // loads stay in bounds; results unused
     const int rd = i & 1, wr = rd ^ 1;
-    const T* __restrict__ r_in = a.r[rd];
-    const T* __restrict__ w_in = a.w[rd];
-    const T* __restrict__ s_in = a.s[rd];
-    T r0, r1, r2, w0, w1, w2, s0 = 0, s1 = 0, s2 = 0, x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
-    ld3(r_in, vr, r0, r1, r2);
-    ld3(w_in, vr, w0, w1, w2);
-    if (!kFirst) {
-        ld3(s_in, vr, s0, s1, s2);
-        ld3(a.p, vr, p0, p1, p2);
-    }
+    const T* __restrict__ rec_in = a.rec[rd];
+    const Rec<T> own = ld_rec(rec_in, vr);
+    const T r0 = own.v[0], r1 = own.v[1], r2 = own.v[2], w0 = own.v[3], w1 = own.v[4], w2 = own.v[5];
+    T s0 = own.v[6], s1 = own.v[7], s2 = own.v[8];
+    const T dv = own.v[9];
+    T x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
+    if (!kFirst) ld3(a.p, vr, p0, p1, p2);
     ld3(a.x, vr, x0, x1, x2);
-    const T dv = a.dinv[vr], d = a.diagA[vr];
+    const T d = a.diagA[vr];
     T yr0 = 0, yr1 = 0, yr2 = 0, yw0 = 0, yw1 = 0, yw2 = 0, ys0 = 0, ys1 = 0, ys2 = 0;
     ell_row<kIterBatch>(a.slice_ptr, a.slice_w, a.ecol, a.eval, vr, [&](uint32_t n, T m) {
-        T nr0, nr1, nr2, nw0, nw1, nw2;
-        ld3(r_in, n, nr0, nr1, nr2);
-        ld3(w_in, n, nw0, nw1, nw2);
-        const T md = m * a.dinv[n];
-        yr0 += md * nr0; yr1 += md * nr1; yr2 += md * nr2;
-        yw0 += md * nw0; yw1 += md * nw1; yw2 += md * nw2;
-        if (!kFirst) {
-            T ns0, ns1, ns2;
-            ld3(s_in, n, ns0, ns1, ns2);
-            ys0 += md * ns0; ys1 += md * ns1; ys2 += md * ns2;
-        }
+        const Rec<T> nb = ld_rec(rec_in, n);
+        const T md = m * nb.v[9];
+        yr0 += md * nb.v[0]; yr1 += md * nb.v[1]; yr2 += md * nb.v[2];
+        yw0 += md * nb.v[3]; yw1 += md * nb.v[4]; yw2 += md * nb.v[5];
+        if (!kFirst) { ys0 += md * nb.v[6]; ys1 += md * nb.v[7]; ys2 += md * nb.v[8]; }
     });
     const PcgState st = pcg_state(a, i);
     if (st.done) return;
@@ -755,9 +789,12 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
         const T y2 = d * un2 + (yr2 - al2 * (kFirst ? yw2 : fma(be2, ys2, yw2)));
         st3(a.p, v, p0, p1, p2);
         st3(a.x, v, x0, x1, x2);
-        st3(a.s[wr], v, s0, s1, s2);
-        st3(a.r[wr], v, rn0, rn1, rn2);
-        st3(a.w[wr], v, y0, y1, y2);
+        Rec<T> out;
+        out.v[0] = rn0; out.v[1] = rn1; out.v[2] = rn2;
+        out.v[3] = y0; out.v[4] = y1; out.v[5] = y2;
+        out.v[6] = s0; out.v[7] = s1; out.v[8] = s2;
+        out.v[9] = dv; out.v[10] = (T)0; out.v[11] = (T)0;
+        st_rec(a.rec[wr], v, out);
         vals[0] = (double)rn0 * un0; vals[1] = (double)rn1 * un1; vals[2] = (double)rn2 * un2;
         vals[3] = (double)y0 * un0; vals[4] = (double)y1 * un1; vals[5] = (double)y2 * un2;
         vals[6] = (double)un0 * un0; vals[7] = (double)un1 * un1; vals[8] = (double)un2 * un2;
@@ -1045,7 +1082,7 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
     const size_t n3 = 3 * (size_t)nv_;
     c_.alloc(nv_); diagL_.alloc(nv_); diagA_.alloc(nv_); dinv_.alloc(nv_);
     sn_.alloc(n3); sn_old_.alloc(n3); prev_.alloc(n3); next_.alloc(n3); rhs_.alloc(n3);
-    for (int k = 0; k < 2; ++k) { cg_r_[k].alloc(n3); cg_w_[k].alloc(n3); cg_s_[k].alloc(n3); }
+    for (int k = 0; k < 2; ++k) cg_rec_[k].alloc(kRec * (size_t)nv_);
     cg_p_.alloc(n3);
     slots_.alloc((size_t)kNumSlots * kSlotSize);
     ctl_.alloc(sizeof(CtlBlock));
@@ -1118,9 +1155,7 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
     a.diagA = diagA_.get();
     a.dinv = dinv_.get();
     for (int k = 0; k < 2; ++k) {
-        a.r[k] = cg_r_[k].get();
-        a.w[k] = cg_w_[k].get();
-        a.s[k] = cg_s_[k].get();
+        a.rec[k] = cg_rec_[k].get();
     }
     a.p = cg_p_.get();
     a.sn = sn_.get();

@@ -201,7 +201,7 @@ private:
     DevBuf<uint32_t> ecol_;
     DevBuf<T> eval_;
     // PCG workspace and control
-    DevBuf<T> cg_r_[2], cg_w_[2], cg_s_[2], cg_p_;
+    DevBuf<T> cg_rec_[2], cg_p_;        // per-vertex PCG records (r, w, s, D^-1), double-buffered
     DevBuf<double> slots_;              // address-striped reduction slots
     DevBuf<unsigned char> ctl_;         // device control block
     void* host_ctl_ = nullptr;          // pinned copy of the control block (host checks)
