@@ -53,6 +53,10 @@ extern "C" {
 #define SBS_SOLVER_CUSOLVER_CHOLESKY 1
 #define SBS_SOLVER_EIGEN_CHOLESKY 2
 #define SBS_SOLVER_PCG 3
+/* Per-body dense inverses applied with MFMA + FP64 iterative refinement: the
+ * CholeskyDnLinearSolver path (src/simulation/solver/linear/cholesky.cu:26-70,
+ * 161-166) for scenes of small bodies (contiguous vertex ranges). */
+#define SBS_SOLVER_DENSE_CHOLESKY 4
 
 /* Mirrors SolverData<Scalar> (src/def.h:22-56); pointers are device memory
  * holding float (precision 32) or double (precision 64). */
@@ -92,6 +96,9 @@ typedef struct SbsSolverParams {
     int pcg_max_iter;     /* PCG iteration limit per solve                     */
     int pcg_chebyshev;    /* apply the Chebyshev recurrence after each solve   */
     int pcg_adaptive;     /* 1: learned launch cap (no host sync); 0: exact    */
+    int dense_precision;  /* dense solver: 0 FP32 MFMA, 1 BF16 MFMA            */
+    double dense_tol;     /* dense solver: max|D^-1 r| <= tol max|D^-1 b|      */
+    int dense_max_iter;   /* dense solver: FP64 refinement corrections per solve */
 } SbsSolverParams;
 
 typedef struct SbsPd SbsPd;
@@ -135,6 +142,13 @@ int sbs_pd_stats(SbsPd* pd, SbsPdStats* out);
 #define SBS_KERNEL_COUNT 6
 int sbs_pd_set_kernel_timing(SbsPd* pd, unsigned mask);
 int sbs_pd_kernel_time(SbsPd* pd, int kernel_id, const char** name, int* launches, double* ms);
+
+/* Dense solver of the handle (after an update with SBS_SOLVER_DENSE_CHOLESKY):
+ * number of bodies, largest body, bytes of the inverse fragments, and the last
+ * solve's refinement corrections (-1-max_iter if not converged) and residual
+ * ratio.  Waits for the handle's stream. */
+int sbs_pd_dense_info(SbsPd* pd, int* bodies, int* max_body, long long* inverse_bytes, int* last_iterations,
+                      double* last_ratio);
 
 /* LinearSolver::Solve(3N, d_b, d_x[, d_guess]) against the prepared PD matrix
  * (direct when a Cholesky solver type is set, else PCG); synchronous. */

@@ -90,6 +90,9 @@ sbs::PdParams to_params(const SbsSolverParams* p) {
     o.pcg_max_iter = p->pcg_max_iter;
     o.pcg_chebyshev = p->pcg_chebyshev;
     o.pcg_adaptive = p->pcg_adaptive;
+    o.dense_precision = p->dense_precision;
+    o.dense_tol = p->dense_tol;
+    o.dense_max_iter = p->dense_max_iter;
     return o;
 }
 
@@ -133,7 +136,7 @@ int sbs_pd_destroy(SbsPd* pd) {
 
 int sbs_pd_set_global_solver(SbsPd* pd, int solver_type) {
     return guarded([&] {
-        SBS_REQUIRE(solver_type >= 0 && solver_type <= 3, "sbs_pd_set_global_solver: unknown solver type");
+        SBS_REQUIRE(solver_type >= 0 && solver_type <= 4, "sbs_pd_set_global_solver: unknown solver type");
         with_engine(pd, [&](auto& e) { e.set_global_solver(static_cast<sbs::GlobalSolver>(solver_type)); });
     });
 }
@@ -303,6 +306,12 @@ int sbs_memset(void* dst, int value, size_t bytes) {
 int sbs_device_synchronize(void) {
     return guarded([&] { SBS_HIP(hipDeviceSynchronize()); });
 }
+int sbs_pd_dense_info(SbsPd* pd, int* bodies, int* max_body, long long* inverse_bytes, int* last_iterations,
+                      double* last_ratio) {
+    return guarded(
+        [&] { with_engine(pd, [&](auto& e) { e.dense_info(bodies, max_body, inverse_bytes, last_iterations, last_ratio); }); });
+}
+
 int sbs_collision_create(int precision, int num_verts, int num_tris, const uint32_t* d_tri,
                          const uint32_t* d_tri_fathers, SbsCollision** out) {
     return guarded([&] {

@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "collision.hpp"
+#include "dense_refine.hpp"
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
 #include "svd3.hpp"
@@ -1095,21 +1096,59 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
     // DBCX <- X0 (pdSolver.cu:134)
     SBS_HIP(hipMemcpyAsync(data.DBCX, data.X0, n3 * sizeof(T), hipMemcpyDeviceToDevice, stream_));
     direct_.reset();
-    if (solver_ == GlobalSolver::CuSolverCholesky || solver_ == GlobalSolver::EigenCholesky) direct_factor();
+    dense_.reset();
+    if (is_direct()) direct_factor(params);
     SBS_HIP(hipStreamSynchronize(stream_));
     ready_ = true;
 }
 
-template <class T> void PdEngine<T>::direct_factor() {
-    SBS_REQUIRE(system_built_, "direct factorisation requested before the system was assembled");
-    Export ex = export_system();
-    direct_ = std::make_unique<BandCholesky<T>>();
-    direct_->factor(nv_, ex.rowptr, ex.col, ex.val, stream_);
+template <class T> bool PdEngine<T>::is_direct() const {
+    return solver_ == GlobalSolver::CuSolverCholesky || solver_ == GlobalSolver::EigenCholesky ||
+           solver_ == GlobalSolver::DenseCholesky;
 }
 
-template <class T> void PdEngine<T>::direct_solve(const T* b, T* x) {
+// Band Cholesky for the sparse direct modes; per-body dense inverses with
+// FP64 refinement for DenseCholesky.
+template <class T> void PdEngine<T>::direct_factor(const PdParams& params) {
+    SBS_REQUIRE(system_built_, "direct factorisation requested before the system was assembled");
+    Export ex = export_system();
+    if (solver_ == GlobalSolver::DenseCholesky) {
+        EllView<T> A;
+        A.n = nv_;
+        A.slice_ptr = slice_ptr_.get();
+        A.slice_w = slice_w_.get();
+        A.ecol = ecol_.get();
+        A.eval = eval_.get();
+        A.diag = diagA_.get();
+        SBS_REQUIRE(params.dense_precision == 0 || params.dense_precision == 1,
+                    "PdSolver: dense_precision must be 0 (FP32) or 1 (BF16)");
+        dense_ = std::make_unique<DenseRefinedSolver<T>>();
+        dense_->setup(A, ex.rowptr, ex.col, ex.val, static_cast<DensePrecision>(params.dense_precision), stream_);
+        dense_prec_ = params.dense_precision;
+    } else {
+        direct_ = std::make_unique<BandCholesky<T>>();
+        direct_->factor(nv_, ex.rowptr, ex.col, ex.val, stream_);
+    }
+}
+
+template <class T> void PdEngine<T>::direct_solve(const T* b, T* x, const PdParams& params) {
+    const int* skip = &reinterpret_cast<CtlBlock*>(ctl_.get())->outer_done;
+    if (dense_) {
+        dense_->set_stream(stream_);
+        dense_->solve(b, x, params.dense_tol, params.dense_max_iter, skip);
+        return;
+    }
     SBS_REQUIRE(direct_ != nullptr, "direct solve requested before factorisation");
-    direct_->solve(b, x, stream_, &reinterpret_cast<CtlBlock*>(ctl_.get())->outer_done);
+    direct_->solve(b, x, stream_, skip);
+}
+
+template <class T> void PdEngine<T>::dense_info(int* bodies, int* max_body, long long* bytes, int* iterations,
+                                               double* ratio) {
+    SBS_REQUIRE(dense_ != nullptr, "dense solver not in use (set the DenseCholesky global solver)");
+    if (bodies) *bodies = dense_->bodies();
+    if (max_body) *max_body = dense_->max_body();
+    if (bytes) *bytes = (long long)dense_->inverse_bytes();
+    if (iterations || ratio) dense_->last_result(iterations, ratio);
 }
 
 // LDS of one local-step workgroup: corner forces, staged q, incidence list.
@@ -1242,8 +1281,13 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     data_ = data;
     SBS_REQUIRE(params.num_iterations >= 0, "PdSolver: numIterations must be >= 0");
     SBS_REQUIRE(params.pcg_max_iter >= 1, "PdSolver: pcg_max_iter must be >= 1");
-    const bool direct = solver_ == GlobalSolver::CuSolverCholesky || solver_ == GlobalSolver::EigenCholesky;
-    if (direct && !direct_) direct_factor();
+    const bool direct = is_direct();
+    if (direct && ((solver_ == GlobalSolver::DenseCholesky) ? (!dense_ || dense_prec_ != params.dense_precision)
+                                                             : !direct_)) {
+        direct_.reset();
+        dense_.reset();
+        direct_factor(params);
+    }
     // this step's pinned counter slot was last written two steps ago: fold it in
     const int slot = (int)(steps_ & 1);
     fold_slot(slot);
@@ -1297,7 +1341,7 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
             hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, rhs_.get(), skip);
             tick(kKRhs, false);
             tick(kKDirect, true);
-            direct_solve(rhs_.get(), sn_.get());
+            direct_solve(rhs_.get(), sn_.get(), params);
             tick(kKDirect, false);
             hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(),
                                (T*)nullptr, prev_.get(), slots_.get(), skip);
@@ -1383,10 +1427,12 @@ template <class T> int PdEngine<T>::solve(const T* b, T* x, const PdParams& para
     sync_counters();
     CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
     hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl);
-    if (direct_) {
-        direct_solve(b, x);
+    if (direct_ || dense_) {
+        direct_solve(b, x, params);
         SBS_HIP(hipStreamSynchronize(stream_));
-        return 0;
+        int it = 0;
+        if (dense_) dense_->last_result(&it, nullptr);
+        return it;
     }
     // x0 = the caller's guess (copied: x is written while x0 is still read) or 0
     SBS_REQUIRE(!use_guess || x != nullptr, "solve: guess requested without x");

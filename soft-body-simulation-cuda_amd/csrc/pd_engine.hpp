@@ -34,9 +34,10 @@
 namespace sbs {
 
 template <class T> class BandCholesky;
+template <class T> class DenseRefinedSolver;
 template <class T> class CollisionDetector;
 
-enum class GlobalSolver : int { Jacobi = 0, CuSolverCholesky = 1, EigenCholesky = 2, PCG = 3 };
+enum class GlobalSolver : int { Jacobi = 0, CuSolverCholesky = 1, EigenCholesky = 2, PCG = 3, DenseCholesky = 4 };
 
 // Mirrors the fields of SolverParams<Scalar> (src/def.h:83-100) used by the
 // PD step, plus the settings of the PCG global step.
@@ -51,7 +52,10 @@ struct PdParams {
     double pcg_floor = 1e-8;    // ... or <= pcg_floor ||D^-1 b||
     int pcg_max_iter = 100;
     int pcg_chebyshev = 0;      // apply the Chebyshev recurrence after each PCG solve
-    int pcg_adaptive = 1;       // accepted for ABI compatibility; the PCG loop stops on the device
+    int pcg_adaptive = 1;       // 1: learned launch caps; 0: every solve checked by the host (exact)
+    int dense_precision = 0;    // DenseCholesky: 0 FP32 MFMA, 1 BF16 MFMA inverse fragments
+    double dense_tol = 1e-10;   // DenseCholesky: max|D^-1 r| <= tol max|D^-1 b| (FP64 refinement)
+    int dense_max_iter = 8;     // DenseCholesky: refinement corrections per solve
 };
 
 // Caller-owned device arrays: the reference's SolverData<Scalar> (src/def.h:22-56).
@@ -148,6 +152,9 @@ public:
     void sync_counters();
     void set_kernel_timing(unsigned mask);
     KernelTime kernel_time(int id);
+    // DenseCholesky: bodies, largest body, bytes of the inverse fragments, and
+    // the last solve's corrections and residual ratio (waits for the stream).
+    void dense_info(int* bodies, int* max_body, long long* bytes, int* iterations, double* ratio);
 
 private:
     void build_chunks(const std::vector<double>& X0h);
@@ -159,8 +166,9 @@ private:
                    const int* skip, int solve_index, bool exact);
     void reset_caps_if_changed(const PdParams& p);
     static int cap_margin(int used);
-    void direct_factor();
-    void direct_solve(const T* b, T* x);
+    bool is_direct() const;
+    void direct_factor(const PdParams& params);
+    void direct_solve(const T* b, T* x, const PdParams& params);
     void tick(int id, bool start);
     void flush_timing();
 
@@ -213,6 +221,8 @@ private:
     long long step_caps_[2] = {-1, -1};  // cap epoch of each counter slot
     // direct solver (band Cholesky factor, GPU triangular solves) for small bodies
     std::unique_ptr<BandCholesky<T>> direct_;
+    std::unique_ptr<DenseRefinedSolver<T>> dense_;   // per-body dense inverses (DenseCholesky)
+    int dense_prec_ = -1;
     // collision handling after the step (built on first use)
     std::unique_ptr<CollisionDetector<T>> collision_;
     int last_iters_ = 0, last_pcg_iters_ = 0;

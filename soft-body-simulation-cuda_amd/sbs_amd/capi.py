@@ -19,6 +19,7 @@ SOLVER_JACOBI = 0
 SOLVER_CUSOLVER_CHOLESKY = 1
 SOLVER_EIGEN_CHOLESKY = 2
 SOLVER_PCG = 3
+SOLVER_DENSE_CHOLESKY = 4
 
 KERNELS = ["local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve", "rhs_gather", "pcg_iteration"]
 
@@ -47,7 +48,8 @@ class SbsSolverParams(ctypes.Structure):
     _fields_ = [("num_iterations", ctypes.c_int), ("dt", ctypes.c_double), ("gravity", ctypes.c_double),
                 ("tol", ctypes.c_double), ("rho", ctypes.c_double), ("handle_collision", ctypes.c_int),
                 ("pcg_tol", ctypes.c_double), ("pcg_floor", ctypes.c_double), ("pcg_max_iter", ctypes.c_int),
-                ("pcg_chebyshev", ctypes.c_int), ("pcg_adaptive", ctypes.c_int)]
+                ("pcg_chebyshev", ctypes.c_int), ("pcg_adaptive", ctypes.c_int), ("dense_precision", ctypes.c_int),
+                ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int)]
 
 
 class SbsPdStats(ctypes.Structure):
@@ -77,6 +79,8 @@ SIGNATURES = {
     "sbs_pd_local_step": (_I, [_VP, _VP, _VP, _I, _D, _VP]),
     "sbs_pd_system_nnz": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
     "sbs_pd_export_system": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "sbs_pd_dense_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong),
+                               ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sbs_collision_create": (_I, [_I, _I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
     "sbs_collision_destroy": (_I, [_VP]),
     "sbs_collision_set_stream": (_I, [_VP, _VP]),

@@ -24,6 +24,7 @@ class SolverType(enum.IntEnum):
     CuSolverCholesky = capi.SOLVER_CUSOLVER_CHOLESKY
     EigenCholesky = capi.SOLVER_EIGEN_CHOLESKY
     PCG = capi.SOLVER_PCG
+    DenseCholesky = capi.SOLVER_DENSE_CHOLESKY
 
 
 @dataclasses.dataclass
@@ -40,12 +41,16 @@ class SolverParams:
     pcg_max_iter: int = 100
     pcg_chebyshev: bool = False
     pcg_adaptive: bool = True
+    dense_precision: int = 0      # 0: FP32 MFMA inverse, 1: BF16 MFMA inverse
+    dense_tol: float = 1e-10
+    dense_max_iter: int = 8
 
     def to_c(self) -> capi.SbsSolverParams:
         return capi.SbsSolverParams(int(self.numIterations), float(self.dt), float(self.gravity), float(self.tol),
                                     float(self.rho), int(self.handleCollision), float(self.pcg_tol),
                                     float(self.pcg_floor), int(self.pcg_max_iter), int(self.pcg_chebyshev),
-                                    int(self.pcg_adaptive))
+                                    int(self.pcg_adaptive), int(self.dense_precision), float(self.dense_tol),
+                                    int(self.dense_max_iter))
 
 
 class SolverData:
@@ -171,6 +176,15 @@ class PdSolver:
 
     def local_step(self, d_q: int, d_sn_old: int, residual_form: bool, dt: float, d_b: int) -> None:
         capi.check(self._L.sbs_pd_local_step(self.h, d_q, d_sn_old, int(residual_form), float(dt), d_b))
+
+    def dense_info(self) -> dict:
+        """DenseCholesky mode: bodies, largest body, inverse bytes, and the last
+        solve's refinement corrections and residual ratio."""
+        b, m, it = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nb, r = ctypes.c_longlong(), ctypes.c_double()
+        capi.check(self._L.sbs_pd_dense_info(self.h, ctypes.byref(b), ctypes.byref(m), ctypes.byref(nb),
+                                             ctypes.byref(it), ctypes.byref(r)))
+        return dict(bodies=b.value, max_body=m.value, inverse_bytes=nb.value, iterations=it.value, ratio=r.value)
 
     def export_system(self) -> dict:
         dt = self.data.dtype
