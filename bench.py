@@ -176,6 +176,80 @@ def run_dense_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def batched_scene(nbodies: int = 1024, seed: int = 0):
+    """The batched workload: nbodies cubes of (nx, ny, nz) hexes drawn per
+    axis uniformly from 3..20, each with a rigid offset on a grid (no
+    overlap) and E uniform in [5e4, 5e5] Pa; top faces pinned."""
+    idx = np.arange(4 * nbodies, dtype=np.uint64)
+    u = mesh.uniform01(seed + 101, idx)
+    dims = (3 + np.floor(u[: 3 * nbodies] * 18)).astype(int).reshape(nbodies, 3)
+    E = u[3 * nbodies:] * 4.5e5 + 5e4
+    side = int(np.ceil(np.sqrt(nbodies)))
+    bodies = [dict(nx=int(d[0]), ny=int(d[1]), nz=int(d[2]), spacing=0.1,
+                   origin=(2.5 * (i % side), 0.0, 2.5 * (i // side)), jitter=1e-3, seed=seed + 7 + i,
+                   E=float(E[i]), nu=0.3, density=1000.0, pin_top=True) for i, d in enumerate(dims)]
+    return mesh.make_scene(bodies), bodies
+
+
+def run_batched_workload(args):
+    """1024 bodies of uneven size in one engine (every kernel covers all
+    bodies) against per-body engines (one launch sequence per body) on a
+    sample of the bodies, extrapolated to all of them."""
+    scene, bodies = batched_scene()
+    print(f"# batched workload: {scene.nv} vertices, {scene.nt} tets, {len(bodies)} bodies", file=sys.stderr,
+          flush=True)
+    p = SolverParams(numIterations=10, dt=1.0 / 60.0, gravity=9.8, tol=1e-6, rho=0.9992, pcg_tol=1e-6,
+                     pcg_floor=1e-8, pcg_max_iter=200, pcg_chebyshev=True, pcg_adaptive=True)
+    data = SolverData(scene, 32)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(SolverType.PCG)
+    for _ in range(args.warmup):
+        pd.Update(data, p)
+    pd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pd.Update(data, p)
+    pd.synchronize()
+    t_batched = time.perf_counter() - t0
+    Xb = data.get("X")
+    pd.close()
+    data.free()
+    # per-body engines on a sample, same steps
+    sample = list(range(0, len(bodies), max(1, len(bodies) // args.per_body_sample)))[: args.per_body_sample]
+    t_sample, errs = 0.0, []
+    for i in sample:
+        sb = mesh.make_scene([bodies[i]])
+        d = SolverData(sb, 32)
+        e = PdSolver(128, d)
+        e.SetGlobalSolver(SolverType.PCG)
+        for _ in range(args.warmup):
+            e.Update(d, p)
+        e.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            e.Update(d, p)
+        e.synchronize()
+        t_sample += time.perf_counter() - t0
+        v0, v1 = scene.body_vert_start[i], scene.body_vert_start[i + 1]
+        xb = Xb[v0:v1]
+        errs.append(float(np.abs(d.get("X") - xb).max() / max(np.abs(sb.X - xb).max(), 1e-12)))
+        e.close()
+        d.free()
+    t_per_body_all = t_sample / len(sample) * len(bodies)
+    sps_b = args.steps / t_batched
+    sps_p = args.steps / t_per_body_all
+    line = {"metric": "PD steps/sec, 1024 uneven bodies (3..20 hexes per axis) in one engine vs per-body engines",
+            "value": round(sps_b, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(1e3 * t_batched / args.steps, 4), "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "1024 bodies, Jacobi-PCG + Chebyshev, 10 it/step", "verts": scene.nv,
+                       "tets": scene.nt, "bodies": len(bodies)},
+            "per_body_engines": {"steps_per_s_extrapolated": round(sps_p, 4), "sample_bodies": len(sample),
+                                 "speedup_batched": round(sps_b / sps_p, 2),
+                                 "max_err_over_displacement_vs_batched": max(errs)}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,8 +260,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--timing-steps", type=int, default=3, help="extra steps timed kernel by kernel")
-    ap.add_argument("--workload", default="cube64", choices=["cube64", "dense256"],
-                    help="cube64: the headline 64^3 cube; dense256: 256 small bodies on the dense solver")
+    ap.add_argument("--workload", default="cube64", choices=["cube64", "dense256", "batched"],
+                    help="cube64: the headline 64^3 cube; dense256: 256 small bodies on the dense solver; "
+                         "batched: 1024 uneven bodies in one engine vs per-body engines")
+    ap.add_argument("--per-body-sample", type=int, default=16, help="bodies timed as separate engines (batched)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU (the PD step of one body on one MI355X); "
@@ -195,6 +271,9 @@ def main():
         sys.exit(2)
     if args.workload == "dense256":
         run_dense_workload(args)
+        return
+    if args.workload == "batched":
+        run_batched_workload(args)
         return
 
     scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
