@@ -34,10 +34,10 @@ METRIC = "PD steps/sec at 64^3-hex cube vs plain FP32 at stated error; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_params(fp64: bool = False) -> SolverParams:
+def workload_params(fp64: bool = False, halo: bool = True) -> SolverParams:
     return SolverParams(numIterations=20, dt=1.0 / 60.0, gravity=9.8, tol=1e-6, rho=0.9992, pcg_tol=1e-6,
                         pcg_floor=0.0 if fp64 else 1e-8, pcg_max_iter=200, pcg_chebyshev=True,
-                        pcg_adaptive=True)
+                        pcg_adaptive=True, pcg_halo=halo)
 
 
 def local_step_bytes(nt: int, nv: int) -> int:
@@ -55,7 +55,8 @@ def pcg_iteration_bytes(nnz_off: int, nv: int) -> int:
 
 TRAFFIC_FILE = "profiles/pd_step_64cube_fp32_kernels.json"
 _TRAFFIC_KERNELS = {"local_step": ["local_chunk_kernel<float>"],
-                    "pcg_iteration": ["pcg_iter_kernel<float, false>", "pcg_iter_kernel<float, true>"]}
+                    "pcg_iteration": ["pcg_iter_halo_kernel<float, false>", "pcg_iter_halo_kernel<float, true>"],
+                    "pcg_iteration_gather": ["pcg_iter_kernel<float, false>", "pcg_iter_kernel<float, true>"]}
 
 
 def measured_traffic(kernel: str):
@@ -73,11 +74,11 @@ def measured_traffic(kernel: str):
     return int(sum(p["mean_mb"] * p["dispatches"] for p in parts) / n * 1e6)
 
 
-def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int):
+def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int, halo: bool = True):
     data = SolverData(scene, precision)
     pd = PdSolver(128, data)
     pd.SetGlobalSolver(SolverType.PCG)
-    p = workload_params(precision == 64)
+    p = workload_params(precision == 64, halo)
     for _ in range(warmup):
         pd.Update(data, p)
     pd.synchronize()
@@ -286,6 +287,8 @@ def main():
                     help="cube64: the headline 64^3 cube; dense256: 256 small bodies on the dense solver; "
                          "batched: 1024 uneven bodies in one engine vs per-body engines")
     ap.add_argument("--per-body-sample", type=int, default=16, help="bodies timed as separate engines (batched)")
+    ap.add_argument("--pcg-kernels", default="halo", choices=["halo", "gather"],
+                    help="PCG kernel family: halo-staged (LDS) or per-row neighbour gathers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU (the PD step of one body on one MI355X); "
@@ -300,13 +303,14 @@ def main():
 
     scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
     print(f"# scene: {scene.nv} vertices, {scene.nt} tets", file=sys.stderr, flush=True)
-    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=args.timing_steps)
+    halo = args.pcg_kernels == "halo"
+    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=args.timing_steps, halo=halo)
     print(f"# gpu fp32: {args.steps} steps in {g['seconds']:.3f} s", file=sys.stderr, flush=True)
     sps = args.steps / g["seconds"]
 
     err = None
     if not args.no_fp64:
-        r = run_gpu(scene, args.steps, args.warmup, 64, timing_steps=0)
+        r = run_gpu(scene, args.steps, args.warmup, 64, timing_steps=0, halo=halo)
         print(f"# gpu fp64 reference run: {r['seconds']:.3f} s", file=sys.stderr, flush=True)
         disp = np.abs(r["X"] - scene.X).max()
         err = dict(rel_l2=float(np.linalg.norm(g["X"] - r["X"]) / np.linalg.norm(r["X"])),
@@ -333,7 +337,7 @@ def main():
             kinfo[k] = d
         dom = max(per_launch, key=lambda k: kt[k][1])
         ach = kinfo[dom]["gbs"]
-        traffic = measured_traffic(dom)
+        traffic = measured_traffic(dom if halo or dom != "pcg_iteration" else "pcg_iteration_gather")
         roof = dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic,
                     traffic_source=TRAFFIC_FILE if traffic is not None else None,
@@ -353,7 +357,8 @@ def main():
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.n}^3-hex cube PD step (Jacobi-PCG + Chebyshev, 20 it/step)",
                    "verts": scene.nv, "tets": scene.nt, "nnz_scalar": g["nnz_off"] + scene.nv,
-                   "iterations_per_step": 20, "dt": 1.0 / 60.0, "pcg_tol": 1e-6, "pcg_floor": 1e-8},
+                   "iterations_per_step": 20, "dt": 1.0 / 60.0, "pcg_tol": 1e-6, "pcg_floor": 1e-8,
+                   "pcg_kernels": args.pcg_kernels, "halo_groups": int(g["stats"].halo_groups)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "error_vs_fp64": err,

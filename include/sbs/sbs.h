@@ -99,6 +99,7 @@ typedef struct SbsSolverParams {
     int dense_precision;  /* dense solver: 0 FP32 MFMA, 1 BF16 MFMA            */
     double dense_tol;     /* dense solver: max|D^-1 r| <= tol max|D^-1 b|      */
     int dense_max_iter;   /* dense solver: FP64 refinement corrections per solve */
+    int pcg_halo;         /* PCG kernels: 1 halo-staged (LDS), 0 per-row gathers */
 } SbsSolverParams;
 
 typedef struct SbsPd SbsPd;
@@ -111,6 +112,7 @@ typedef struct SbsPdStats {
     long long nnz_offdiag;        /* off-diagonal nonzeros of the scalar matrix */
     long long pcg_iterations_total; /* PCG iterations over all completed steps  */
     long long pcg_solves_total;     /* PCG solves over all completed steps      */
+    int halo_groups;                /* workgroups of the halo-staged PCG (0: mesh not tiled) */
 } SbsPdStats;
 
 const char* sbs_last_error(void);

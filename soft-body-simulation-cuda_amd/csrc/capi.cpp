@@ -93,6 +93,7 @@ sbs::PdParams to_params(const SbsSolverParams* p) {
     o.dense_precision = p->dense_precision;
     o.dense_tol = p->dense_tol;
     o.dense_max_iter = p->dense_max_iter;
+    o.pcg_halo = p->pcg_halo;
     return o;
 }
 
@@ -196,6 +197,7 @@ int sbs_pd_stats(SbsPd* pd, SbsPdStats* out) {
             out->nnz_offdiag = (long long)e.nnz_offdiag();
             out->pcg_iterations_total = e.pcg_iterations_total();
             out->pcg_solves_total = e.pcg_solves_total();
+            out->halo_groups = e.halo_groups();
         });
     });
 }

@@ -9,6 +9,7 @@
 #include "collision.hpp"
 #include "dense_refine.hpp"
 #include "direct_solver.hpp"
+#include "halo_tiles.hpp"
 #include "pd_engine.hpp"
 #include "svd3.hpp"
 
@@ -49,10 +50,12 @@ template <class T> __device__ __forceinline__ void st3(T* __restrict__ p, size_t
 }
 
 // The PCG vectors a row's neighbours read (r, w, s and D^-1) live in one
-// 12-scalar record per vertex: [r xyz, w xyz, s xyz, D^-1, 0, 0], so a
+// 12-scalar record per row: [r xyz, D^-1 | w xyz, 0 | s xyz, 0], so a
 // neighbour costs three aligned 16-byte (float) loads from one or two cache
-// lines instead of four gathers from four arrays.
+// lines instead of four gathers from four arrays, and r with D^-1 (all that
+// u = D^-1 r needs) is the first 16 bytes.
 constexpr int kRec = 12;
+constexpr int kRecR = 0, kRecD = 3, kRecW = 4, kRecS = 8;
 template <class T> struct Rec {
     T v[kRec];
 };
@@ -71,6 +74,27 @@ template <class T> __device__ __forceinline__ Rec<T> ld_rec(const T* __restrict_
         }
     }
     return r;
+}
+// r and D^-1 of a record (its first 4 scalars).
+template <class T>
+__device__ __forceinline__ void ld_rec_rd(const T* __restrict__ base, size_t i, T& r0, T& r1, T& r2, T& d) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(base + kRec * i);
+        r0 = q.x; r1 = q.y; r2 = q.z; d = q.w;
+    } else {
+        const double2 q0 = *reinterpret_cast<const double2*>(base + kRec * i);
+        const double2 q1 = *reinterpret_cast<const double2*>(base + kRec * i + 2);
+        r0 = q0.x; r1 = q0.y; r2 = q1.x; d = q1.y;
+    }
+}
+// w of a record (scalars 4..7, the pad written 0).
+template <class T> __device__ __forceinline__ void st_rec_w(T* __restrict__ base, size_t i, T w0, T w1, T w2) {
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(base + kRec * i + kRecW) = make_float4(w0, w1, w2, 0.f);
+    } else {
+        *reinterpret_cast<double2*>(base + kRec * i + kRecW) = make_double2(w0, w1);
+        *reinterpret_cast<double2*>(base + kRec * i + kRecW + 2) = make_double2(w2, 0.0);
+    }
 }
 template <class T> __device__ __forceinline__ void st_rec(T* __restrict__ base, size_t i, const Rec<T>& r) {
     using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
@@ -578,6 +602,29 @@ __device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const
 // iterates are identical.  A component stops when
 // ||u_k|| <= max(tol ||u_0||, floor ||D^-1 b||); launches after the stop
 // return at once.
+//
+// Two kernel families run this iteration: the row-gather kernels below read
+// each neighbour's record from memory per nonzero; the halo-staged kernels
+// further down stage a workgroup's whole neighbourhood in LDS once per launch.
+
+// Halo tiles: rows renumbered along a Morton curve of the rest positions
+// (internal row i = vertex perm[i]) and grouped into workgroups of up to kTpb
+// consecutive internal rows; a workgroup's halo is the ascending list of the
+// internal rows its rows touch (their own and their neighbours').
+constexpr int kHaloStride = 768;              // halo entries (LDS slots) per workgroup
+constexpr int kHaloPer = kHaloStride / kTpb;  // halo entries staged per thread
+constexpr int kGrpInts = 8;                   // per workgroup: row0, rows, halo size, width, first entry
+template <class T> struct HaloView {
+    int groups = 0;
+    const int* grp = nullptr;
+    const uint32_t* halo = nullptr;    // groups x kHaloStride internal rows
+    const uint32_t* halo_v = nullptr;  // the same entries as vertex ids
+    const uint16_t* col = nullptr;     // entry e of thread t at first + e * kTpb + t: halo-local column
+    const T* val = nullptr;            // its value (diagonal included, padding 0)
+    const uint32_t* perm = nullptr;    // internal row -> vertex
+    T* x = nullptr;                    // the iterate in internal order
+};
+
 template <class T> struct PcgArgs {
     int nv;
     int gathered;             // 1: rhs from the local-step partials (+ pins), 0: plain b
@@ -602,6 +649,7 @@ template <class T> struct PcgArgs {
     double tol2;
     double floor2;
     const int* skip;          // outer loop already converged
+    HaloView<T> h;            // halo-staged kernels only: records and p are then in internal order
 };
 
 __device__ __forceinline__ double* iter_slot(double* slots, int i) {
@@ -631,9 +679,9 @@ __global__ void __launch_bounds__(kTpb) pcg_init1_kernel(PcgArgs<T> a) {
         st3(a.x, v, x0, x1, x2);
         const T di = a.dinv[v];
         Rec<T> rc;
-        rc.v[0] = b0 - ax0; rc.v[1] = b1 - ax1; rc.v[2] = b2 - ax2;
-        for (int k = 3; k < kRec; ++k) rc.v[k] = (T)0;
-        rc.v[9] = di;
+        for (int k = 0; k < kRec; ++k) rc.v[k] = (T)0;
+        rc.v[kRecR] = b0 - ax0; rc.v[kRecR + 1] = b1 - ax1; rc.v[kRecR + 2] = b2 - ax2;
+        rc.v[kRecD] = di;
         st_rec(a.rec[0], v, rc);
         const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
         vals[0] = d0 * d0; vals[1] = d1 * d1; vals[2] = d2 * d2;
@@ -648,17 +696,18 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_kernel(PcgArgs<T> a) {
     double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (v < (unsigned)a.nv) {
         T* rec = a.rec[0];
-        const T r0 = rec[kRec * (size_t)v], r1 = rec[kRec * (size_t)v + 1], r2 = rec[kRec * (size_t)v + 2];
-        const T dv = rec[kRec * (size_t)v + 9], d = a.diagA[v];
+        T r0, r1, r2, dv;
+        ld_rec_rd(rec, v, r0, r1, r2, dv);
+        const T d = a.diagA[v];
         const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
         T w0 = d * u0, w1 = d * u1, w2 = d * u2;
         ell_row(a.slice_ptr, a.slice_w, a.ecol, a.eval, v, [&](uint32_t n, T m) {
-            const T q0 = rec[kRec * (size_t)n], q1 = rec[kRec * (size_t)n + 1], q2 = rec[kRec * (size_t)n + 2];
-            const T dn = rec[kRec * (size_t)n + 9];
+            T q0, q1, q2, dn;
+            ld_rec_rd(rec, n, q0, q1, q2, dn);
             w0 += m * (dn * q0); w1 += m * (dn * q1); w2 += m * (dn * q2);
         });
         // w0 into the record (neighbours read only r and D^-1 of it in this launch)
-        rec[kRec * (size_t)v + 3] = w0; rec[kRec * (size_t)v + 4] = w1; rec[kRec * (size_t)v + 5] = w2;
+        st_rec_w(rec, v, w0, w1, w2);
         vals[0] = (double)r0 * u0; vals[1] = (double)r1 * u1; vals[2] = (double)r2 * u2;
         vals[3] = (double)w0 * u0; vals[4] = (double)w1 * u1; vals[5] = (double)w2 * u2;
         vals[6] = (double)u0 * u0; vals[7] = (double)u1 * u1; vals[8] = (double)u2 * u2;
@@ -730,15 +779,10 @@ constexpr int kIterBatch = SBS_ITER_BATCH;  // neighbours in flight per row in t
 // 5 wavefronts per SIMD keep every row of the 64^3 cube resident at once
 // (4292 wavefronts on 1024 SIMDs).
 template <class T, bool kFirst>
-__global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) pcg_iter_kernel(PcgArgs<T> a, int i,
-                                                                                               int probe) {
+__global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) pcg_iter_kernel(PcgArgs<T> a, int i) {
     if (a.skip && *a.skip) return;
     if (!kFirst && a.ctl->st[(i - 1) & 1].done) {  // stopped earlier: carry the state forward
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[i & 1] = a.ctl->st[(i - 1) & 1];
-        return;
-    }
-    if (probe) {
-        (void)pcg_state(a, i);
         return;
     }
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -747,9 +791,10 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
     const int rd = i & 1, wr = rd ^ 1;
     const T* __restrict__ rec_in = a.rec[rd];
     const Rec<T> own = ld_rec(rec_in, vr);
-    const T r0 = own.v[0], r1 = own.v[1], r2 = own.v[2], w0 = own.v[3], w1 = own.v[4], w2 = own.v[5];
-    T s0 = own.v[6], s1 = own.v[7], s2 = own.v[8];
-    const T dv = own.v[9];
+    const T r0 = own.v[kRecR], r1 = own.v[kRecR + 1], r2 = own.v[kRecR + 2];
+    const T w0 = own.v[kRecW], w1 = own.v[kRecW + 1], w2 = own.v[kRecW + 2];
+    T s0 = own.v[kRecS], s1 = own.v[kRecS + 1], s2 = own.v[kRecS + 2];
+    const T dv = own.v[kRecD];
     T x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
     if (!kFirst) ld3(a.p, vr, p0, p1, p2);
     ld3(a.x, vr, x0, x1, x2);
@@ -757,10 +802,10 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
     T yr0 = 0, yr1 = 0, yr2 = 0, yw0 = 0, yw1 = 0, yw2 = 0, ys0 = 0, ys1 = 0, ys2 = 0;
     ell_row<kIterBatch>(a.slice_ptr, a.slice_w, a.ecol, a.eval, vr, [&](uint32_t n, T m) {
         const Rec<T> nb = ld_rec(rec_in, n);
-        const T md = m * nb.v[9];
-        yr0 += md * nb.v[0]; yr1 += md * nb.v[1]; yr2 += md * nb.v[2];
-        yw0 += md * nb.v[3]; yw1 += md * nb.v[4]; yw2 += md * nb.v[5];
-        if (!kFirst) { ys0 += md * nb.v[6]; ys1 += md * nb.v[7]; ys2 += md * nb.v[8]; }
+        const T md = m * nb.v[kRecD];
+        yr0 += md * nb.v[kRecR]; yr1 += md * nb.v[kRecR + 1]; yr2 += md * nb.v[kRecR + 2];
+        yw0 += md * nb.v[kRecW]; yw1 += md * nb.v[kRecW + 1]; yw2 += md * nb.v[kRecW + 2];
+        if (!kFirst) { ys0 += md * nb.v[kRecS]; ys1 += md * nb.v[kRecS + 1]; ys2 += md * nb.v[kRecS + 2]; }
     });
     const PcgState st = pcg_state(a, i);
     if (st.done) return;
@@ -791,10 +836,9 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
         st3(a.p, v, p0, p1, p2);
         st3(a.x, v, x0, x1, x2);
         Rec<T> out;
-        out.v[0] = rn0; out.v[1] = rn1; out.v[2] = rn2;
-        out.v[3] = y0; out.v[4] = y1; out.v[5] = y2;
-        out.v[6] = s0; out.v[7] = s1; out.v[8] = s2;
-        out.v[9] = dv; out.v[10] = (T)0; out.v[11] = (T)0;
+        out.v[kRecR] = rn0; out.v[kRecR + 1] = rn1; out.v[kRecR + 2] = rn2; out.v[kRecD] = dv;
+        out.v[kRecW] = y0; out.v[kRecW + 1] = y1; out.v[kRecW + 2] = y2; out.v[kRecW + 3] = (T)0;
+        out.v[kRecS] = s0; out.v[kRecS + 1] = s1; out.v[kRecS + 2] = s2; out.v[kRecS + 3] = (T)0;
         st_rec(a.rec[wr], v, out);
         vals[0] = (double)rn0 * un0; vals[1] = (double)rn1 * un1; vals[2] = (double)rn2 * un2;
         vals[3] = (double)y0 * un0; vals[4] = (double)y1 * un1; vals[5] = (double)y2 * un2;
@@ -803,23 +847,35 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
     stripe_add<9>(vals, iter_slot(a.slots, i));
 }
 
-// After the probe K(L): bookkeeping (block 0), and with `cheb` the
+// Solve log and counters of the control block; st is the state after the
+// L launched iterations.
+__device__ __forceinline__ void pcg_log_solve(CtlBlock* c, const PcgState& st, int L) {
+    const int its = st.done ? st.iters : L;
+    const bool cut = st.unconverged || !st.done;  // max_iter or the launch cap
+    if (c->solves < kMaxSolveLog) c->log[c->solves] = cut ? -its - 1 : its;
+    c->pcg_iters += its;
+    c->solves += 1;
+    c->last_iters = its;
+    if (its > c->max_used) c->max_used = its;
+    if (cut) c->unconverged += 1;
+}
+
+// State after the L launched iterations (the stop test of iteration L,
+// evaluated by the first wavefront of block 0 only).
+template <class T> __device__ __forceinline__ void pcg_close(const PcgArgs<T>& a, int L) {
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+    PcgState st;
+    if (L > 0 && a.ctl->st[(L - 1) & 1].done) st = a.ctl->st[(L - 1) & 1];
+    else st = pcg_state(a, L);
+    if (threadIdx.x == 0) pcg_log_solve(a.ctl, st, L);
+}
+
+// After K(L-1): the stop test of iteration L and bookkeeping (block 0), and with `cheb` the
 // Chebyshev recurrence of the global step (pdSolver.cu:196-205) into sn/prev.
 template <class T>
 __global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, int cheb) {
     if (a.skip && *a.skip) return;
-    const PcgState st = a.ctl->st[L & 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        CtlBlock* c = a.ctl;
-        const int its = st.done ? st.iters : L;
-        const bool cut = st.unconverged || !st.done;  // max_iter or the launch cap
-        if (c->solves < kMaxSolveLog) c->log[c->solves] = cut ? -its - 1 : its;
-        c->pcg_iters += its;
-        c->solves += 1;
-        c->last_iters = its;
-        if (its > c->max_used) c->max_used = its;
-        if (cut) c->unconverged += 1;
-    }
+    pcg_close(a, L);
     if (!cheb) return;
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v < (unsigned)a.nv) {
@@ -832,6 +888,285 @@ __global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, i
             a.sn[i] = nx;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Halo-staged PCG kernels (same algorithm and stop test as above).  A launch
+// stages what the product needs at every halo row into LDS once, then each
+// thread forms its row's product from LDS through 16-bit halo-local columns,
+// so a row's neighbours cost LDS reads instead of dependent memory round
+// trips.  In the iteration the staged quantity is u_{i+1} = D^-1 r_{i+1} of
+// the halo row, recomputed from its record (r_i, w_i, s_{i-1}, D^-1) with
+// the owner's exact operation sequence, so every copy equals the owner's.
+// Records, p and x are kept in internal order: a workgroup's own rows are
+// contiguous and its halo is a few contiguous runs.
+
+// kEllBatch entries of row t of the workgroup (columns and values), from
+// entry e0 on; positions past the row's width repeat its last entry with value 0.
+template <class T> struct RowBatch {
+    unsigned c[kEllBatch];
+    T m[kEllBatch];
+};
+template <class T>
+__device__ __forceinline__ void row_batch(const HaloView<T>& h, int first, int width, int t, int e0, RowBatch<T>& b) {
+    const uint16_t* __restrict__ c = h.col + first + t;
+    const T* __restrict__ m = h.val + first + t;
+#pragma unroll
+    for (int j = 0; j < kEllBatch; ++j) {
+        const int e = min(e0 + j, width - 1);
+        b.c[j] = c[(size_t)e * kTpb];
+        const T mv = m[(size_t)e * kTpb];
+        b.m[j] = e0 + j < width ? mv : (T)0;
+    }
+}
+template <class T>
+__device__ __forceinline__ void row_apply(const RowBatch<T>& b, T (*lu)[kHaloStride], T& y0, T& y1, T& y2) {
+#pragma unroll
+    for (int j = 0; j < kEllBatch; ++j) {
+        y0 += b.m[j] * lu[0][b.c[j]];
+        y1 += b.m[j] * lu[1][b.c[j]];
+        y2 += b.m[j] * lu[2][b.c[j]];
+    }
+}
+// Row t of the workgroup: y += sum over its entries of m * lu[.][col], the
+// first batch already loaded (before the LDS stage, so its memory latency
+// overlaps the stage's).
+template <class T>
+__device__ __forceinline__ void halo_row(const HaloView<T>& h, int first, int width, int t, const RowBatch<T>& b0,
+                                         T (*lu)[kHaloStride], T& y0, T& y1, T& y2) {
+    row_apply(b0, lu, y0, y1, y2);
+    for (int e0 = kEllBatch; e0 < width; e0 += kEllBatch) {
+        RowBatch<T> b;
+        row_batch(h, first, width, t, e0, b);
+        row_apply(b, lu, y0, y1, y2);
+    }
+}
+
+struct HaloGroup {
+    int row0, rows, hn, width, first;
+};
+template <class T> __device__ __forceinline__ HaloGroup halo_group(const HaloView<T>& h, unsigned g) {
+    const int* gd = h.grp + kGrpInts * (size_t)g;
+    return HaloGroup{gd[0], gd[1], gd[2], gd[3], gd[4]};
+}
+
+// init1: r0 = b - A x0 (x0 staged at the halo), x = x0, (D^-1 b)^2.
+template <class T>
+__global__ void __launch_bounds__(kTpb) pcg_init1_halo_kernel(PcgArgs<T> a) {
+    if (a.skip && *a.skip) return;
+    __shared__ T lx[3][kHaloStride];
+    const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+    const HaloGroup G = halo_group(a.h, g);
+    const int t = threadIdx.x;
+    const bool row = t < G.rows;
+    const size_t ir = (size_t)G.row0 + (row ? t : 0);
+    const uint32_t v = a.h.perm[ir];
+    T b0, b1, b2;
+    if (a.gathered) a.g.at(v, b0, b1, b2);
+    else ld3(a.b, v, b0, b1, b2);
+    T ax0 = 0, ax1 = 0, ax2 = 0, x0 = 0, x1 = 0, x2 = 0;
+    if (a.x0) {
+        RowBatch<T> b0;
+        row_batch(a.h, G.first, G.width, t, 0, b0);
+        const uint32_t* __restrict__ hv = a.h.halo_v + (size_t)g * kHaloStride;
+#pragma unroll
+        for (int k = 0; k < kHaloPer; ++k) {
+            const int e = t + k * kTpb;
+            if (e < G.hn) {
+                T q0, q1, q2;
+                ld3(a.x0, hv[e], q0, q1, q2);
+                lx[0][e] = q0; lx[1][e] = q1; lx[2][e] = q2;
+            }
+        }
+        ld3(a.x0, v, x0, x1, x2);
+        __syncthreads();
+        halo_row(a.h, G.first, G.width, t, b0, lx, ax0, ax1, ax2);
+    }
+    double vals[3] = {0, 0, 0};
+    if (row) {
+        st3(a.h.x, ir, x0, x1, x2);
+        const T di = a.dinv[v];
+        Rec<T> rc;
+        for (int k = 0; k < kRec; ++k) rc.v[k] = (T)0;
+        rc.v[kRecR] = b0 - ax0; rc.v[kRecR + 1] = b1 - ax1; rc.v[kRecR + 2] = b2 - ax2;
+        rc.v[kRecD] = di;
+        st_rec(a.rec[0], ir, rc);
+        const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
+        vals[0] = d0 * d0; vals[1] = d1 * d1; vals[2] = d2 * d2;
+    }
+    stripe_add<3>(vals, a.slots + kSlotInit * kSlotSize);
+}
+
+// init2: u0 = D^-1 r0 staged at the halo, w0 = A u0 into the record,
+// gamma0 = r0.u0, delta0 = w0.u0, u0.u0.  Other workgroups read only the
+// first 16 bytes (r, D^-1) of the records this launch writes w into.
+template <class T>
+__global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
+    if (a.skip && *a.skip) return;
+    __shared__ T lu[3][kHaloStride];
+    const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+    const HaloGroup G = halo_group(a.h, g);
+    const int t = threadIdx.x;
+    const bool row = t < G.rows;
+    const size_t ir = (size_t)G.row0 + (row ? t : 0);
+    T* rec = a.rec[0];
+    RowBatch<T> b0;
+    row_batch(a.h, G.first, G.width, t, 0, b0);
+    const uint32_t* __restrict__ hh = a.h.halo + (size_t)g * kHaloStride;
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+        const int e = t + k * kTpb;
+        if (e < G.hn) {
+            T q0, q1, q2, dq;
+            ld_rec_rd(rec, hh[e], q0, q1, q2, dq);
+            lu[0][e] = dq * q0; lu[1][e] = dq * q1; lu[2][e] = dq * q2;
+        }
+    }
+    T r0, r1, r2, dv;
+    ld_rec_rd(rec, ir, r0, r1, r2, dv);
+    __syncthreads();
+    T w0 = 0, w1 = 0, w2 = 0;
+    halo_row(a.h, G.first, G.width, t, b0, lu, w0, w1, w2);
+    double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (row) {
+        const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
+        st_rec_w(rec, ir, w0, w1, w2);
+        vals[0] = (double)r0 * u0; vals[1] = (double)r1 * u1; vals[2] = (double)r2 * u2;
+        vals[3] = (double)w0 * u0; vals[4] = (double)w1 * u1; vals[5] = (double)w2 * u2;
+        vals[6] = (double)u0 * u0; vals[7] = (double)u1 * u1; vals[8] = (double)u2 * u2;
+    }
+    stripe_add<9>(vals, iter_slot(a.slots, -1));
+}
+
+// One PCG iteration K(i), halo-staged.  The halo's row indices are requested
+// with the reduction stripes of the previous launch; once the scalars are
+// formed the halo records, the row's own record, p and x are requested
+// together, so the launch costs about two dependent memory round trips.  The
+// iteration that finds the solve converged still runs the LDS stage (its
+// result is discarded) so that every thread reaches the barrier.
+template <class T, bool kFirst>
+__global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
+    if (a.skip && *a.skip) return;
+    if (!kFirst && a.ctl->st[(i - 1) & 1].done) {  // stopped earlier: carry the state forward
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[i & 1] = a.ctl->st[(i - 1) & 1];
+        return;
+    }
+    __shared__ T lu[3][kHaloStride];
+    const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+    const HaloGroup G = halo_group(a.h, g);
+    const int t = threadIdx.x;
+    const bool row = t < G.rows;
+    const size_t ir = (size_t)G.row0 + (row ? t : 0);
+    const int rd = i & 1, wr = rd ^ 1;
+    const T* __restrict__ rec_in = a.rec[rd];
+    const uint32_t* __restrict__ hh = a.h.halo + (size_t)g * kHaloStride;
+    uint32_t hidx[kHaloPer];
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) hidx[k] = hh[t + k * kTpb];  // in bounds: kHaloStride per workgroup
+    const PcgState st = pcg_state(a, i);
+    const bool live = !st.done;
+    Rec<T> hr[kHaloPer];
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+        const int e = t + k * kTpb;
+        if (e < G.hn) hr[k] = ld_rec(rec_in, hidx[k]);
+    }
+    const Rec<T> own = ld_rec(rec_in, ir);
+    T x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
+    if (!kFirst) ld3(a.p, ir, p0, p1, p2);
+    ld3(a.h.x, ir, x0, x1, x2);
+    RowBatch<T> b0;
+    row_batch(a.h, G.first, G.width, t, 0, b0);
+    const T al0 = (T)st.alpha[0], al1 = (T)st.alpha[1], al2 = (T)st.alpha[2];
+    const T be0 = (T)st.beta[0], be1 = (T)st.beta[1], be2 = (T)st.beta[2];
+    // u_{i+1} of the halo rows: s_i = w_i + beta s_{i-1}, r_{i+1} = r_i - alpha s_i
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+        const int e = t + k * kTpb;
+        if (e < G.hn) {
+            const Rec<T>& q = hr[k];
+            const T dq = q.v[kRecD];
+            T sq0 = q.v[kRecW], sq1 = q.v[kRecW + 1], sq2 = q.v[kRecW + 2];
+            if (!kFirst) {
+                sq0 = fma(be0, q.v[kRecS], sq0); sq1 = fma(be1, q.v[kRecS + 1], sq1);
+                sq2 = fma(be2, q.v[kRecS + 2], sq2);
+            }
+            lu[0][e] = dq * fma(-al0, sq0, q.v[kRecR]);
+            lu[1][e] = dq * fma(-al1, sq1, q.v[kRecR + 1]);
+            lu[2][e] = dq * fma(-al2, sq2, q.v[kRecR + 2]);
+        }
+    }
+    if (live && blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
+        double* z = iter_slot(a.slots, i + 1);
+        for (int k = threadIdx.x; k < kSlotSize; k += kTpb) z[k] = 0.0;
+    }
+    __syncthreads();
+    T y0 = 0, y1 = 0, y2 = 0;  // w_{i+1} = A u_{i+1}
+    halo_row(a.h, G.first, G.width, t, b0, lu, y0, y1, y2);
+    if (!live) return;  // uniform over the grid
+    double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (row) {
+        const T r0 = own.v[kRecR], r1 = own.v[kRecR + 1], r2 = own.v[kRecR + 2], dv = own.v[kRecD];
+        const T w0 = own.v[kRecW], w1 = own.v[kRecW + 1], w2 = own.v[kRecW + 2];
+        const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
+        T s0, s1, s2;
+        if (!kFirst) {
+            p0 = fma(be0, p0, u0); p1 = fma(be1, p1, u1); p2 = fma(be2, p2, u2);
+            s0 = fma(be0, own.v[kRecS], w0); s1 = fma(be1, own.v[kRecS + 1], w1); s2 = fma(be2, own.v[kRecS + 2], w2);
+        } else {
+            p0 = u0; p1 = u1; p2 = u2;
+            s0 = w0; s1 = w1; s2 = w2;
+        }
+        x0 = fma(al0, p0, x0); x1 = fma(al1, p1, x1); x2 = fma(al2, p2, x2);
+        const T rn0 = fma(-al0, s0, r0), rn1 = fma(-al1, s1, r1), rn2 = fma(-al2, s2, r2);
+        const T un0 = dv * rn0, un1 = dv * rn1, un2 = dv * rn2;
+        st3(a.p, ir, p0, p1, p2);
+        st3(a.h.x, ir, x0, x1, x2);
+        Rec<T> out;
+        out.v[kRecR] = rn0; out.v[kRecR + 1] = rn1; out.v[kRecR + 2] = rn2; out.v[kRecD] = dv;
+        out.v[kRecW] = y0; out.v[kRecW + 1] = y1; out.v[kRecW + 2] = y2; out.v[kRecW + 3] = (T)0;
+        out.v[kRecS] = s0; out.v[kRecS + 1] = s1; out.v[kRecS + 2] = s2; out.v[kRecS + 3] = (T)0;
+        st_rec(a.rec[wr], ir, out);
+        vals[0] = (double)rn0 * un0; vals[1] = (double)rn1 * un1; vals[2] = (double)rn2 * un2;
+        vals[3] = (double)y0 * un0; vals[4] = (double)y1 * un1; vals[5] = (double)y2 * un2;
+        vals[6] = (double)un0 * un0; vals[7] = (double)un1 * un1; vals[8] = (double)un2 * un2;
+    }
+    stripe_add<9>(vals, iter_slot(a.slots, i));
+}
+
+// After K(L-1): the stop test of iteration L and bookkeeping (block 0), the iterate back to vertex order, and
+// with `cheb` the Chebyshev recurrence (pdSolver.cu:196-205) into sn/prev.
+template <class T>
+__global__ void __launch_bounds__(kTpb) pcg_finish_halo_kernel(PcgArgs<T> a, int L, int cheb) {
+    if (a.skip && *a.skip) return;
+    pcg_close(a, L);
+    const size_t ir = (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (ir >= (size_t)a.nv) return;
+    const uint32_t v = a.h.perm[ir];
+    T xv[3];
+    ld3(a.h.x, ir, xv[0], xv[1], xv[2]);
+    st3(a.x, v, xv[0], xv[1], xv[2]);
+    if (!cheb) return;
+    for (int k = 0; k < 3; ++k) {
+        const size_t j = 3 * (size_t)v + k;
+        const T s = a.sn[j];
+        T nx = (T)0.9 * (xv[k] - s) + s;
+        nx = (nx - a.prev[j]) * a.omega + a.prev[j];
+        a.prev[j] = s;
+        a.sn[j] = nx;
+    }
+}
+
+// Entry values of the halo tiles from the assembled matrix: src >= 0 an
+// off-diagonal sliced-ELL position, -1 padding, <= -2 the diagonal of row -2-src.
+template <class T>
+__global__ void halo_values_kernel(size_t n, const int* __restrict__ src, const T* __restrict__ eval,
+                                   const T* __restrict__ diagA, T* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = src[i];
+    out[i] = s >= 0 ? eval[s] : (s == -1 ? (T)0 : diagA[-2 - s]);
 }
 
 // Early exit of the local/global loop from the computeError sum (pdSolver.cu:164).
@@ -873,6 +1208,16 @@ template <class T> PdEngine<T>::PdEngine(int threads_per_block, const PdData<T>&
     SBS_HIP(hipMemcpy(X0t.data(), data.X0, X0t.size() * sizeof(T), hipMemcpyDeviceToHost));
     std::vector<double> X0h(X0t.begin(), X0t.end());
     build_chunks(X0h);
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int v = 0; v < nv_; ++v)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], X0h[3 * (size_t)v + k]);
+                hi[k] = std::max(hi[k], X0h[3 * (size_t)v + k]);
+            }
+        vertex_key_.resize(nv_);
+        for (int v = 0; v < nv_; ++v) vertex_key_[v] = morton_code(&X0h[3 * (size_t)v], lo, hi);
+    }
     dminv_.alloc(9 * (size_t)nt_);
     w_.alloc(nt_);
     v0_.alloc(nt_);
@@ -1062,6 +1407,66 @@ template <class T> void PdEngine<T>::build_structure() {
     SBS_HIP(hipMemcpy(rowlen_.get(), rowlen.data(), rowlen.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!ecol.empty())
         SBS_HIP(hipMemcpy(ecol_.get(), ecol.data(), ecol.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    build_halo(rowstart, keys, sptr);
+}
+
+// Halo tiles over the full pattern (diagonal first in each row).  A mesh
+// whose rows cannot be tiled (one row touching more than kHaloStride rows, or
+// more entries than 32-bit positions hold) keeps the row-gather kernels.
+template <class T>
+void PdEngine<T>::build_halo(const std::vector<int>& rowstart, const std::vector<uint64_t>& keys,
+                             const std::vector<int>& sptr) {
+    halo_groups_ = 0;
+    std::vector<int> rs(nv_ + 1, 0);
+    std::vector<uint32_t> cols;
+    std::vector<int64_t> vidx;  // value code of each CSR position
+    cols.reserve(keys.size() + nv_);
+    vidx.reserve(keys.size() + nv_);
+    for (int v = 0; v < nv_; ++v) {
+        cols.push_back((uint32_t)v);
+        vidx.push_back(-2 - (int64_t)v);
+        const int s = v / kSlice, lane = v % kSlice;
+        for (int e = 0; e < rowstart[v + 1] - rowstart[v]; ++e) {
+            cols.push_back((uint32_t)(keys[rowstart[v] + e] & 0xffffffffu));
+            vidx.push_back((int64_t)sptr[s] + (int64_t)e * kSlice + lane);
+        }
+        rs[v + 1] = (int)cols.size();
+    }
+    HaloTiles h;
+    try {
+        h = build_halo_tiles(nv_, rs, cols, [&](int v) { return vertex_key_[v]; },
+                             [&](int k) { return vidx[k]; }, kTpb, kHaloStride);
+    } catch (const std::invalid_argument&) {
+        return;
+    }
+    for (int64_t s : h.src)
+        if (s > INT32_MAX || s < INT32_MIN) return;
+    const int ng = (int)h.g_row0.size();
+    std::vector<int> grp((size_t)ng * kGrpInts, 0);
+    for (int g = 0; g < ng; ++g) {
+        int* d = &grp[(size_t)g * kGrpInts];
+        d[0] = h.g_row0[g]; d[1] = h.g_rows[g]; d[2] = h.g_halo_n[g]; d[3] = h.g_width[g]; d[4] = h.g_eptr[g];
+    }
+    std::vector<int> src(h.src.begin(), h.src.end());
+    h_grp_.alloc(grp.size());
+    h_halo_.alloc(h.halo.size());
+    h_halo_v_.alloc(h.halo_vertex.size());
+    h_perm_.alloc(nv_);
+    h_col_.alloc(std::max<size_t>(1, h.col.size()));
+    h_src_.alloc(std::max<size_t>(1, src.size()));
+    h_val_.alloc(std::max<size_t>(1, src.size()));
+    cg_x_.alloc(3 * (size_t)nv_);
+    SBS_HIP(hipMemcpy(h_grp_.get(), grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(h_halo_.get(), h.halo.data(), h.halo.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(h_halo_v_.get(), h.halo_vertex.data(), h.halo_vertex.size() * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(h_perm_.get(), h.row_perm.data(), nv_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!src.empty()) {
+        SBS_HIP(hipMemcpy(h_col_.get(), h.col.data(), h.col.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        SBS_HIP(hipMemcpy(h_src_.get(), src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    halo_entries_ = src.size();
+    halo_groups_ = ng;
 }
 
 template <class T> void PdEngine<T>::assemble(const PdParams& params) {
@@ -1075,6 +1480,12 @@ template <class T> void PdEngine<T>::assemble(const PdParams& params) {
     hipLaunchKernelGGL(mass_diag_kernel<T>, dim3(ceil_div(nv_, kTpb)), dim3(kTpb), 0, stream_, nv_, data_.mass,
                        data_.DBC, (T)1e6, dt2, diagL_.get(), c_.get(), diagA_.get(), dinv_.get());
     SBS_HIP(hipGetLastError());
+    if (halo_groups_ > 0) {
+        hipLaunchKernelGGL(halo_values_kernel<T>, dim3(ceil_div(halo_entries_, kTpb)), dim3(kTpb), 0, stream_,
+                           halo_entries_, (const int*)h_src_.get(), (const T*)eval_.get(), (const T*)diagA_.get(),
+                           h_val_.get());
+        SBS_HIP(hipGetLastError());
+    }
 }
 
 template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& params) {
@@ -1206,27 +1617,48 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
     a.tol2 = params.pcg_tol * params.pcg_tol;
     a.floor2 = params.pcg_floor * params.pcg_floor;
     a.skip = skip;
-    const dim3 grid(ceil_div(nv_, kTpb)), block(kTpb);
+    const bool halo = params.pcg_halo && halo_groups_ > 0;
+    if (halo) {
+        a.h.groups = halo_groups_;
+        a.h.grp = h_grp_.get();
+        a.h.halo = h_halo_.get();
+        a.h.halo_v = h_halo_v_.get();
+        a.h.col = h_col_.get();
+        a.h.val = h_val_.get();
+        a.h.perm = h_perm_.get();
+        a.h.x = cg_x_.get();
+    }
+    const dim3 grid(ceil_div(nv_, kTpb)), block(kTpb), hgrid(halo ? halo_groups_ : 1);
     const int max_iter = params.pcg_max_iter;
     tick(kKPcg, true);
-    hipLaunchKernelGGL(pcg_init1_kernel<T>, grid, block, 0, stream_, a);
-    hipLaunchKernelGGL(pcg_init2_kernel<T>, grid, block, 0, stream_, a);
-    auto iteration = [&](int i, int probe) {
+    if (halo) {
+        hipLaunchKernelGGL(pcg_init1_halo_kernel<T>, hgrid, block, 0, stream_, a);
+        hipLaunchKernelGGL(pcg_init2_halo_kernel<T>, hgrid, block, 0, stream_, a);
+    } else {
+        hipLaunchKernelGGL(pcg_init1_kernel<T>, grid, block, 0, stream_, a);
+        hipLaunchKernelGGL(pcg_init2_kernel<T>, grid, block, 0, stream_, a);
+    }
+    auto iteration = [&](int i) {
         tick(kKPcgIter, true);
-        if (i == 0) hipLaunchKernelGGL((pcg_iter_kernel<T, true>), grid, block, 0, stream_, a, i, probe);
-        else hipLaunchKernelGGL((pcg_iter_kernel<T, false>), grid, block, 0, stream_, a, i, probe);
+        if (halo) {
+            if (i == 0) hipLaunchKernelGGL((pcg_iter_halo_kernel<T, true>), hgrid, block, 0, stream_, a, i);
+            else hipLaunchKernelGGL((pcg_iter_halo_kernel<T, false>), hgrid, block, 0, stream_, a, i);
+        } else {
+            if (i == 0) hipLaunchKernelGGL((pcg_iter_kernel<T, true>), grid, block, 0, stream_, a, i);
+            else hipLaunchKernelGGL((pcg_iter_kernel<T, false>), grid, block, 0, stream_, a, i);
+        }
         tick(kKPcgIter, false);
     };
     const bool learned = !exact && solve_index >= 0 && solve_index < (int)cap_.size();
     int L = 0;
     if (learned && cap_[solve_index] > 0) {
         L = std::min(cap_[solve_index], max_iter);
-        for (int i = 0; i < L; ++i) iteration(i, 0);
+        for (int i = 0; i < L; ++i) iteration(i);
     } else {
         const CtlBlock* hc = reinterpret_cast<const CtlBlock*>(host_ctl_);
         while (L < max_iter) {
             const int n = std::min(kCheckChunk, max_iter - L);
-            for (int j = 0; j < n; ++j) iteration(L++, 0);
+            for (int j = 0; j < n; ++j) iteration(L++);
             SBS_HIP(hipMemcpyAsync(host_ctl_, a.ctl, sizeof(CtlBlock), hipMemcpyDeviceToHost, stream_));
             SBS_HIP(hipStreamSynchronize(stream_));
             // st[(L-1)&1] is the state launch L-1 saw: done there means no later iteration ran
@@ -1234,8 +1666,8 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
         }
         if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L));
     }
-    iteration(L, 1);
-    hipLaunchKernelGGL(pcg_finish_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
+    if (halo) hipLaunchKernelGGL(pcg_finish_halo_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
+    else hipLaunchKernelGGL(pcg_finish_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
     tick(kKPcg, false);
     SBS_HIP(hipGetLastError());
     return L;

@@ -56,6 +56,7 @@ struct PdParams {
     int dense_precision = 0;    // DenseCholesky: 0 FP32 MFMA, 1 BF16 MFMA inverse fragments
     double dense_tol = 1e-10;   // DenseCholesky: max|D^-1 r| <= tol max|D^-1 b| (FP64 refinement)
     int dense_max_iter = 8;     // DenseCholesky: refinement corrections per solve
+    int pcg_halo = 1;           // 1: halo-staged PCG kernels (when the tiles could be built); 0: row gathers
 };
 
 // Caller-owned device arrays: the reference's SolverData<Scalar> (src/def.h:22-56).
@@ -148,6 +149,7 @@ public:
     long long pcg_cap_hits() const { return unconverged_total_; }
     long long pcg_iterations_total() const { return pcg_iters_total_; }
     long long pcg_solves_total() const { return pcg_solves_total_; }
+    int halo_groups() const { return halo_groups_; }  // workgroups of the halo tiles (0: not built)
     // Waits for queued work and folds the last step's counters in.
     void sync_counters();
     void set_kernel_timing(unsigned mask);
@@ -159,6 +161,8 @@ public:
 private:
     void build_chunks(const std::vector<double>& X0h);
     void build_structure();
+    void build_halo(const std::vector<int>& rowstart, const std::vector<uint64_t>& keys,
+                    const std::vector<int>& sptr);
     void assemble(const PdParams& params);
     void launch_local(const T* q, bool residual_form, const int* skip);
     size_t local_lds_bytes() const;
@@ -209,7 +213,15 @@ private:
     DevBuf<uint32_t> ecol_;
     DevBuf<T> eval_;
     // PCG workspace and control
-    DevBuf<T> cg_rec_[2], cg_p_;        // per-vertex PCG records (r, w, s, D^-1), double-buffered
+    DevBuf<T> cg_rec_[2], cg_p_;        // per-row PCG records (r, D^-1, w, s), double-buffered
+    // halo tiles of the halo-staged PCG kernels (rows in Morton order)
+    std::vector<uint32_t> vertex_key_;  // Morton code of each vertex's rest position
+    int halo_groups_ = 0;
+    size_t halo_entries_ = 0;
+    DevBuf<int> h_grp_, h_src_;
+    DevBuf<uint32_t> h_halo_, h_halo_v_, h_perm_;
+    DevBuf<uint16_t> h_col_;
+    DevBuf<T> h_val_, cg_x_;
     DevBuf<double> slots_;              // address-striped reduction slots
     DevBuf<unsigned char> ctl_;         // device control block
     void* host_ctl_ = nullptr;          // pinned copy of the control block (host checks)

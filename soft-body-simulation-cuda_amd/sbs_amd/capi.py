@@ -49,13 +49,14 @@ class SbsSolverParams(ctypes.Structure):
                 ("tol", ctypes.c_double), ("rho", ctypes.c_double), ("handle_collision", ctypes.c_int),
                 ("pcg_tol", ctypes.c_double), ("pcg_floor", ctypes.c_double), ("pcg_max_iter", ctypes.c_int),
                 ("pcg_chebyshev", ctypes.c_int), ("pcg_adaptive", ctypes.c_int), ("dense_precision", ctypes.c_int),
-                ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int)]
+                ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int), ("pcg_halo", ctypes.c_int)]
 
 
 class SbsPdStats(ctypes.Structure):
     _fields_ = [("pcg_iterations_last_step", ctypes.c_int), ("pcg_cap", ctypes.c_int),
                 ("pcg_cap_hits", ctypes.c_longlong), ("nnz_offdiag", ctypes.c_longlong),
-                ("pcg_iterations_total", ctypes.c_longlong), ("pcg_solves_total", ctypes.c_longlong)]
+                ("pcg_iterations_total", ctypes.c_longlong), ("pcg_solves_total", ctypes.c_longlong),
+                ("halo_groups", ctypes.c_int)]
 
 
 # Every function the header declares, with its ctypes signature.

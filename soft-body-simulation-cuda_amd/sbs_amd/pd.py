@@ -44,13 +44,14 @@ class SolverParams:
     dense_precision: int = 0      # 0: FP32 MFMA inverse, 1: BF16 MFMA inverse
     dense_tol: float = 1e-10
     dense_max_iter: int = 8
+    pcg_halo: bool = True         # halo-staged PCG kernels (LDS) where the mesh can be tiled
 
     def to_c(self) -> capi.SbsSolverParams:
         return capi.SbsSolverParams(int(self.numIterations), float(self.dt), float(self.gravity), float(self.tol),
                                     float(self.rho), int(self.handleCollision), float(self.pcg_tol),
                                     float(self.pcg_floor), int(self.pcg_max_iter), int(self.pcg_chebyshev),
                                     int(self.pcg_adaptive), int(self.dense_precision), float(self.dense_tol),
-                                    int(self.dense_max_iter))
+                                    int(self.dense_max_iter), int(self.pcg_halo))
 
 
 class SolverData:

@@ -23,7 +23,7 @@ def params_for(solver, n_iter=10, **kw):
     p = SolverParams(numIterations=n_iter, dt=1.0 / 60.0, gravity=9.8, tol=kw.pop("tol", 1e-6), rho=0.9992,
                      pcg_tol=kw.pop("pcg_tol", 1e-6), pcg_floor=kw.pop("pcg_floor", 1e-8),
                      pcg_max_iter=kw.pop("pcg_max_iter", 200), pcg_chebyshev=kw.pop("pcg_chebyshev", False),
-                     pcg_adaptive=kw.pop("pcg_adaptive", False))
+                     pcg_adaptive=kw.pop("pcg_adaptive", False), pcg_halo=kw.pop("pcg_halo", True))
     assert not kw
     return p
 
@@ -107,16 +107,17 @@ def test_local_step_matches_oracle(gpu, precision, residual_form):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-@pytest.mark.parametrize("solver", [SolverType.Jacobi, SolverType.CuSolverCholesky, SolverType.PCG])
-def test_plain_pd_cube_100_steps(gpu, precision, solver):
+@pytest.mark.parametrize("solver,halo", [(SolverType.Jacobi, True), (SolverType.CuSolverCholesky, True),
+                                         (SolverType.PCG, True), (SolverType.PCG, False)])
+def test_plain_pd_cube_100_steps(gpu, precision, solver, halo):
     """The 8^3-hex cube (729 v / 3072 t), 10 iterations per step, 100 steps."""
     s = mesh.cube_scene(8)
     if precision == 32:
-        p = params_for(solver, n_iter=10)
+        p = params_for(solver, n_iter=10, pcg_halo=halo)
     else:
         # FP64 bar is 1e-10: no threshold (early exit, PCG stop) may flip between
         # the two implementations, so solve to machine precision without early exit
-        p = params_for(solver, n_iter=10, tol=0.0, pcg_tol=1e-13, pcg_floor=0.0, pcg_max_iter=400)
+        p = params_for(solver, n_iter=10, tol=0.0, pcg_tol=1e-13, pcg_floor=0.0, pcg_max_iter=400, pcg_halo=halo)
     g, st = run_gpu(s, solver, p, 100, precision)
     r = run_oracle(s, solver, p, 100, precision)
     for k in ("X", "XTilde"):
@@ -139,16 +140,48 @@ def test_eigen_cholesky_mode_matches_cusolver_mode(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-def test_pcg_chebyshev_midsize(gpu, precision):
-    """The config-1 global step (PCG + Chebyshev, 20 iterations) on a 16^3 cube."""
+@pytest.mark.parametrize("halo", [True, False])
+def test_pcg_chebyshev_midsize(gpu, precision, halo):
+    """The 64^3 workload's global step (PCG + Chebyshev, 20 iterations) on a 16^3 cube."""
     s = mesh.cube_scene(16)
-    p = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True,
+    p = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_halo=halo,
                    pcg_floor=1e-8 if precision == 32 else 0.0, pcg_tol=1e-6 if precision == 32 else 1e-12)
     g, st = run_gpu(s, SolverType.PCG, p, 5, precision)
+    assert st.halo_groups >= -(-s.nv // 256)   # workgroups of up to 256 rows
     r = run_oracle(s, SolverType.PCG, p, 5, precision)
     assert rel_err(g["X"], r["X"]) < TOL[precision]
     assert max_rel(g["X"], r["X"]) < TOL[precision]
     assert st.pcg_cap_hits == 0
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_pcg_multi_body_halo_matches_row_gather_and_oracle(gpu, precision):
+    """Bodies of different sizes and stiffness: workgroups span body boundaries."""
+    s = mesh.make_scene([dict(nx=9, ny=5, nz=4, seed=1, pin_top=True, E=1e5),
+                         dict(nx=3, ny=7, nz=3, origin=(1.2, 0.0, 0.0), seed=2, pin_top=True, E=4e5),
+                         dict(nx=6, ny=6, nz=6, origin=(0.0, 0.0, 0.8), seed=3, pin_top=True, E=6e4)])
+    kw = dict(n_iter=20, pcg_chebyshev=True)
+    if precision == 64:
+        kw.update(pcg_tol=1e-13, pcg_floor=0.0, pcg_max_iter=400)
+    h, sh = run_gpu(s, SolverType.PCG, params_for(SolverType.PCG, pcg_halo=True, **kw), 10, precision)
+    g, sg = run_gpu(s, SolverType.PCG, params_for(SolverType.PCG, pcg_halo=False, **kw), 10, precision)
+    r = run_oracle(s, SolverType.PCG, params_for(SolverType.PCG, **kw), 10, precision)
+    assert sh.halo_groups >= 3 and sh.pcg_cap_hits == 0 and sg.pcg_cap_hits == 0
+    for out in (h, g):
+        assert rel_err(out["X"], r["X"]) < TOL[precision]
+        if precision == 64:
+            assert max_rel(out["X"], r["X"]) < TOL[precision]
+    assert rel_err(h["X"], g["X"]) < TOL[precision]
+    if precision == 32:
+        # the stiff body amplifies FP32 rounding: the FP32 oracle itself sits
+        # ~1.8e-5 (max-norm) from the exact solution of this scene, so the
+        # max-norm bar is "as close to the exact solution as the FP32 oracle"
+        px = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_tol=1e-13, pcg_floor=0.0,
+                        pcg_max_iter=400)
+        exact = run_oracle(s, SolverType.PCG, px, 10, 64)["X"]
+        own = max_rel(r["X"], exact)
+        for out in (h, g):
+            assert max_rel(out["X"], exact) < 2.0 * own + 1e-6, (max_rel(out["X"], exact), own)
 
 
 def test_pcg_adaptive_launch_cap_matches_exact(gpu):
@@ -165,7 +198,8 @@ def test_pcg_adaptive_launch_cap_matches_exact(gpu):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-def test_linear_solve_matches_dense_solve(gpu, precision):
+@pytest.mark.parametrize("halo", [True, False])
+def test_linear_solve_matches_dense_solve(gpu, precision, halo):
     from sbs_amd import capi
     s = mesh.cube_scene(5)
     rng = np.random.default_rng(3)
@@ -174,7 +208,8 @@ def test_linear_solve_matches_dense_solve(gpu, precision):
         data = SolverData(s, precision)
         pd = PdSolver(128, data)
         pd.SetGlobalSolver(solver)
-        p = params_for(solver, pcg_tol=1e-7 if precision == 32 else 1e-14, pcg_floor=0.0, pcg_max_iter=500)
+        p = params_for(solver, pcg_tol=1e-7 if precision == 32 else 1e-14, pcg_floor=0.0, pcg_max_iter=500,
+                       pcg_halo=halo)
         pd.Update(data, p)
         ex = pd.export_system()
         A = np.zeros((s.nv, s.nv))
