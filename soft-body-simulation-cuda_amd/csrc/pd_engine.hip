@@ -1042,8 +1042,8 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
 // with the reduction stripes of the previous launch; once the scalars are
 // formed the halo records, the row's own record, p and x are requested
 // together, so the launch costs about two dependent memory round trips.  The
-// iteration that finds the solve converged still runs the LDS stage (its
-// result is discarded) so that every thread reaches the barrier.
+// iteration that finds the solve converged returns after the scalars, before
+// any of those loads (the decision is the same in every wavefront).
 template <class T, bool kFirst>
 __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
 pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
@@ -1065,7 +1065,7 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) hidx[k] = hh[t + k * kTpb];  // in bounds: kHaloStride per workgroup
     const PcgState st = pcg_state(a, i);
-    const bool live = !st.done;
+    if (st.done) return;
     Rec<T> hr[kHaloPer];
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) {
@@ -1097,14 +1097,13 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
             lu[2][e] = dq * fma(-al2, sq2, q.v[kRecR + 2]);
         }
     }
-    if (live && blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
+    if (blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
         double* z = iter_slot(a.slots, i + 1);
         for (int k = threadIdx.x; k < kSlotSize; k += kTpb) z[k] = 0.0;
     }
     __syncthreads();
     T y0 = 0, y1 = 0, y2 = 0;  // w_{i+1} = A u_{i+1}
     halo_row(a.h, G.first, G.width, t, b0, lu, y0, y1, y2);
-    if (!live) return;  // uniform over the grid
     double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (row) {
         const T r0 = own.v[kRecR], r1 = own.v[kRecR + 1], r2 = own.v[kRecR + 2], dv = own.v[kRecD];
@@ -1135,8 +1134,9 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     stripe_add<9>(vals, iter_slot(a.slots, i));
 }
 
-// After K(L-1): the stop test of iteration L and bookkeeping (block 0), the iterate back to vertex order, and
-// with `cheb` the Chebyshev recurrence (pdSolver.cu:196-205) into sn/prev.
+// After K(L-1): the stop test of iteration L and bookkeeping (block 0); the
+// iterate goes back to vertex order, into x without `cheb`, and with `cheb`
+// into the Chebyshev recurrence (pdSolver.cu:196-205) of sn/prev only.
 template <class T>
 __global__ void __launch_bounds__(kTpb) pcg_finish_halo_kernel(PcgArgs<T> a, int L, int cheb) {
     if (a.skip && *a.skip) return;
@@ -1146,8 +1146,10 @@ __global__ void __launch_bounds__(kTpb) pcg_finish_halo_kernel(PcgArgs<T> a, int
     const uint32_t v = a.h.perm[ir];
     T xv[3];
     ld3(a.h.x, ir, xv[0], xv[1], xv[2]);
-    st3(a.x, v, xv[0], xv[1], xv[2]);
-    if (!cheb) return;
+    if (!cheb) {
+        st3(a.x, v, xv[0], xv[1], xv[2]);
+        return;
+    }
     for (int k = 0; k < 3; ++k) {
         const size_t j = 3 * (size_t)v + k;
         const T s = a.sn[j];
