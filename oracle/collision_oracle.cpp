@@ -20,6 +20,9 @@
 //   * hit resolution (min toi per vertex / edge) src/collision/narrowphase.cu:9-57,129-131
 //   * storeTi                                   src/collision/narrowphase.cu:72-112
 //   * CCDKernel response                        src/simulation/collisionUtil.cu:50-70
+//   * ComputeMinStepSize (min toi of the CCD candidates) src/collision/bvh.cu:183-190
+//   * UpdateQueries' candidates: boxes of X grown by 2 dhat src/collision/broadphase.cu:557-577,
+//       computeTriTrajBBox src/collision/ccd.cu:74-85
 //
 // The broad phase here enumerates all ordered triangle pairs (O(n^2)); the
 // reference finds the same pairs through its LBVH, whose internal boxes are
@@ -37,6 +40,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -193,6 +197,23 @@ template <class T> Box<T> leaf_box(const uint32_t* tri, const T* X, const T* XT)
     return b;
 }
 
+// Leaf box of a triangle at rest, grown by `bound` (computeTriTrajBBox,
+// ccd.cu:74-85; the coordinates pass through float there as well).
+template <class T> Box<T> leaf_box_bound(const uint32_t* tri, const T* X, T bound) {
+    Box<T> b;
+    for (int k = 0; k < 3; ++k) {
+        float mn = (float)X[3 * (size_t)tri[0] + k], mx = mn;
+        for (int j = 1; j < 3; ++j) {
+            const float a = (float)X[3 * (size_t)tri[j] + k];
+            mn = std::fmin(mn, a);
+            mx = std::fmax(mx, a);
+        }
+        b.lo[k] = (T)mn - bound;
+        b.hi[k] = (T)mx + bound;
+    }
+    return b;
+}
+
 template <class T> bool overlap(const Box<T>& a, const Box<T>& b) {
     for (int k = 0; k < 3; ++k)
         if (a.hi[k] < b.lo[k] || a.lo[k] > b.hi[k]) return false;
@@ -238,10 +259,8 @@ void pair_queries(const uint32_t* a, const uint32_t* b, Q* q) {
 }
 
 template <class T>
-std::vector<Q> broad_phase(int ntri, const uint32_t* tri, const uint32_t* fathers, const T* X, const T* XT,
-                           int ignore_self) {
-    std::vector<Box<T>> box(ntri);
-    for (int i = 0; i < ntri; ++i) box[i] = leaf_box(tri + 3 * (size_t)i, X, XT);
+std::vector<Q> broad_phase_boxes(int ntri, const uint32_t* tri, const uint32_t* fathers, const std::vector<Box<T>>& box,
+                                 int ignore_self) {
     std::vector<Q> out;
     Q q[12];
     for (int i = 0; i < ntri; ++i) {
@@ -262,6 +281,24 @@ std::vector<Q> broad_phase(int ntri, const uint32_t* tri, const uint32_t* father
     std::sort(out.begin(), out.end(), q_less);
     out.erase(std::unique(out.begin(), out.end(), q_eq), out.end());
     return out;
+}
+
+// Candidates of DetectCollision / ComputeMinStepSize: trajectory boxes X -> XT.
+template <class T>
+std::vector<Q> broad_phase(int ntri, const uint32_t* tri, const uint32_t* fathers, const T* X, const T* XT,
+                           int ignore_self) {
+    std::vector<Box<T>> box(ntri);
+    for (int i = 0; i < ntri; ++i) box[i] = leaf_box(tri + 3 * (size_t)i, X, XT);
+    return broad_phase_boxes(ntri, tri, fathers, box, ignore_self);
+}
+
+// Candidates of UpdateQueries (broadphase.cu:574-577): boxes of X grown by `bound`.
+template <class T>
+std::vector<Q> broad_phase_rest(int ntri, const uint32_t* tri, const uint32_t* fathers, const T* X, T bound,
+                                int ignore_self) {
+    std::vector<Box<T>> box(ntri);
+    for (int i = 0; i < ntri; ++i) box[i] = leaf_box_bound(tri + 3 * (size_t)i, X, bound);
+    return broad_phase_boxes(ntri, tri, fathers, box, ignore_self);
 }
 
 // Hit resolution + storeTi on the sorted query list; returns the number of
@@ -359,6 +396,38 @@ void* orc_col_detect(int precision, int nv, int ntri, const uint32_t* tri, const
                               (double*)tI, normals);
     return detect<float>(nv, ntri, tri, fathers, (const float*)X, (const float*)XT, ignore_self, (float*)tI,
                          normals);
+}
+
+// ComputeMinStepSize (bvh.cu:183-190): the smallest toi over the CCD
+// candidates of X -> XT (1 when there are none).
+double orc_col_min_step(int precision, int nv, int ntri, const uint32_t* tri, const uint32_t* fathers, const void* X,
+                        const void* XT, int ignore_self) {
+    auto run = [&](auto* x, auto* xt) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(x)>>;
+        const std::vector<Q> qs = broad_phase<T>(ntri, tri, fathers, x, xt, ignore_self);
+        T best = T(1);
+        for (const Q& q : qs) {
+            V3<T> n;
+            best = std::min(best, ccd_test<T>((int)q.type, q.v, x, xt, n));
+        }
+        return (double)best;
+    };
+    (void)nv;
+    if (precision == 64) return run((const double*)X, (const double*)XT);
+    return run((const float*)X, (const float*)XT);
+}
+
+// Candidate queries of UpdateQueries: boxes of X grown by `bound`; returns
+// the count and, when `q` is not null, 5 u32 per query (type, v0..v3).
+long long orc_col_proximity(int ntri, const uint32_t* tri, const uint32_t* fathers, const double* X, double bound,
+                            int ignore_self, uint32_t* q) {
+    const std::vector<Q> qs = broad_phase_rest<double>(ntri, tri, fathers, X, bound, ignore_self);
+    if (q)
+        for (size_t k = 0; k < qs.size(); ++k) {
+            q[5 * k] = qs[k].type;
+            for (int j = 0; j < 4; ++j) q[5 * k + 1 + j] = qs[k].v[j];
+        }
+    return (long long)qs.size();
 }
 
 void orc_col_destroy(int precision, void* h) {
