@@ -86,6 +86,33 @@ def lib(bench: bool = False) -> ctypes.CDLL:
     L.orc_solve_cubic.restype = ctypes.c_int
     L.orc_solve_cubic.argtypes = [ctypes.c_double] * 4 + [dp]
     L.orc_ccd_response.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, fp]
+    L.orc_col_min_step.restype = ctypes.c_double
+    L.orc_col_min_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, up, up, vp, vp, ctypes.c_int]
+    L.orc_col_proximity.restype = ctypes.c_longlong
+    L.orc_col_proximity.argtypes = [ctypes.c_int, up, up, dp, ctypes.c_double, ctypes.c_int, up]
+    if hasattr(L, "orc_ipc_create"):
+        ucp = ctypes.POINTER(ctypes.c_ubyte)
+        L.orc_ipc_distance_type.restype = ctypes.c_int
+        L.orc_ipc_distance_type.argtypes = [ctypes.c_int, dp]
+        L.orc_ipc_distance.restype = ctypes.c_double
+        L.orc_ipc_distance.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        L.orc_ipc_barrier.restype = ctypes.c_double
+        L.orc_ipc_barrier.argtypes = [ctypes.c_double] * 3 + [dp, dp]
+        L.orc_ipc_make_pd.argtypes = [ctypes.c_int, dp]
+        L.orc_ipc_create.restype = vp
+        L.orc_ipc_create.argtypes = [ctypes.c_int] * 3 + [dp, dp, up, up, up, dp, dp, dp, ucp] + \
+            [ctypes.c_double] * 5 + [ctypes.c_int]
+        L.orc_ipc_destroy.argtypes = [vp]
+        L.orc_ipc_step.restype = ctypes.c_int
+        L.orc_ipc_step.argtypes = [vp]
+        L.orc_ipc_get.argtypes = [vp, dp, dp]
+        L.orc_ipc_set_x.argtypes = [vp, dp]
+        L.orc_ipc_stats.argtypes = [vp, ip, dp]
+        L.orc_ipc_energy.restype = ctypes.c_double
+        L.orc_ipc_energy.argtypes = [vp, dp, dp]
+        L.orc_ipc_set_state.argtypes = [vp, dp, ctypes.c_double]
+        L.orc_ipc_queries.restype = ctypes.c_longlong
+        L.orc_ipc_queries.argtypes = [vp, up, ip, dp]
     _LIBS[name] = L
     return L
 
@@ -249,3 +276,107 @@ def ccd_response(X, XT, V, tI, vertex_normals, fp64=False):
     lib().orc_ccd_response(64 if fp64 else 32, X.size // 3, X.ctypes.data, XT.ctypes.data, V.ctypes.data,
                            tI.ctypes.data, _p(nrm, ctypes.c_float))
     return X.reshape(-1, 3), V.reshape(-1, 3)
+
+
+# ----------------------------------------------------------------- IPC --
+QUERY_TYPES = {"VF": 1, "EE": 2}
+DISTANCE_TYPES = ["P_T0", "P_T1", "P_T2", "P_E0", "P_E1", "P_E2", "P_T", "EA0_EB0", "EA0_EB1", "EA1_EB0",
+                  "EA1_EB1", "EA_EB0", "EA_EB1", "EA0_EB", "EA1_EB", "EA_EB"]
+
+
+def ipc_distance_type(qtype, x12):
+    x = np.ascontiguousarray(x12, dtype=np.float64).reshape(12)
+    return lib().orc_ipc_distance_type(int(qtype), _p(x, ctypes.c_double))
+
+
+def ipc_distance(qtype, dtype, x12):
+    """Squared distance of the given type, its gradient (12) and Hessian (12x12)."""
+    x = np.ascontiguousarray(x12, dtype=np.float64).reshape(12)
+    g = np.zeros(12)
+    H = np.zeros((12, 12))
+    d = lib().orc_ipc_distance(int(qtype), int(dtype), _p(x, ctypes.c_double), _p(g, ctypes.c_double),
+                               _p(H, ctypes.c_double))
+    return d, g, H
+
+
+def ipc_barrier(d, dhat, kappa):
+    d1, d2 = ctypes.c_double(), ctypes.c_double()
+    b = lib().orc_ipc_barrier(d, dhat, kappa, ctypes.byref(d1), ctypes.byref(d2))
+    return b, d1.value, d2.value
+
+
+def ipc_make_pd(M):
+    A = np.ascontiguousarray(M, dtype=np.float64).copy()
+    lib().orc_ipc_make_pd(A.shape[0], _p(A, ctypes.c_double))
+    return A
+
+
+class IpcOracle:
+    """The reference's IPCSolver step (ipc.cu:142-258) on the CPU, FP64."""
+
+    def __init__(self, X, V, tet, tri, fathers, mass, mu, lam, fixed, dt=0.01, damp=0.999, dhat=1e-3, tol=5e-2,
+                 lin_tol=1e-13, max_iter=100):
+        self.L = lib()
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
+        self.nv = X.size // 3
+        tet = np.ascontiguousarray(tet, dtype=np.uint32).reshape(-1)
+        tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1)
+        fathers = np.ascontiguousarray(fathers, dtype=np.uint32).reshape(-1)
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (mass, mu, lam)]
+        fx = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        self._keep += [X, V, tet, tri, fathers, fx]
+        cd, cu = ctypes.c_double, ctypes.c_uint32
+        self.h = self.L.orc_ipc_create(self.nv, tet.size // 4, tri.size // 3, _p(X, cd), _p(V, cd), _p(tet, cu),
+                                       _p(tri, cu), _p(fathers, cu), _p(self._keep[0], cd), _p(self._keep[1], cd),
+                                       _p(self._keep[2], cd), _p(fx, ctypes.c_ubyte), dt, damp, dhat, tol, lin_tol,
+                                       int(max_iter))
+
+    def step(self, n=1):
+        ok = True
+        for _ in range(n):
+            ok = bool(self.L.orc_ipc_step(self.h)) and ok
+        return ok
+
+    def get(self):
+        X = np.zeros(3 * self.nv)
+        V = np.zeros(3 * self.nv)
+        self.L.orc_ipc_get(self.h, _p(X, ctypes.c_double), _p(V, ctypes.c_double))
+        return X.reshape(-1, 3), V.reshape(-1, 3)
+
+    def stats(self):
+        ints = np.zeros(4, np.int32)
+        reals = np.zeros(2)
+        self.L.orc_ipc_stats(self.h, _p(ints, ctypes.c_int), _p(reals, ctypes.c_double))
+        return dict(newton_iterations=int(ints[0]), line_search_halvings=int(ints[1]), linear_iterations=int(ints[2]),
+                    queries=int(ints[3]), kappa=float(reals[0]), alpha0=float(reals[1]))
+
+    def set_state(self, XT, kappa):
+        XT = np.ascontiguousarray(XT, dtype=np.float64).reshape(-1)
+        self.L.orc_ipc_set_state(self.h, _p(XT, ctypes.c_double), float(kappa))
+
+    def energy(self, X, grad=False):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        g = np.zeros(X.size) if grad else None
+        e = self.L.orc_ipc_energy(self.h, _p(X, ctypes.c_double), None if g is None else _p(g, ctypes.c_double))
+        return (e, g.reshape(-1, 3)) if grad else e
+
+    def queries(self):
+        n = self.L.orc_ipc_queries(self.h, None, None, None)
+        q = np.zeros((n, 5), np.uint32)
+        dt = np.zeros(n, np.int32)
+        d = np.zeros(n)
+        if n:
+            self.L.orc_ipc_queries(self.h, _p(q, ctypes.c_uint32), _p(dt, ctypes.c_int), _p(d, ctypes.c_double))
+        return q, dt, d
+
+    def close(self):
+        if self.h:
+            self.L.orc_ipc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

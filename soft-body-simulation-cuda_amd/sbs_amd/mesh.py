@@ -154,7 +154,8 @@ def make_scene(bodies: list[dict]) -> Scene:
     """Concatenate bodies like DataLoader::AllocData (dataLoader.cu:292-378).
 
     Each body dict: nx, ny, nz, spacing, origin, jitter, seed, density, E, nu,
-    pin_top (bool: pin vertices with the maximum grid y index).
+    pin_top (bool: pin vertices with the maximum grid y index), pin_bottom
+    (bool: pin vertices with grid y index 0).
     """
     Xs, tets, tris, fathers, masses, mus, lams, dbcs = [], [], [], [], [], [], [], []
     vstart, tstart = [0], [0]
@@ -168,9 +169,11 @@ def make_scene(bodies: list[dict]) -> Scene:
         vol = (nx * sp) * (ny * sp) * (nz * sp)
         mu, lam = lame(b.get("E", 1e5), b.get("nu", 0.3))
         dbc = np.zeros(nvb)
+        jj = (np.arange(nvb) // (nx + 1)) % (ny + 1)
         if b.get("pin_top", False):
-            jj = (np.arange(nvb) // (nx + 1)) % (ny + 1)
             dbc[jj == ny] = 1.0
+        if b.get("pin_bottom", False):
+            dbc[jj == 0] = 1.0
         Xs.append(P)
         tets.append(tet.astype(np.int64) + off)
         tris.append(tri.astype(np.int64) + off)
@@ -202,3 +205,14 @@ def scalar_laplacian_nnz(tet: np.ndarray, nv: int) -> int:
     pairs.sort(axis=1)
     e = np.unique(pairs[:, 0] * nv + pairs[:, 1])
     return nv + 2 * int(e.size)
+
+
+def stack_scene(n: int, spacing: float = 0.1, gap: float = 2e-3, jitter: float = 5e-4, seed: int = 0,
+                E: float = 1e5, nu: float = 0.3, density: float = 1000.0) -> Scene:
+    """Two n^3-hex cubes, the lower one pinned on its bottom face, the upper one
+    `gap` above it: the contact workload of the IPC step."""
+    top = n * spacing + gap + 2 * jitter
+    return make_scene([dict(nx=n, ny=n, nz=n, spacing=spacing, jitter=jitter, seed=seed, E=E, nu=nu,
+                            density=density, pin_bottom=True),
+                       dict(nx=n, ny=n, nz=n, spacing=spacing, jitter=jitter, seed=seed + 1, E=E, nu=nu,
+                            density=density, origin=(0.0, top, 0.0))])
