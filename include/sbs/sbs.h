@@ -203,6 +203,81 @@ int sbs_collision_stats(SbsCollision* c, long long* candidate_pairs, int* hits);
 int sbs_ccd_response(int precision, int num_verts, void* d_X, const void* d_XTilde, void* d_V, const void* d_tI,
                      const float* d_normals);
 
+/* ---------------------------------------------------------------- IPC --
+ * The FP64 incremental-potential-contact solver of the reference:
+ *   IPCSolver(int threadsPerBlock, const SolverData<double>&)
+ *        src/simulation/solver/IPC/ipc.h:19, ipc.cu:101-117 -> sbs_ipc_create
+ *   IPCSolver::Update(SolverData&, const SolverParams&)
+ *        ipc.h:21, ipc.cu:128-135 (SolverStep :142-228)    -> sbs_ipc_update
+ *   IPCSolver::Reset()            ipc.h:22, ipc.cu:137-141 -> sbs_ipc_reset
+ *   IPCSolver::SetLinearSolver(SolverType) ipc.h:23        -> sbs_ipc_set_linear_solver
+ * Newton on 1/2|x - x~|_M^2 + h^2 (gravity + corotated elasticity + barrier on
+ * the squared distances of the contact queries), CCD-limited line search,
+ * stop when |p|_inf / dt < tol.  All arrays are device memory of doubles
+ * (positions and velocities interleaved xyz), indices u32. */
+typedef struct SbsIpcData {
+    int num_verts;
+    int num_tets;
+    int num_tris;
+    double* X;                  /* 3*num_verts: positions, updated by a step     */
+    double* V;                  /* 3*num_verts: velocities, updated by a step    */
+    double* XTilde;             /* 3*num_verts: x~ of the step (written)         */
+    const uint32_t* Tet;        /* 4*num_tets                                    */
+    const uint32_t* Tri;        /* 3*num_tris surface triangles                  */
+    const uint32_t* TriFathers; /* num_tris: body of each triangle               */
+    const double* mass;         /* num_verts                                     */
+    const double* mu;           /* num_tets                                      */
+    const double* lambda;       /* num_tets                                      */
+    const double* DBC;          /* num_verts: nonzero = fixed vertex (DBCIdx)    */
+} SbsIpcData;
+
+/* The IPC fields of SolverParams<double> (src/def.h:83-100) plus the linear solve. */
+typedef struct SbsIpcParams {
+    double dt;
+    double damp;          /* x~ = x + dt v damp                                 */
+    double dhat;          /* barrier distance (queries kept while d^2 <= dhat)  */
+    double tol;           /* Newton stop: |p|_inf / dt < tol                    */
+    int max_iterations;   /* Newton iterations per step (failure beyond)        */
+    double lin_tol;       /* Jacobi-PCG: |r| <= lin_tol |g|                     */
+    int lin_max_iter;
+} SbsIpcParams;
+
+typedef struct SbsIpcStats {
+    int newton_iterations;    /* of the last step                              */
+    int line_search_halvings; /* of the last step                              */
+    int linear_iterations;    /* PCG iterations of the last solve              */
+    int failed;               /* latched: a step failed (reset clears it)      */
+    long long queries;        /* contact queries kept at the last update       */
+    double kappa;             /* barrier stiffness of the last step            */
+    double alpha0;            /* first line-search step of the last step       */
+    double energy;            /* incremental potential at the end of the step  */
+} SbsIpcStats;
+
+/* IPCSolver::SolverType (ipc.h:14-17). */
+#define SBS_IPC_CUSOLVER_CHOLESKY 0
+#define SBS_IPC_ICPCG 1
+#define SBS_IPC_PCG_JACOBI 2
+#define SBS_IPC_JACOBI 3
+
+typedef struct SbsIpc SbsIpc;
+int sbs_ipc_create(const SbsIpcData* data, SbsIpc** out);
+int sbs_ipc_destroy(SbsIpc* h);
+/* One SolverStep; *converged = 0 when the step failed (then, like the
+ * reference's latched `failed`, later updates do nothing until a reset). */
+int sbs_ipc_update(SbsIpc* h, SbsIpcData* data, const SbsIpcParams* params, int* converged);
+int sbs_ipc_reset(SbsIpc* h);
+/* Linear solver of the Newton system.  Every type that converges to the
+ * solution (0, 1, 2) runs Jacobi-PCG to lin_tol; 3 (plain Jacobi iteration)
+ * is rejected. */
+int sbs_ipc_set_linear_solver(SbsIpc* h, int solver_type);
+int sbs_ipc_set_stream(SbsIpc* h, void* hip_stream);
+int sbs_ipc_stats(SbsIpc* h, SbsIpcStats* out);
+/* Energy (and gradient, host buffer of 3*num_verts, may be NULL) at the host
+ * positions X with the queries recomputed there, with the current x~ and kappa
+ * (set by the last step or by sbs_ipc_set_state, host x~). */
+int sbs_ipc_energy(SbsIpc* h, const double* X, double* energy, double* grad);
+int sbs_ipc_set_state(SbsIpc* h, const double* XTilde, double kappa);
+
 /* Device memory and copies for callers without their own allocator. */
 int sbs_malloc(void** ptr, size_t bytes);
 int sbs_free(void* ptr);

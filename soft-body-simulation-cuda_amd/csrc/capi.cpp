@@ -8,6 +8,7 @@
 
 #include "../../include/sbs/sbs.h"
 #include "collision.hpp"
+#include "ipc_engine.hpp"
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
 
@@ -26,6 +27,10 @@ struct SbsPd {
     int precision = 32;
     std::unique_ptr<sbs::PdEngine<float>> f32;
     std::unique_ptr<sbs::PdEngine<double>> f64;
+};
+
+struct SbsIpc {
+    std::unique_ptr<sbs::IpcEngine> e;
 };
 
 struct SbsCollision {
@@ -406,3 +411,105 @@ int sbs_device_count(int* count) {
 }
 
 }  // extern "C"
+
+namespace {
+sbs::IpcData to_ipc(const SbsIpcData* d) {
+    sbs::IpcData o;
+    o.num_verts = d->num_verts;
+    o.num_tets = d->num_tets;
+    o.num_tris = d->num_tris;
+    o.X = d->X;
+    o.V = d->V;
+    o.XTilde = d->XTilde;
+    o.Tet = d->Tet;
+    o.Tri = d->Tri;
+    o.TriFathers = d->TriFathers;
+    o.mass = d->mass;
+    o.mu = d->mu;
+    o.lambda = d->lambda;
+    o.DBC = d->DBC;
+    return o;
+}
+}  // namespace
+
+int sbs_ipc_create(const SbsIpcData* data, SbsIpc** out) {
+    return guarded([&] {
+        SBS_REQUIRE(data != nullptr && out != nullptr, "sbs_ipc_create: null argument");
+        *out = nullptr;
+        auto h = std::make_unique<SbsIpc>();
+        h->e = std::make_unique<sbs::IpcEngine>(to_ipc(data));
+        *out = h.release();
+    });
+}
+
+int sbs_ipc_destroy(SbsIpc* h) {
+    return guarded([&] { delete h; });
+}
+
+int sbs_ipc_update(SbsIpc* h, SbsIpcData* data, const SbsIpcParams* params, int* converged) {
+    return guarded([&] {
+        SBS_REQUIRE(h && data && params, "sbs_ipc_update: null argument");
+        sbs::IpcParams p;
+        p.dt = params->dt;
+        p.damp = params->damp;
+        p.dhat = params->dhat;
+        p.tol = params->tol;
+        p.max_iterations = params->max_iterations;
+        p.lin_tol = params->lin_tol;
+        p.lin_max_iter = params->lin_max_iter;
+        sbs::IpcData d = to_ipc(data);
+        const bool ok = h->e->update(d, p);
+        if (converged) *converged = ok ? 1 : 0;
+    });
+}
+
+int sbs_ipc_reset(SbsIpc* h) {
+    return guarded([&] {
+        SBS_REQUIRE(h, "sbs_ipc_reset: null handle");
+        h->e->reset();
+    });
+}
+
+int sbs_ipc_set_linear_solver(SbsIpc* h, int solver_type) {
+    return guarded([&] {
+        SBS_REQUIRE(h, "sbs_ipc_set_linear_solver: null handle");
+        SBS_REQUIRE(solver_type >= 0 && solver_type <= 2,
+                    "sbs_ipc_set_linear_solver: types 0..2 run Jacobi-PCG; the plain Jacobi iteration is not provided");
+    });
+}
+
+int sbs_ipc_set_stream(SbsIpc* h, void* hip_stream) {
+    return guarded([&] {
+        SBS_REQUIRE(h, "sbs_ipc_set_stream: null handle");
+        h->e->set_stream(static_cast<hipStream_t>(hip_stream));
+    });
+}
+
+int sbs_ipc_stats(SbsIpc* h, SbsIpcStats* out) {
+    return guarded([&] {
+        SBS_REQUIRE(h && out, "sbs_ipc_stats: null argument");
+        const sbs::IpcStats& s = h->e->stats();
+        out->newton_iterations = s.newton_iterations;
+        out->line_search_halvings = s.line_search_halvings;
+        out->linear_iterations = s.linear_iterations;
+        out->failed = s.failed;
+        out->queries = s.queries;
+        out->kappa = s.kappa;
+        out->alpha0 = s.alpha0;
+        out->energy = s.energy;
+    });
+}
+
+int sbs_ipc_energy(SbsIpc* h, const double* X, double* energy, double* grad) {
+    return guarded([&] {
+        SBS_REQUIRE(h && X && energy, "sbs_ipc_energy: null argument");
+        *energy = h->e->energy_at(X, grad);
+    });
+}
+
+int sbs_ipc_set_state(SbsIpc* h, const double* XTilde, double kappa) {
+    return guarded([&] {
+        SBS_REQUIRE(h && XTilde, "sbs_ipc_set_state: null argument");
+        h->e->set_state(XTilde, kappa);
+    });
+}

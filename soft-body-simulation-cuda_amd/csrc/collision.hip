@@ -164,6 +164,7 @@ template <class T> __device__ T ccd_test(const QueryKey& q, const T* X, const T*
 // Scene bounds over X and XTilde (Morton normalisation only: the pair set
 // does not depend on the tree shape): per-block partials, then one block.
 template <class T> __global__ void scene_partial_kernel(int nv, const T* X, const T* XT, T* part) {
+    if (!XT) XT = X;
     __shared__ T red[6][kTpbC];
     const int v = blockIdx.x * kTpbC + threadIdx.x;
     T lo[3] = {(T)FLT_MAX, (T)FLT_MAX, (T)FLT_MAX}, hi[3] = {(T)-FLT_MAX, (T)-FLT_MAX, (T)-FLT_MAX};
@@ -206,12 +207,13 @@ __device__ __forceinline__ uint32_t spread_bits(uint32_t x) {  // 10 bits -> eve
 }
 
 // Leaf box of a triangle's trajectory (ccd.cu:53-65): coordinates pass
-// through float (fminf/fmaxf), then -/+ 0.01 in the solver precision.  The
-// key is the 30-bit Morton code of the box centre above the triangle index,
-// so keys are unique.
+// through float (fminf/fmaxf), then -/+ 0.01 in the solver precision.  With
+// XT null, the box of the triangle at X grown by `bound` (computeTriTrajBBox,
+// ccd.cu:74-85, the UpdateQueries candidates).  The key is the 30-bit Morton
+// code of the box centre above the triangle index, so keys are unique.
 template <class T>
-__global__ void leaf_kernel(int nt, const uint32_t* tri, const T* X, const T* XT, const T* scene, T* leaf_box,
-                            unsigned long long* keys, uint32_t* order) {
+__global__ void leaf_kernel(int nt, const uint32_t* tri, const T* X, const T* XT, T bound, const T* scene,
+                            T* leaf_box, unsigned long long* keys, uint32_t* order) {
     const int i = blockIdx.x * kTpbC + threadIdx.x;
     if (i >= nt) return;
     const uint32_t t0 = tri[3 * (size_t)i], t1 = tri[3 * (size_t)i + 1], t2 = tri[3 * (size_t)i + 2];
@@ -219,11 +221,17 @@ __global__ void leaf_kernel(int nt, const uint32_t* tri, const T* X, const T* XT
     uint32_t cell[3];
     for (int k = 0; k < 3; ++k) {
         const float p0 = (float)X[3 * (size_t)t0 + k], p1 = (float)X[3 * (size_t)t1 + k], p2 = (float)X[3 * (size_t)t2 + k];
-        const float q0 = (float)XT[3 * (size_t)t0 + k], q1 = (float)XT[3 * (size_t)t1 + k], q2 = (float)XT[3 * (size_t)t2 + k];
-        const float mn = fminf(fminf(fminf(fminf(fminf(p0, p1), p2), q0), q1), q2);
-        const float mx = fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(p0, p1), p2), q0), q1), q2);
-        lo[k] = (T)mn - (T)0.01;
-        hi[k] = (T)mx + (T)0.01;
+        if (XT) {
+            const float q0 = (float)XT[3 * (size_t)t0 + k], q1 = (float)XT[3 * (size_t)t1 + k],
+                        q2 = (float)XT[3 * (size_t)t2 + k];
+            const float mn = fminf(fminf(fminf(fminf(fminf(p0, p1), p2), q0), q1), q2);
+            const float mx = fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(p0, p1), p2), q0), q1), q2);
+            lo[k] = (T)mn - (T)0.01;
+            hi[k] = (T)mx + (T)0.01;
+        } else {
+            lo[k] = (T)fminf(fminf(p0, p1), p2) - bound;
+            hi[k] = (T)fmaxf(fmaxf(p0, p1), p2) + bound;
+        }
         const double ext = (double)scene[3 + k] - (double)scene[k];
         double u = ext > 0 ? (0.5 * ((double)lo[k] + (double)hi[k]) - (double)scene[k]) / ext : 0.0;
         u = fmin(fmax(u * 1024.0, 0.0), 1023.0);
@@ -583,7 +591,7 @@ template <class T> CollisionDetector<T>::~CollisionDetector() {
     if (host_) (void)hipHostFree(host_);
 }
 
-template <class T> void CollisionDetector<T>::broad_phase(const T* X, const T* XT, bool ignore_self) {
+template <class T> void CollisionDetector<T>::broad_phase(const T* X, const T* XT, bool ignore_self, T bound) {
     nq_ = 0;
     npairs_ = 0;
     if (nt_ < 2) return;
@@ -592,7 +600,7 @@ template <class T> void CollisionDetector<T>::broad_phase(const T* X, const T* X
     hipLaunchKernelGGL(scene_partial_kernel<T>, dim3(vb), dim3(kTpbC), 0, stream_, nv_, X, XT, scene_.get());
     hipLaunchKernelGGL(scene_final_kernel<T>, dim3(1), dim3(64), 0, stream_, (int)vb, scene_.get(),
                        scene_.get() + 6 * (size_t)vb);
-    hipLaunchKernelGGL(leaf_kernel<T>, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n, tri_, X, XT,
+    hipLaunchKernelGGL(leaf_kernel<T>, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n, tri_, X, XT, bound,
                        (const T*)(scene_.get() + 6 * (size_t)vb), leaf_box_.get(), keys_.get(), order_in_.get());
     SBS_HIP(hipGetLastError());
     size_t bytes = sort_tmp_bytes_;
@@ -703,6 +711,12 @@ template <class T> double CollisionDetector<T>::min_step(const T* X, const T* XT
     SBS_HIP(hipMemcpyAsync(&r, red_.get(), sizeof(T), hipMemcpyDeviceToHost, stream_));
     SBS_HIP(hipStreamSynchronize(stream_));
     return (double)r;
+}
+
+template <class T> long long CollisionDetector<T>::proximity(const T* X, T bound, bool ignore_self) {
+    SBS_REQUIRE(X, "UpdateQueries: null array");
+    broad_phase(X, nullptr, ignore_self, bound);
+    return nq_;
 }
 
 template <class T> void CollisionDetector<T>::get_queries(uint32_t* q5, double* toi, double* n3) const {

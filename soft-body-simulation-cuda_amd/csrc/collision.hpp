@@ -40,6 +40,11 @@ public:
     // there is none).
     double min_step(const T* X, const T* XT, bool ignore_self);
 
+    // UpdateQueries' candidates (broadphase.cu:557-577): the canonical queries
+    // of the triangle pairs whose boxes at X, grown by `bound`, overlap; left
+    // on the device (queries()), count returned.  Synchronises the stream.
+    long long proximity(const T* X, T bound, bool ignore_self);
+    const QueryKey* queries() const { return qunique_.get(); }
     long long num_queries() const { return nq_; }
     int num_hits() const { return hits_; }
     long long num_pairs() const { return npairs_; }
@@ -48,7 +53,7 @@ public:
     void set_stream(hipStream_t s) { stream_ = s; }
 
 private:
-    void broad_phase(const T* X, const T* XT, bool ignore_self);
+    void broad_phase(const T* X, const T* XT, bool ignore_self, T bound = 0);
     void narrow_phase(const T* X, const T* XT);
 
     int nv_ = 0, nt_ = 0;

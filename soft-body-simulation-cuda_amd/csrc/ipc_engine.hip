@@ -1,0 +1,787 @@
+// FP64 IPC step on gfx950: energies, gradients, element Hessians with the
+// positive-definite projection, matrix-free Jacobi-PCG and the Newton loop.
+// Reference citations are per kernel; see ipc_engine.hpp for the layout.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "ipc_distance.hpp"
+#include "ipc_engine.hpp"
+#include "svd3.hpp"
+
+namespace sbs {
+
+namespace {
+
+constexpr int kT = 256;   // threads per block of the per-element kernels
+constexpr int kQB = 16;   // queries per block of the barrier kernel (LDS: 2 x 144 doubles per query)
+constexpr double kG = 9.8;  // GravityEnergy::g (gravity.h:14)
+
+unsigned nblk(size_t n) { return (unsigned)std::max<size_t>(1, (n + kT - 1) / kT); }
+
+using ipc::D3;
+
+__device__ __forceinline__ D3 ldp(const double* X, uint32_t v) {
+    return {X[3 * (size_t)v], X[3 * (size_t)v + 1], X[3 * (size_t)v + 2]};
+}
+
+// Block sums of NQ per-thread values (fixed tree order), written to
+// parts[NQ * block + k].  Every thread of the block calls it.
+template <int NQ> __device__ void block_sums(const double (&v)[NQ], double* parts) {
+    __shared__ double red[NQ][kT];
+    for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = kT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < NQ) parts[NQ * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[k] = sum over blocks of parts[NQ * b + k], in block order (one block).
+template <int NQ> __global__ void sum_parts_kernel(int nb, const double* parts, double* out) {
+    __shared__ double red[NQ][kT];
+    double acc[NQ];
+    for (int k = 0; k < NQ; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kT)
+        for (int k = 0; k < NQ; ++k) acc[k] += parts[NQ * (size_t)b + k];
+    for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = kT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < NQ) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void max_abs_partial_kernel(size_t n, const double* v, double* parts) {
+    __shared__ double red[kT];
+    double m = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+        const double a = fabs(v[i]);
+        m = (a > m || a != a) ? a : m;  // NaN propagates
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double a = red[threadIdx.x + s];
+            if (a > red[threadIdx.x] || a != a) red[threadIdx.x] = a;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = red[0];
+}
+__global__ void max_final_kernel(int nb, const double* parts, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double m = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const double a = parts[b];
+        if (a > m || a != a) m = a;
+    }
+    out[0] = m;
+}
+
+// FEMSolver ctor for the IPC data (femSolver.cu:7-17, solverUtil.cuh:109-137):
+// Dm^-1 (row-major) and V0 = |det Dm| / 6 at the positions given at creation.
+__global__ void ipc_precompute_kernel(int nt, const uint32_t* tet, const double* X, double* dm, double* v0) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t* tv = tet + 4 * (size_t)t;
+    double D[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 3; ++j) D[r][j] = X[3 * (size_t)tv[j + 1] + r] - X[3 * (size_t)tv[0] + r];
+    const double det = D[0][0] * (D[1][1] * D[2][2] - D[2][1] * D[1][2]) -
+                       D[0][1] * (D[1][0] * D[2][2] - D[2][0] * D[1][2]) +
+                       D[0][2] * (D[1][0] * D[2][1] - D[2][0] * D[1][1]);
+    const double id = 1.0 / det;
+    double* m = dm + 9 * (size_t)t;
+    m[0] = (D[1][1] * D[2][2] - D[1][2] * D[2][1]) * id;
+    m[1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) * id;
+    m[2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) * id;
+    m[3] = (D[1][2] * D[2][0] - D[1][0] * D[2][2]) * id;
+    m[4] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
+    m[5] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
+    m[6] = (D[1][0] * D[2][1] - D[1][1] * D[2][0]) * id;
+    m[7] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * id;
+    m[8] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
+    v0[t] = fabs(det) / 6.0;
+}
+
+__global__ void ipc_fixed_kernel(int nv, const double* dbc, unsigned char* fixed) {
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v < nv) fixed[v] = dbc[v] != 0.0;
+}
+
+// computeXTilde (ipc.cu:30-35) and x_n = X.
+__global__ void ipc_begin_kernel(int n3, const double* X, const double* V, double h, double damp, double* XT,
+                                 double* xn) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n3) return;
+    XT[i] = X[i] + h * V[i] * damp;
+    xn[i] = X[i];
+}
+
+// computeXMinusAP (ipc.cu:37-44): out = X - alpha p.
+__global__ void ipc_axpy_kernel(int n3, const double* X, const double* p, double alpha, double* out) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < n3) out[i] = X[i] - alpha * p[i];
+}
+
+// updateVel (ipc.cu:46-53).
+__global__ void ipc_velocity_kernel(int n3, const double* X, const double* xn, double h_inv, double* V) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < n3) V[i] = (X[i] - xn[i]) * h_inv;
+}
+
+// GetDistanceType + ComputeDistance (distance_type.cuh:15-45) per candidate;
+// count of the queries UpdateQueries keeps (d <= dhat, broadphase.cu:582).
+__global__ void ipc_query_kernel(long long nq, const QueryKey* q, const double* X, double dhat, int* dtype, double* d,
+                                 unsigned long long* kept) {
+    const long long i = (long long)blockIdx.x * kT + threadIdx.x;
+    if (i >= nq) return;
+    const QueryKey k = q[i];
+    const D3 P[4] = {ldp(X, k.v0), ldp(X, k.v1), ldp(X, k.v2), ldp(X, k.v3)};
+    const int t = ipc::distance_type((int)k.type, P);
+    const double dd = ipc::distance((int)k.type, t, P);
+    dtype[i] = t;
+    d[i] = dd;
+    if (dd <= dhat) atomicAdd(kept, 1ull);
+}
+
+__device__ __forceinline__ void deformation(const double* X, const uint32_t* tv, const double* m, double F[3][3]) {
+    double Ds[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 3; ++j) Ds[r][j] = X[3 * (size_t)tv[j + 1] + r] - X[3 * (size_t)tv[0] + r];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) F[r][c] = Ds[r][0] * m[c] + Ds[r][1] * m[3 + c] + Ds[r][2] * m[6 + c];
+}
+
+// b(d) = 1/2 dhat kappa / 8 (s - 1) ln s, s = d / dhat^2 (barrier.cu:11-15),
+// and its first and second derivatives in d (:17-29).
+__device__ __forceinline__ double barrier(double d, double dhat, double kappa) {
+    const double s = d / (dhat * dhat);
+    return 0.5 * dhat * kappa * 0.125 * (s - 1) * log(s);
+}
+__device__ __forceinline__ double barrier_d1(double d, double dhat, double kappa) {
+    const double dh2 = dhat * dhat, s = d / dh2;
+    return 0.5 * dhat * (kappa / 8 * (log(s) / dh2 + (s - 1) / d));
+}
+__device__ __forceinline__ double barrier_d2(double d, double dhat, double kappa) {
+    const double dh2 = dhat * dhat;
+    return 0.5 * dhat * (kappa / (8 * d * d * dh2) * (d + dh2));
+}
+
+// IPEnergy::Val (ip.cu:49-52) per element, summed per block:
+//   inertia 1/2 m |x - x~|^2 (inertia.cu:71-85)
+//   + h^2 (g m y (gravity.cu:23-34) + V0 (mu |S - I|^2 + lambda/2 tr(S - I)^2)
+//          (corotated.cu:196-214) + b(d) for d < dhat^2 (barrier.cu:137-155)).
+__global__ void ipc_energy_kernel(int nv, int nt, long long nq, const double* X, const double* XT,
+                                  const double* mass, const uint32_t* tet, const double* dm, const double* v0,
+                                  const double* mu, const double* lam, const double* qd, double dhat, double kappa,
+                                  double h2, double* parts) {
+    const long long i = (long long)blockIdx.x * kT + threadIdx.x;
+    double e[1] = {0.0};
+    if (i < nv) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double d = X[3 * i + k] - XT[3 * i + k];
+            s += d * d;
+        }
+        e[0] += 0.5 * mass[i] * s + h2 * kG * X[3 * i + 1] * mass[i];
+    }
+    if (i < nt) {
+        double F[3][3], U[3][3], sg[3], Vm[3][3];
+        deformation(X, tet + 4 * i, dm + 9 * i, F);
+        Svd3<double>::run(F, U, sg, Vm);
+        const double a = sg[0] - 1, b = sg[1] - 1, c = sg[2] - 1, tr = a + b + c;
+        e[0] += h2 * v0[i] * (mu[i] * (a * a + b * b + c * c) + 0.5 * lam[i] * tr * tr);
+    }
+    if (i < nq) {
+        const double d = qd[i];
+        if (d < dhat * dhat) e[0] += h2 * barrier(d, dhat, kappa);
+    }
+    block_sums<1>(e, parts);
+}
+
+// Inertia (coef 1) and gravity (coef h^2) gradients, inertia.cu:88-105 and
+// gravity.cu:6-13; diag = m (the inertia Hessian, inertia.cu:8-22).
+__global__ void ipc_grad_vertex_kernel(int nv, const double* X, const double* XT, const double* mass, double h2,
+                                       double* g, double* diag) {
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v >= nv) return;
+    const double m = mass[v];
+    for (int k = 0; k < 3; ++k) {
+        g[3 * (size_t)v + k] = m * (X[3 * (size_t)v + k] - XT[3 * (size_t)v + k]) + (k == 1 ? h2 * kG * m : 0.0);
+        if (diag) diag[3 * (size_t)v + k] = m;
+    }
+}
+
+// Corotated elasticity of one tet (CorotatedEnergy::GradHessianKern,
+// corotated.cu:100-186), scaled by coef: the gradient V0 P Dm^-T with
+// P = 2 mu (F - R) + lambda tr(R^T F - I) R into g (atomics), and with ht the
+// 12x12 Hessian G^T d2Psi/dF2 G (G = dvec(F)/dx), where
+//   d2Psi/dF2 = 2 mu I + lambda g1 g1^T + sum_k c lambda_k t_k t_k^T,
+//   g1 = vec R, t_k = vec(U T_k V^T)/sqrt 2 (twists of the pairs 01, 12, 20),
+//   c = lambda tr S - 2 mu - 3 lambda, lambda_k = 2/(s_a + s_b) (1 if s_a + s_b < 2);
+// the elastic Newton Hessian is not projected to positive definite there.
+__global__ void __launch_bounds__(kT) ipc_elastic_kernel(int nt, const uint32_t* tet, const double* X, const double* dm,
+                                                         const double* v0, const double* mu, const double* lam,
+                                                         double coef, const unsigned char* fixed, double* g, double* ht,
+                                                         double* diag) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t* tv = tet + 4 * (size_t)t;
+    const double* m = dm + 9 * (size_t)t;
+    double F[3][3], U[3][3], sg[3], Vm[3][3];
+    deformation(X, tv, m, F);
+    Svd3<double>::run(F, U, sg, Vm);
+    double R[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) R[r][c] = U[r][0] * Vm[c][0] + U[r][1] * Vm[c][1] + U[r][2] * Vm[c][2];
+    double trRF = 0.0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) trRF += R[r][c] * F[r][c];
+    const double mu_ = mu[t], la = lam[t], w = coef * v0[t];
+    // Gv[a][c]: d F[r][c] / d x_{a, r}
+    double Gv[4][3];
+    for (int c = 0; c < 3; ++c) {
+        Gv[1][c] = m[c];
+        Gv[2][c] = m[3 + c];
+        Gv[3][c] = m[6 + c];
+        Gv[0][c] = -(m[c] + m[3 + c] + m[6 + c]);
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int c = 0; c < 3; ++c)
+                acc += (2 * mu_ * (F[r][c] - R[r][c]) + la * (trRF - 3.0) * R[r][c]) * Gv[a][c];
+            atomicAdd(&g[3 * (size_t)tv[a] + r], w * acc);
+        }
+    if (!ht) return;
+    double tw[3][3][3];  // tw[k][r][c]
+    const int ta[3] = {0, 1, 2}, tb[3] = {1, 2, 0};
+    const double is2 = 1.0 / sqrt(2.0);
+    for (int k = 0; k < 3; ++k)
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                tw[k][r][c] = (U[r][ta[k]] * Vm[c][tb[k]] - U[r][tb[k]] * Vm[c][ta[k]]) * is2;
+    double lk[3] = {2 / (sg[0] + sg[1]), 2 / (sg[1] + sg[2]), 2 / (sg[0] + sg[2])};
+    if (sg[0] + sg[1] < 2) lk[0] = 1;
+    if (sg[1] + sg[2] < 2) lk[1] = 1;
+    if (sg[0] + sg[2] < 2) lk[2] = 1;
+    const double cc = la * (sg[0] + sg[1] + sg[2]) - 2 * mu_ - 3 * la;
+    double* H = ht + 144 * (size_t)t;
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k)
+            for (int b = 0; b < 4; ++b)
+                for (int l = 0; l < 3; ++l) {
+                    // sum_{c,c'} Gv[a][c] Gv[b][c'] D((k,c), (l,c'))
+                    double acc = 0.0;
+                    for (int c = 0; c < 3; ++c)
+                        for (int c2 = 0; c2 < 3; ++c2) {
+                            double Dv = (k == l && c == c2 ? 2 * mu_ : 0.0) + la * R[k][c] * R[l][c2];
+                            for (int q = 0; q < 3; ++q) Dv += cc * lk[q] * tw[q][k][c] * tw[q][l][c2];
+                            acc += Gv[a][c] * Gv[b][c2] * Dv;
+                        }
+                    H[12 * (3 * a + k) + 3 * b + l] = w * acc;
+                }
+    if (diag)
+        for (int a = 0; a < 4; ++a)
+            if (!fixed[tv[a]])
+                for (int k = 0; k < 3; ++k) atomicAdd(&diag[3 * (size_t)tv[a] + k], H[13 * (3 * a + k)]);
+}
+
+// makePD (matrix.h:441-518) on a 12x12 matrix held in LDS (entry (i, j) of
+// this thread at M[(12 i + j) * kQB + lane]): cyclic Jacobi over the pairs
+// p < q, rotations skipped when |a_pq| < 1e-9, at most 20 sweeps; eigenvalues
+// below 1e-6 raised to 1e-6; M <- V diag V^T.
+__device__ void make_pd_lds(double* M, double* V, int lane) {
+    auto m = [&](int i, int j) -> double& { return M[(12 * i + j) * kQB + lane]; };
+    auto v = [&](int i, int j) -> double& { return V[(12 * i + j) * kQB + lane]; };
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) v(i, j) = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 20; ++sweep) {
+        bool converged = true;
+        for (int p = 0; p < 12; ++p)
+            for (int q = p + 1; q < 12; ++q) {
+                const double apq = m(p, q);
+                if (fabs(apq) < 1e-9) continue;
+                converged = false;
+                const double app = m(p, p), aqq = m(q, q);
+                const double theta = 0.5 * (aqq - app) / apq;
+                double t;
+                if (fabs(theta) > 1e10) t = 0.5 / theta;
+                else t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                m(p, q) = 0.0;
+                m(q, p) = 0.0;
+                m(p, p) = c * c * app - 2.0 * s * c * apq + s * s * aqq;
+                m(q, q) = s * s * app + 2.0 * s * c * apq + c * c * aqq;
+                for (int k = 0; k < 12; ++k) {
+                    if (k == p || k == q) continue;
+                    const double akp = m(k, p), akq = m(k, q);
+                    m(k, p) = c * akp - s * akq;
+                    m(p, k) = m(k, p);
+                    m(k, q) = s * akp + c * akq;
+                    m(q, k) = m(k, q);
+                }
+                for (int k = 0; k < 12; ++k) {
+                    const double vkp = v(k, p), vkq = v(k, q);
+                    v(k, p) = c * vkp - s * vkq;
+                    v(k, q) = s * vkp + c * vkq;
+                }
+            }
+        if (converged) break;
+    }
+    double lamv[12];
+    for (int i = 0; i < 12; ++i) lamv[i] = fmax(m(i, i), 1e-6);
+    for (int i = 0; i < 12; ++i)
+        for (int j = i; j < 12; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 12; ++k) s += v(i, k) * lamv[k] * v(j, k);
+            m(i, j) = s;  // only the upper triangle is read back: rows below are still the input
+        }
+    for (int i = 1; i < 12; ++i)
+        for (int j = 0; j < i; ++j) m(i, j) = m(j, i);
+}
+
+// Barrier of one query (Barrier::gradHessianKern, barrier.cu:99-173),
+// scaled by coef: gradient coef b'(d) grad d into g (atomics), and with hq
+// the Hessian coef (b''(d) grad d grad d^T + b'(d) hess d), made positive
+// definite.  Queries with d > dhat^2 contribute nothing.
+__global__ void __launch_bounds__(kQB) ipc_barrier_kernel(long long nq, const QueryKey* qk, const int* dtype,
+                                                          const double* qd, const double* X, double dhat, double kappa,
+                                                          double coef, const unsigned char* fixed, double* g,
+                                                          double* hq, double* diag) {
+    __shared__ double sM[144 * kQB];
+    __shared__ double sV[144 * kQB];
+    const long long i = (long long)blockIdx.x * kQB + threadIdx.x;
+    if (i >= nq) return;  // no barrier below
+    const double d = qd[i];
+    if (d > dhat * dhat) return;
+    const QueryKey k = qk[i];
+    const D3 P[4] = {ldp(X, k.v0), ldp(X, k.v1), ldp(X, k.v2), ldp(X, k.v3)};
+    const uint32_t vv[4] = {k.v0, k.v1, k.v2, k.v3};
+    const int lane = threadIdx.x;
+    double gd[12];
+    double* M = sM;
+    if (hq)
+        for (int e = 0; e < 144; ++e) M[e * kQB + lane] = 0.0;
+    ipc::distance_derivatives((int)k.type, dtype[i], P, gd, [&](int a, int b, const ipc::M3& blk, double sgn) {
+        if (!hq) return;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) M[(12 * (3 * a + r) + 3 * b + c) * kQB + lane] += sgn * blk.m[r][c];
+    });
+    const double d1 = barrier_d1(d, dhat, kappa);
+    for (int j = 0; j < 12; ++j) atomicAdd(&g[3 * (size_t)vv[j / 3] + j % 3], coef * d1 * gd[j]);
+    if (!hq) return;
+    const double d2 = barrier_d2(d, dhat, kappa);
+    for (int a = 0; a < 12; ++a)
+        for (int b = 0; b < 12; ++b) {
+            double& e = M[(12 * a + b) * kQB + lane];
+            e = coef * (d2 * gd[a] * gd[b] + d1 * e);
+        }
+    make_pd_lds(M, sV, lane);
+    double* H = hq + 144 * (size_t)i;
+    for (int e = 0; e < 144; ++e) H[e] = M[e * kQB + lane];
+    if (diag)
+        for (int a = 0; a < 4; ++a)
+            if (!fixed[vv[a]])
+                for (int r = 0; r < 3; ++r) atomicAdd(&diag[3 * (size_t)vv[a] + r], H[13 * (3 * a + r)]);
+}
+
+// DOFElimination (ipc.cu:55-80): zero gradient and identity row/column on fixed vertices.
+__global__ void ipc_fix_kernel(int nv, const unsigned char* fixed, double* g, double* diag) {
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v >= nv || !fixed[v]) return;
+    for (int k = 0; k < 3; ++k) {
+        g[3 * (size_t)v + k] = 0.0;
+        if (diag) diag[3 * (size_t)v + k] = 1.0;
+    }
+}
+
+// y = H x, element by element: inertia diagonal (identity on fixed rows),
+// then each tet's and each active query's 12x12 block on free rows/columns.
+__global__ void ipc_mv_vertex_kernel(int nv, const double* mass, const unsigned char* fixed, const double* x,
+                                     double* y) {
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v >= nv) return;
+    const double m = fixed[v] ? 1.0 : mass[v];
+    for (int k = 0; k < 3; ++k) y[3 * (size_t)v + k] = m * x[3 * (size_t)v + k];
+}
+__device__ __forceinline__ void mv_block(const uint32_t (&vv)[4], const double* H, const unsigned char* fixed,
+                                         const double* x, double* y) {
+    double xl[12];
+    bool fr[4];
+    for (int a = 0; a < 4; ++a) {
+        fr[a] = fixed[vv[a]] != 0;
+        for (int r = 0; r < 3; ++r) xl[3 * a + r] = fr[a] ? 0.0 : x[3 * (size_t)vv[a] + r];
+    }
+    for (int a = 0; a < 4; ++a) {
+        if (fr[a]) continue;
+        for (int r = 0; r < 3; ++r) {
+            const double* row = H + 12 * (3 * a + r);
+            double s = 0.0;
+            for (int j = 0; j < 12; ++j) s += row[j] * xl[j];
+            atomicAdd(&y[3 * (size_t)vv[a] + r], s);
+        }
+    }
+}
+__global__ void ipc_mv_tet_kernel(int nt, const uint32_t* tet, const double* ht, const unsigned char* fixed,
+                                  const double* x, double* y) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t vv[4] = {tet[4 * (size_t)t], tet[4 * (size_t)t + 1], tet[4 * (size_t)t + 2], tet[4 * (size_t)t + 3]};
+    mv_block(vv, ht + 144 * (size_t)t, fixed, x, y);
+}
+__global__ void ipc_mv_query_kernel(long long nq, const QueryKey* qk, const double* qd, double dhat2, const double* hq,
+                                    const unsigned char* fixed, const double* x, double* y) {
+    const long long i = (long long)blockIdx.x * kT + threadIdx.x;
+    if (i >= nq || qd[i] > dhat2) return;
+    const QueryKey k = qk[i];
+    const uint32_t vv[4] = {k.v0, k.v1, k.v2, k.v3};
+    mv_block(vv, hq + 144 * (size_t)i, fixed, x, y);
+}
+
+// Jacobi-PCG pieces (n = 3 N_v).
+__global__ void ipc_pcg_init_kernel(int n, const double* b, const double* diag, double* x, double* r, double* z,
+                                    double* p, double* parts) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const double ri = b[i], zi = ri / diag[i];
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        v[0] = ri * zi;
+        v[1] = ri * ri;
+    }
+    block_sums<2>(v, parts);
+}
+__global__ void ipc_dot_kernel(int n, const double* a, const double* b, double* parts) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    double v[1] = {i < n ? a[i] * b[i] : 0.0};
+    block_sums<1>(v, parts);
+}
+__global__ void ipc_pcg_update_kernel(int n, const double* alpha, const double* p, const double* q,
+                                      const double* diag, double* x, double* r, double* z, double* parts) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const double a = alpha[0];
+        x[i] += a * p[i];
+        const double ri = r[i] - a * q[i], zi = ri / diag[i];
+        r[i] = ri;
+        z[i] = zi;
+        v[0] = ri * zi;
+        v[1] = ri * ri;
+    }
+    block_sums<2>(v, parts);
+}
+__global__ void ipc_pcg_dir_kernel(int n, const double* beta, const double* z, double* p) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta[0] * p[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+IpcEngine::IpcEngine(const IpcData& data) {
+    SBS_REQUIRE(data.num_verts > 0 && data.num_tets > 0, "IPCSolver: empty mesh");
+    SBS_REQUIRE(data.X && data.V && data.XTilde && data.Tet && data.mass && data.mu && data.lambda && data.DBC,
+                "IPCSolver: a required SolverData pointer is null");
+    SBS_REQUIRE(data.num_tris == 0 || (data.Tri && data.TriFathers), "IPCSolver: surface arrays are null");
+    nv_ = data.num_verts;
+    nt_ = data.num_tets;
+    ntri_ = data.num_tris;
+    d_ = data;
+    {
+        std::vector<uint32_t> tet(4 * (size_t)nt_);
+        SBS_HIP(hipMemcpy(tet.data(), data.Tet, tet.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (uint32_t v : tet) SBS_REQUIRE(v < (uint32_t)nv_, "IPCSolver: tet index out of range");
+        if (ntri_ > 0) {
+            std::vector<uint32_t> tri(3 * (size_t)ntri_);
+            SBS_HIP(hipMemcpy(tri.data(), data.Tri, tri.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            for (uint32_t v : tri) SBS_REQUIRE(v < (uint32_t)nv_, "IPCSolver: triangle index out of range");
+        }
+    }
+    const size_t n3 = 3 * (size_t)nv_;
+    dm_.alloc(9 * (size_t)nt_);
+    v0_.alloc(nt_);
+    fixed_.alloc(nv_);
+    xn_.alloc(n3); xtmp_.alloc(n3); p_dir_.alloc(n3); g_.alloc(n3); diag_.alloc(n3);
+    r_.alloc(n3); z_.alloc(n3); q_.alloc(n3);
+    ht_.alloc(144 * (size_t)nt_);
+    parts_.alloc(2 * (size_t)nblk(std::max<size_t>(n3, (size_t)nt_)) + 64);
+    out_.alloc(8);
+    SBS_HIP(hipHostMalloc((void**)&host_, 8 * sizeof(double), hipHostMallocDefault));
+    hipLaunchKernelGGL(ipc_precompute_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, data.Tet,
+                       (const double*)data.X, dm_.get(), v0_.get());
+    hipLaunchKernelGGL(ipc_fixed_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, data.DBC, fixed_.get());
+    SBS_HIP(hipGetLastError());
+    col_ = std::make_unique<CollisionDetector<double>>(nv_, ntri_, data.Tri, data.TriFathers, stream_);
+    SBS_HIP(hipStreamSynchronize(stream_));
+}
+
+IpcEngine::~IpcEngine() {
+    if (host_) (void)hipHostFree(host_);
+}
+
+void IpcEngine::set_stream(hipStream_t s) {
+    stream_ = s;
+    col_->set_stream(s);
+}
+
+double IpcEngine::reduce_sum(const double* parts, int n) {
+    hipLaunchKernelGGL(sum_parts_kernel<1>, dim3(1), dim3(kT), 0, stream_, n, parts, out_.get());
+    SBS_HIP(hipMemcpyAsync(host_, out_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SBS_HIP(hipStreamSynchronize(stream_));
+    return host_[0];
+}
+
+double IpcEngine::max_abs(const double* v, size_t n) {
+    const unsigned nb = std::min<unsigned>(nblk(n), 1024);
+    hipLaunchKernelGGL(max_abs_partial_kernel, dim3(nb), dim3(kT), 0, stream_, n, v, parts_.get());
+    hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(64), 0, stream_, (int)nb, (const double*)parts_.get(),
+                       out_.get());
+    SBS_HIP(hipMemcpyAsync(host_, out_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SBS_HIP(hipStreamSynchronize(stream_));
+    return host_[0];
+}
+
+// UpdateQueries (broadphase.cu:574-587): candidates of the boxes at X grown by
+// 2 dhat, then the distance type and squared distance of each.
+void IpcEngine::update_queries(const double* X) {
+    nq_ = ntri_ >= 2 ? col_->proximity(X, 2 * p_.dhat, ignore_self_) : 0;
+    if ((size_t)nq_ > qcap_) {
+        qcap_ = (size_t)nq_ + (size_t)nq_ / 4 + 16;
+        qtype_.alloc(qcap_);
+        qd_.alloc(qcap_);
+        hq_.alloc(144 * qcap_);
+        const size_t need = 2 * (size_t)nblk(std::max<size_t>({3 * (size_t)nv_, (size_t)nt_, qcap_})) + 64;
+        if (need > parts_.n) parts_.alloc(need);
+    }
+    if (nq_ == 0) {
+        stats_.queries = 0;
+        return;
+    }
+    unsigned long long* kept = reinterpret_cast<unsigned long long*>(out_.get() + 7);
+    SBS_HIP(hipMemsetAsync(kept, 0, sizeof(unsigned long long), stream_));
+    hipLaunchKernelGGL(ipc_query_kernel, dim3(nblk(nq_)), dim3(kT), 0, stream_, nq_, col_->queries(),
+                       (const double*)X, p_.dhat, qtype_.get(), qd_.get(), kept);
+    SBS_HIP(hipGetLastError());
+    unsigned long long nk = 0;
+    SBS_HIP(hipMemcpyAsync(&nk, kept, sizeof(nk), hipMemcpyDeviceToHost, stream_));
+    SBS_HIP(hipStreamSynchronize(stream_));
+    stats_.queries = (long long)nk;
+}
+
+double IpcEngine::energy(const double* X) {
+    const size_t n = std::max<size_t>({(size_t)nv_, (size_t)nt_, (size_t)nq_});
+    const unsigned nb = nblk(n);
+    hipLaunchKernelGGL(ipc_energy_kernel, dim3(nb), dim3(kT), 0, stream_, nv_, nt_, nq_, X, (const double*)d_.XTilde,
+                       d_.mass, d_.Tet, (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda,
+                       (const double*)qd_.get(), p_.dhat, kappa_, p_.dt * p_.dt, parts_.get());
+    SBS_HIP(hipGetLastError());
+    return reduce_sum(parts_.get(), (int)nb);
+}
+
+// IPEnergy::GradientHessian (ip.cu:54-75): inertia (coef 1), gravity,
+// elasticity and barrier (coef h^2); with `hessian` the element Hessians,
+// the Jacobi diagonal and the DOF elimination.
+void IpcEngine::gradient(const double* X, bool hessian, double kap) {
+    const double h2 = p_.dt * p_.dt;
+    hipLaunchKernelGGL(ipc_grad_vertex_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, X,
+                       (const double*)d_.XTilde, d_.mass, h2, g_.get(), hessian ? diag_.get() : nullptr);
+    hipLaunchKernelGGL(ipc_elastic_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet, X,
+                       (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda, h2,
+                       (const unsigned char*)fixed_.get(), g_.get(), hessian ? ht_.get() : nullptr,
+                       hessian ? diag_.get() : nullptr);
+    if (nq_ > 0)
+        hipLaunchKernelGGL(ipc_barrier_kernel, dim3((unsigned)((nq_ + kQB - 1) / kQB)), dim3(kQB), 0, stream_, nq_,
+                           col_->queries(), (const int*)qtype_.get(), (const double*)qd_.get(), X, p_.dhat, kap, h2,
+                           (const unsigned char*)fixed_.get(), g_.get(), hessian ? hq_.get() : nullptr,
+                           hessian ? diag_.get() : nullptr);
+    if (hessian)
+        hipLaunchKernelGGL(ipc_fix_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_,
+                           (const unsigned char*)fixed_.get(), g_.get(), diag_.get());
+    SBS_HIP(hipGetLastError());
+}
+
+// UpdateKappa (ip.cu:77-115): kappa = max(100, max|grad of the other terms| /
+// max|grad of the barrier with kappa = 1|), 100 when the latter is below 1e-9.
+void IpcEngine::update_kappa(const double* X) {
+    const size_t n3 = 3 * (size_t)nv_;
+    const double h2 = p_.dt * p_.dt;
+    hipLaunchKernelGGL(ipc_grad_vertex_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, X,
+                       (const double*)d_.XTilde, d_.mass, h2, g_.get(), (double*)nullptr);
+    hipLaunchKernelGGL(ipc_elastic_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet, X,
+                       (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda, h2,
+                       (const unsigned char*)fixed_.get(), g_.get(), (double*)nullptr, (double*)nullptr);
+    SBS_HIP(hipGetLastError());
+    const double ge = max_abs(g_.get(), n3);
+    SBS_HIP(hipMemsetAsync(g_.get(), 0, g_.bytes(), stream_));
+    if (nq_ > 0)
+        hipLaunchKernelGGL(ipc_barrier_kernel, dim3((unsigned)((nq_ + kQB - 1) / kQB)), dim3(kQB), 0, stream_, nq_,
+                           col_->queries(), (const int*)qtype_.get(), (const double*)qd_.get(), X, p_.dhat, 1.0, h2,
+                           (const unsigned char*)fixed_.get(), g_.get(), (double*)nullptr, (double*)nullptr);
+    SBS_HIP(hipGetLastError());
+    const double gb = nq_ > 0 ? max_abs(g_.get(), n3) : 0.0;
+    kappa_ = gb > 1e-9 ? std::max(100.0, ge / gb) : 100.0;
+}
+
+// Jacobi-PCG on H p = g, element by element; returns the iterations.
+int IpcEngine::solve() {
+    const int n = 3 * nv_;
+    const unsigned nb = nblk(n);
+    double* parts = parts_.get();
+    double* out = out_.get();  // [0] rz, [1] rr, [2] pq, [3] alpha, [4] beta
+    hipLaunchKernelGGL(ipc_pcg_init_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)g_.get(),
+                       (const double*)diag_.get(), p_dir_.get(), r_.get(), z_.get(), xtmp_.get(), parts);
+    hipLaunchKernelGGL(sum_parts_kernel<2>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)parts, out);
+    SBS_HIP(hipMemcpyAsync(host_, out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SBS_HIP(hipStreamSynchronize(stream_));
+    double rz = host_[0];
+    const double bn = std::sqrt(host_[1]);
+    if (bn == 0.0) return 0;
+    double* pvec = xtmp_.get();  // the PCG search direction (x_tmp is free during the solve)
+    int it = 0;
+    for (; it < p_.lin_max_iter;) {
+        hipLaunchKernelGGL(ipc_mv_vertex_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, d_.mass,
+                           (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get());
+        hipLaunchKernelGGL(ipc_mv_tet_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet,
+                           (const double*)ht_.get(), (const unsigned char*)fixed_.get(), (const double*)pvec,
+                           q_.get());
+        if (nq_ > 0)
+            hipLaunchKernelGGL(ipc_mv_query_kernel, dim3(nblk(nq_)), dim3(kT), 0, stream_, nq_, col_->queries(),
+                               (const double*)qd_.get(), p_.dhat * p_.dhat, (const double*)hq_.get(),
+                               (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get());
+        hipLaunchKernelGGL(ipc_dot_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)pvec,
+                           (const double*)q_.get(), parts);
+        hipLaunchKernelGGL(sum_parts_kernel<1>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)parts,
+                           out + 2);
+        SBS_HIP(hipMemcpyAsync(host_ + 2, out + 2, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        SBS_HIP(hipStreamSynchronize(stream_));
+        const double pq = host_[2];
+        if (!(pq > 0.0)) break;  // not positive definite along p: keep the iterate
+        host_[3] = rz / pq;
+        SBS_HIP(hipMemcpyAsync(out + 3, host_ + 3, sizeof(double), hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(ipc_pcg_update_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)(out + 3),
+                           (const double*)pvec, (const double*)q_.get(), (const double*)diag_.get(), p_dir_.get(),
+                           r_.get(), z_.get(), parts);
+        hipLaunchKernelGGL(sum_parts_kernel<2>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)parts, out);
+        SBS_HIP(hipMemcpyAsync(host_, out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        SBS_HIP(hipStreamSynchronize(stream_));
+        ++it;
+        if (std::sqrt(host_[1]) <= p_.lin_tol * bn) break;
+        host_[4] = host_[0] / rz;
+        rz = host_[0];
+        SBS_HIP(hipMemcpyAsync(out + 4, host_ + 4, sizeof(double), hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(ipc_pcg_dir_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)(out + 4),
+                           (const double*)z_.get(), pvec);
+        SBS_HIP(hipGetLastError());
+    }
+    return it;
+}
+
+// SearchDirection (ipc.cu:230-239): gradient and Hessian at X, elimination,
+// p = H^-1 g; false when the gradient holds NaN or Inf.
+bool IpcEngine::search_direction(const double* X) {
+    gradient(X, true, kappa_);
+    const double gm = max_abs(g_.get(), 3 * (size_t)nv_);
+    if (!std::isfinite(gm)) return false;
+    stats_.linear_iterations = solve();
+    return true;
+}
+
+bool IpcEngine::step(IpcData& data, const IpcParams& p) {
+    d_ = data;
+    p_ = p;
+    const double h = p.dt;
+    const int n3 = 3 * nv_;
+    const unsigned vb = nblk(n3);
+    stats_.newton_iterations = 0;
+    stats_.line_search_halvings = 0;
+    hipLaunchKernelGGL(ipc_begin_kernel, dim3(vb), dim3(kT), 0, stream_, n3, (const double*)data.X,
+                       (const double*)data.V, h, p.damp, data.XTilde, xn_.get());
+    SBS_HIP(hipGetLastError());
+    update_queries(data.X);
+    update_kappa(data.X);
+    stats_.kappa = kappa_;
+    double E_last = energy(data.X);
+    if (!std::isfinite(E_last)) return false;
+    if (!search_direction(data.X)) return false;
+    double* X = data.X;
+    double* xt = xtmp_.get();
+    while (!(max_abs(p_dir_.get(), n3) / h < p.tol)) {  // EndCondition (ipc.cu:243-250)
+        if (++stats_.newton_iterations > p.max_iterations) return false;
+        hipLaunchKernelGGL(ipc_axpy_kernel, dim3(vb), dim3(kT), 0, stream_, n3, (const double*)X,
+                           (const double*)p_dir_.get(), 1.0, xt);
+        // InitStepSize (ip.cu:116-123, barrier.cu:225-228): CCD over X -> X - p
+        const double toi = ntri_ >= 2 ? col_->min_step(X, xt, true) : 1.0;
+        ignore_self_ = true;
+        double alpha = 0.95 * std::min(1.0, 0.9 * toi);
+        alpha = alpha < 1e-12 ? 0.0 : std::min(1.0, alpha);
+        if (stats_.newton_iterations == 1) stats_.alpha0 = alpha;
+        while (true) {
+            hipLaunchKernelGGL(ipc_axpy_kernel, dim3(vb), dim3(kT), 0, stream_, n3, (const double*)X,
+                               (const double*)p_dir_.get(), alpha, xt);
+            update_queries(xt);
+            const double E = energy(xt);
+            if (!std::isfinite(E)) return false;
+            if (E > E_last) {
+                alpha /= 2;
+                ++stats_.line_search_halvings;
+            } else {
+                break;
+            }
+            if (alpha < DBL_EPSILON) break;
+        }
+        SBS_HIP(hipMemcpyAsync(X, xt, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        update_queries(X);
+        E_last = energy(X);
+        if (!std::isfinite(E_last)) return false;
+        if (!search_direction(X)) return false;
+    }
+    stats_.energy = E_last;
+    hipLaunchKernelGGL(ipc_velocity_kernel, dim3(vb), dim3(kT), 0, stream_, n3, (const double*)X,
+                       (const double*)xn_.get(), 1.0 / h, data.V);
+    SBS_HIP(hipGetLastError());
+    SBS_HIP(hipStreamSynchronize(stream_));
+    return true;
+}
+
+bool IpcEngine::update(IpcData& data, const IpcParams& params) {
+    SBS_REQUIRE(params.dt > 0 && params.dhat > 0 && params.tol > 0, "IPCSolver: dt, dhat and tol must be positive");
+    SBS_REQUIRE(params.max_iterations >= 1 && params.lin_max_iter >= 1, "IPCSolver: iteration limits must be >= 1");
+    if (failed_) return false;
+    const bool ok = step(data, params);
+    if (!ok) failed_ = true;
+    stats_.failed = failed_ ? 1 : 0;
+    return ok;
+}
+
+void IpcEngine::set_state(const double* XTilde, double kappa) {
+    SBS_HIP(hipMemcpy(d_.XTilde, XTilde, 3 * (size_t)nv_ * sizeof(double), hipMemcpyHostToDevice));
+    kappa_ = kappa;
+}
+
+double IpcEngine::energy_at(const double* Xh, double* grad_host) {
+    SBS_HIP(hipMemcpyAsync(xtmp_.get(), Xh, 3 * (size_t)nv_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    update_queries(xtmp_.get());
+    const double e = energy(xtmp_.get());
+    if (grad_host) {
+        gradient(xtmp_.get(), false, kappa_);
+        SBS_HIP(hipMemcpyAsync(grad_host, g_.get(), 3 * (size_t)nv_ * sizeof(double), hipMemcpyDeviceToHost,
+                               stream_));
+        SBS_HIP(hipStreamSynchronize(stream_));
+    }
+    return e;
+}
+
+}  // namespace sbs

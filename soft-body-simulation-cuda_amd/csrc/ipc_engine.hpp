@@ -1,0 +1,105 @@
+// FP64 incremental-potential-contact (IPC) time step on gfx950.
+//
+// Reference: IPCSolver (src/simulation/solver/IPC/ipc.h:11-40, ipc.cu:101-258)
+// with IPEnergy (src/simulation/energy/ip.cu): Newton on
+//   E(x) = 1/2 |x - x~|_M^2 + h^2 (gravity + corotated elasticity + barrier)
+// over the squared distances of the contact queries (UpdateQueries), with a
+// line search started at 0.95 min(1, 0.9 toi) (CCD over x -> x - p) and halved
+// until E does not increase, until |p|_inf / h < tol.
+//
+// Layout: per-tet Dm^-1 (row-major) and V0; per Newton iteration the 12x12
+// Hessian of every tet and of every active query (row-major, the query's or
+// tet's 4 vertices x 3 coordinates) and the gradient; the linear system
+// H p = g is solved by Jacobi-PCG applied element by element (no assembled
+// matrix), with fixed degrees of freedom as rows and columns of the identity.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "collision.hpp"
+#include "common.hpp"
+
+namespace sbs {
+
+struct IpcData {
+    int num_verts = 0, num_tets = 0, num_tris = 0;
+    double* X = nullptr;        // 3*N_v positions (updated by a step)
+    double* V = nullptr;        // 3*N_v velocities (updated by a step)
+    double* XTilde = nullptr;   // 3*N_v x~ of the step (written)
+    const uint32_t* Tet = nullptr;
+    const uint32_t* Tri = nullptr;
+    const uint32_t* TriFathers = nullptr;
+    const double* mass = nullptr;    // N_v
+    const double* mu = nullptr;      // N_t
+    const double* lambda = nullptr;  // N_t
+    const double* DBC = nullptr;     // N_v: nonzero = fixed vertex (DBCIdx of the reference)
+};
+
+struct IpcParams {
+    double dt = 0.01;
+    double damp = 0.999;
+    double dhat = 1e-3;
+    double tol = 5e-2;        // Newton stop: |p|_inf / dt < tol
+    int max_iterations = 100; // Newton iterations per step
+    double lin_tol = 1e-13;   // PCG: |r| <= lin_tol |g|
+    int lin_max_iter = 20000;
+};
+
+struct IpcStats {
+    int newton_iterations = 0, line_search_halvings = 0, linear_iterations = 0;
+    long long queries = 0, active_queries = 0;
+    double kappa = 0.0, alpha0 = 1.0, energy = 0.0;
+    int failed = 0;
+};
+
+class IpcEngine {
+public:
+    explicit IpcEngine(const IpcData& data);
+    ~IpcEngine();
+    // IPCSolver::Update: one SolverStep unless a previous step failed
+    // (the latched `failed`); returns false when this step failed.
+    bool update(IpcData& data, const IpcParams& params);
+    void reset() { failed_ = false; }
+    const IpcStats& stats() const { return stats_; }
+    void set_stream(hipStream_t s);
+    // Energy and gradient at X with the current x~ and kappa, the queries
+    // recomputed at X (tests).
+    double energy_at(const double* X, double* grad_host);
+    void set_state(const double* XTilde, double kappa);
+
+private:
+    bool step(IpcData& data, const IpcParams& p);
+    void update_queries(const double* X);
+    double energy(const double* X);
+    void gradient(const double* X, bool hessian, double coef_barrier_kappa);
+    void update_kappa(const double* X);
+    bool search_direction(const double* X);
+    int solve();
+    double reduce_sum(const double* parts, int n);
+    double max_abs(const double* v, size_t n);
+
+    int nv_ = 0, nt_ = 0, ntri_ = 0;
+    IpcData d_{};
+    IpcParams p_{};
+    hipStream_t stream_ = nullptr;
+    bool failed_ = false;
+    bool ignore_self_ = false;  // the detector's flag: UpdateQueries uses the last value set (bvh.h:82)
+    double kappa_ = 100.0;
+    IpcStats stats_{};
+    std::unique_ptr<CollisionDetector<double>> col_;
+    DevBuf<double> dm_, v0_;                // Dm^-1 (9 per tet, row-major), V0
+    DevBuf<unsigned char> fixed_;
+    DevBuf<double> xn_, xtmp_, p_dir_, g_, diag_;
+    DevBuf<double> ht_, hq_;                // 144 per tet / per query
+    DevBuf<int> qtype_;                     // distance type per candidate query
+    DevBuf<double> qd_;                     // squared distance per candidate query
+    size_t qcap_ = 0;
+    long long nq_ = 0;
+    DevBuf<double> r_, z_, q_;              // PCG vectors
+    DevBuf<double> parts_, out_;            // reduction partials, results
+    double* host_ = nullptr;                // pinned scalars
+};
+
+}  // namespace sbs

@@ -59,6 +59,26 @@ class SbsPdStats(ctypes.Structure):
                 ("halo_groups", ctypes.c_int)]
 
 
+class SbsIpcData(ctypes.Structure):
+    _fields_ = [("num_verts", ctypes.c_int), ("num_tets", ctypes.c_int), ("num_tris", ctypes.c_int),
+                ("X", ctypes.c_void_p), ("V", ctypes.c_void_p), ("XTilde", ctypes.c_void_p),
+                ("Tet", ctypes.c_void_p), ("Tri", ctypes.c_void_p), ("TriFathers", ctypes.c_void_p),
+                ("mass", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("lambda_", ctypes.c_void_p),
+                ("DBC", ctypes.c_void_p)]
+
+
+class SbsIpcParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("damp", ctypes.c_double), ("dhat", ctypes.c_double),
+                ("tol", ctypes.c_double), ("max_iterations", ctypes.c_int), ("lin_tol", ctypes.c_double),
+                ("lin_max_iter", ctypes.c_int)]
+
+
+class SbsIpcStats(ctypes.Structure):
+    _fields_ = [("newton_iterations", ctypes.c_int), ("line_search_halvings", ctypes.c_int),
+                ("linear_iterations", ctypes.c_int), ("failed", ctypes.c_int), ("queries", ctypes.c_longlong),
+                ("kappa", ctypes.c_double), ("alpha0", ctypes.c_double), ("energy", ctypes.c_double)]
+
+
 # Every function the header declares, with its ctypes signature.
 _VP, _CP, _I, _D, _SZ = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
 SIGNATURES = {
@@ -91,6 +111,15 @@ SIGNATURES = {
     "sbs_collision_get_queries": (_I, [_VP, _VP, _VP, _VP]),
     "sbs_collision_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)]),
     "sbs_ccd_response": (_I, [_I, _I, _VP, _VP, _VP, _VP, _VP]),
+    "sbs_ipc_create": (_I, [ctypes.POINTER(SbsIpcData), ctypes.POINTER(_VP)]),
+    "sbs_ipc_destroy": (_I, [_VP]),
+    "sbs_ipc_update": (_I, [_VP, ctypes.POINTER(SbsIpcData), ctypes.POINTER(SbsIpcParams), ctypes.POINTER(_I)]),
+    "sbs_ipc_reset": (_I, [_VP]),
+    "sbs_ipc_set_linear_solver": (_I, [_VP, _I]),
+    "sbs_ipc_set_stream": (_I, [_VP, _VP]),
+    "sbs_ipc_stats": (_I, [_VP, ctypes.POINTER(SbsIpcStats)]),
+    "sbs_ipc_energy": (_I, [_VP, _VP, ctypes.POINTER(_D), _VP]),
+    "sbs_ipc_set_state": (_I, [_VP, _VP, _D]),
     "sbs_malloc": (_I, [ctypes.POINTER(_VP), _SZ]),
     "sbs_free": (_I, [_VP]),
     "sbs_memcpy_htod": (_I, [_VP, _VP, _SZ]),
