@@ -273,6 +273,62 @@ def run_batched_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def run_ipc_workload(args):
+    """The FP64 IPC step on two 16^3-hex cubes: the upper one falls 2 mm onto
+    the lower one, which is pinned on its bottom face (h = 1/100, dhat = 1e-3,
+    Newton tol 5e-2).  Latency-bound (a host-driven Newton loop with one
+    reduction per PCG iteration), so it reports time per step, not a roofline."""
+    from sbs_amd.ipc import IpcData, IpcParams, IpcSolver
+    scene = mesh.stack_scene(16, spacing=0.1, gap=2e-3, jitter=5e-4, seed=0)
+    print(f"# ipc workload: {scene.nv} vertices, {scene.nt} tets, {scene.ntri} surface triangles", file=sys.stderr,
+          flush=True)
+    p = IpcParams(dt=0.01, damp=0.999, dhat=1e-3, tol=5e-2, maxIterations=100, lin_tol=1e-10, lin_max_iter=5000)
+    data = IpcData(scene)
+    solver = IpcSolver(data)
+    steps = max(args.steps, 1)
+    newton, lin, queries, kappa = 0, 0, 0, 0.0
+    for _ in range(args.warmup):
+        assert solver.Update(data, p), solver.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        assert solver.Update(data, p), solver.stats()
+        st = solver.stats()
+        newton += st["newton_iterations"]
+        lin += st["linear_iterations"]
+        queries = max(queries, st["queries"])
+        kappa = max(kappa, st["kappa"])
+    el = time.perf_counter() - t0
+    solver.close()
+    data.free()
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        o = O.IpcOracle(scene.X, np.zeros_like(scene.X), scene.tet, scene.tri, scene.tri_fathers, scene.mass,
+                        scene.mu, scene.lam, scene.dbc > 0, dt=p.dt, damp=p.damp, dhat=p.dhat, tol=p.tol,
+                        lin_tol=p.lin_tol, max_iter=p.maxIterations)
+        n, c0 = 0, time.perf_counter()
+        while n < 3 and time.perf_counter() - c0 < args.cpu_budget:
+            o.step()
+            n += 1
+        cel = time.perf_counter() - c0
+        cpu = dict(value=n / cel, unit="steps/s", cores=1, kind="port",
+                   sample=f"the first {n} step(s) of the same scene, FP64, oracle C++ restatement (one thread), "
+                          f"{cel:.1f} s")
+        o.close()
+    line = {"metric": "IPC steps/sec, two 16^3-hex cubes in contact (FP64 Newton + barrier)",
+            "value": round(steps / el, 3), "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": round(1e3 * el / steps, 3), "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "2 x 16^3-hex cubes, IPC step (h 1/100, dhat 1e-3, tol 5e-2)",
+                       "verts": scene.nv, "tets": scene.nt, "tris": scene.ntri},
+            "ipc": {"newton_iterations_per_step": round(newton / steps, 2),
+                    "pcg_iterations_per_newton_solve": round(lin / max(newton, 1), 1),
+                    "max_queries": queries, "max_kappa": kappa},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,9 +339,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--timing-steps", type=int, default=3, help="extra steps timed kernel by kernel")
-    ap.add_argument("--workload", default="cube64", choices=["cube64", "dense256", "batched"],
+    ap.add_argument("--workload", default="cube64", choices=["cube64", "dense256", "batched", "ipc"],
                     help="cube64: the headline 64^3 cube; dense256: 256 small bodies on the dense solver; "
-                         "batched: 1024 uneven bodies in one engine vs per-body engines")
+                         "batched: 1024 uneven bodies in one engine vs per-body engines; "
+                         "ipc: the FP64 IPC step on two 16^3 cubes in contact")
     ap.add_argument("--per-body-sample", type=int, default=16, help="bodies timed as separate engines (batched)")
     ap.add_argument("--pcg-kernels", default="halo", choices=["halo", "gather"],
                     help="PCG kernel family: halo-staged (LDS) or per-row neighbour gathers")
@@ -299,6 +356,9 @@ def main():
         return
     if args.workload == "batched":
         run_batched_workload(args)
+        return
+    if args.workload == "ipc":
+        run_ipc_workload(args)
         return
 
     scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
