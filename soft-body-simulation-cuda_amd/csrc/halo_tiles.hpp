@@ -46,6 +46,7 @@ struct HaloTiles {
     std::vector<uint32_t> row_of;        // vertex -> internal row
     // per workgroup
     std::vector<int> g_row0, g_rows, g_halo_n, g_width, g_eptr;
+    std::vector<int> g_own0;             // halo-local index of the workgroup's first row (its rows follow it)
     // halo of workgroup g at [g * halo_stride, g * halo_stride + g_halo_n[g]),
     // ascending internal rows; unused positions repeat the workgroup's first row
     std::vector<uint32_t> halo;          // internal rows
@@ -159,6 +160,7 @@ HaloTiles build_halo_tiles(int nv, const std::vector<int>& rowstart, const std::
         auto local = [&](uint32_t i) {
             return (uint16_t)(std::lower_bound(set.begin(), set.end(), i) - set.begin());
         };
+        h.g_own0.push_back(local((uint32_t)r));  // rows r..r+n-1 are consecutive in the sorted halo
         const size_t base = h.col.size();
         if (base + (size_t)width * rows_per_group > (size_t)INT32_MAX)
             throw std::invalid_argument("halo tiles: too many entries");

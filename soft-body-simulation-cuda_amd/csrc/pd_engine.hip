@@ -613,7 +613,7 @@ __device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const
 // internal rows its rows touch (their own and their neighbours').
 constexpr int kHaloStride = 768;              // halo entries (LDS slots) per workgroup
 constexpr int kHaloPer = kHaloStride / kTpb;  // halo entries staged per thread
-constexpr int kGrpInts = 8;                   // per workgroup: row0, rows, halo size, width, first entry
+constexpr int kGrpInts = 8;                   // per workgroup: row0, rows, halo size, width, first entry, own0
 template <class T> struct HaloView {
     int groups = 0;
     const int* grp = nullptr;
@@ -650,6 +650,7 @@ template <class T> struct PcgArgs {
     double floor2;
     const int* skip;          // outer loop already converged
     HaloView<T> h;            // halo-staged kernels only: records and p are then in internal order
+    T* hrec[2];               // halo kernels: R', W', S' planes (9 N_v scalars), double-buffered
 };
 
 __device__ __forceinline__ double* iter_slot(double* slots, int i) {
@@ -943,14 +944,53 @@ __device__ __forceinline__ void halo_row(const HaloView<T>& h, int first, int wi
 }
 
 struct HaloGroup {
-    int row0, rows, hn, width, first;
+    int row0, rows, hn, width, first, own0;  // own0: halo-local column of row row0 (row row0+t is own0+t)
 };
 template <class T> __device__ __forceinline__ HaloGroup halo_group(const HaloView<T>& h, unsigned g) {
     const int* gd = h.grp + kGrpInts * (size_t)g;
-    return HaloGroup{gd[0], gd[1], gd[2], gd[3], gd[4]};
+    return HaloGroup{gd[0], gd[1], gd[2], gd[3], gd[4], gd[5]};
 }
 
-// init1: r0 = b - A x0 (x0 staged at the halo), x = x0, (D^-1 b)^2.
+// The halo kernels keep the PCG vectors that neighbours read scaled by D^-1,
+// in three planes per buffer (xyz interleaved, internal row order):
+//   R' = D^-1 r (= u), W' = D^-1 w, S' = D^-1 s,
+// so a halo row costs three 12-byte loads (36 bytes, no D^-1 entry) and its
+// u_{i+1} = R' - alpha (W' + beta S') needs no scaling.  The row's own
+// diagonal (for the dot products r.u = sum d R'^2, w.u = sum w R') comes from
+// its matrix entries.  The CG recurrences are linear, so scaling them row by
+// row gives the same iterates in exact arithmetic.
+template <class T> struct Planes {
+    T* r;
+    T* w;
+    T* s;
+};
+template <class T> __device__ __forceinline__ Planes<T> planes(T* base, int nv) {
+    return Planes<T>{base, base + 3 * (size_t)nv, base + 6 * (size_t)nv};
+}
+
+// halo_row, also summing the entries in the row's own column (its diagonal;
+// the padding entries there carry 0).
+template <class T>
+__device__ __forceinline__ void halo_row_diag(const HaloView<T>& h, int first, int width, int t, const RowBatch<T>& b0,
+                                              T (*lu)[kHaloStride], unsigned own, T& y0, T& y1, T& y2, T& d) {
+    auto apply = [&](const RowBatch<T>& b) {
+#pragma unroll
+        for (int j = 0; j < kEllBatch; ++j) {
+            y0 += b.m[j] * lu[0][b.c[j]];
+            y1 += b.m[j] * lu[1][b.c[j]];
+            y2 += b.m[j] * lu[2][b.c[j]];
+            d += b.c[j] == own ? b.m[j] : (T)0;
+        }
+    };
+    apply(b0);
+    for (int e0 = kEllBatch; e0 < width; e0 += kEllBatch) {
+        RowBatch<T> b;
+        row_batch(h, first, width, t, e0, b);
+        apply(b);
+    }
+}
+
+// init1: r0 = b - A x0 (x0 staged at the halo), R'0 = D^-1 r0, x = x0, (D^-1 b)^2.
 template <class T>
 __global__ void __launch_bounds__(kTpb) pcg_init1_halo_kernel(PcgArgs<T> a) {
     if (a.skip && *a.skip) return;
@@ -966,8 +1006,8 @@ __global__ void __launch_bounds__(kTpb) pcg_init1_halo_kernel(PcgArgs<T> a) {
     else ld3(a.b, v, b0, b1, b2);
     T ax0 = 0, ax1 = 0, ax2 = 0, x0 = 0, x1 = 0, x2 = 0;
     if (a.x0) {
-        RowBatch<T> b0;
-        row_batch(a.h, G.first, G.width, t, 0, b0);
+        RowBatch<T> bt;
+        row_batch(a.h, G.first, G.width, t, 0, bt);
         const uint32_t* __restrict__ hv = a.h.halo_v + (size_t)g * kHaloStride;
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) {
@@ -980,26 +1020,21 @@ __global__ void __launch_bounds__(kTpb) pcg_init1_halo_kernel(PcgArgs<T> a) {
         }
         ld3(a.x0, v, x0, x1, x2);
         __syncthreads();
-        halo_row(a.h, G.first, G.width, t, b0, lx, ax0, ax1, ax2);
+        halo_row(a.h, G.first, G.width, t, bt, lx, ax0, ax1, ax2);
     }
     double vals[3] = {0, 0, 0};
     if (row) {
         st3(a.h.x, ir, x0, x1, x2);
         const T di = a.dinv[v];
-        Rec<T> rc;
-        for (int k = 0; k < kRec; ++k) rc.v[k] = (T)0;
-        rc.v[kRecR] = b0 - ax0; rc.v[kRecR + 1] = b1 - ax1; rc.v[kRecR + 2] = b2 - ax2;
-        rc.v[kRecD] = di;
-        st_rec(a.rec[0], ir, rc);
+        st3(planes(a.hrec[0], a.nv).r, ir, di * (b0 - ax0), di * (b1 - ax1), di * (b2 - ax2));
         const double d0 = (double)di * b0, d1 = (double)di * b1, d2 = (double)di * b2;
         vals[0] = d0 * d0; vals[1] = d1 * d1; vals[2] = d2 * d2;
     }
     stripe_add<3>(vals, a.slots + kSlotInit * kSlotSize);
 }
 
-// init2: u0 = D^-1 r0 staged at the halo, w0 = A u0 into the record,
-// gamma0 = r0.u0, delta0 = w0.u0, u0.u0.  Other workgroups read only the
-// first 16 bytes (r, D^-1) of the records this launch writes w into.
+// init2: u0 = R'0 staged at the halo, w0 = A u0, W'0 = D^-1 w0;
+// gamma0 = r0.u0 = sum d R'0^2, delta0 = w0.u0, u0.u0.
 template <class T>
 __global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
     if (a.skip && *a.skip) return;
@@ -1009,7 +1044,7 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
     const int t = threadIdx.x;
     const bool row = t < G.rows;
     const size_t ir = (size_t)G.row0 + (row ? t : 0);
-    T* rec = a.rec[0];
+    const Planes<T> P = planes(a.hrec[0], a.nv);
     RowBatch<T> b0;
     row_batch(a.h, G.first, G.width, t, 0, b0);
     const uint32_t* __restrict__ hh = a.h.halo + (size_t)g * kHaloStride;
@@ -1017,21 +1052,21 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
     for (int k = 0; k < kHaloPer; ++k) {
         const int e = t + k * kTpb;
         if (e < G.hn) {
-            T q0, q1, q2, dq;
-            ld_rec_rd(rec, hh[e], q0, q1, q2, dq);
-            lu[0][e] = dq * q0; lu[1][e] = dq * q1; lu[2][e] = dq * q2;
+            T q0, q1, q2;
+            ld3(P.r, hh[e], q0, q1, q2);
+            lu[0][e] = q0; lu[1][e] = q1; lu[2][e] = q2;
         }
     }
-    T r0, r1, r2, dv;
-    ld_rec_rd(rec, ir, r0, r1, r2, dv);
+    T u0, u1, u2;
+    ld3(P.r, ir, u0, u1, u2);
     __syncthreads();
-    T w0 = 0, w1 = 0, w2 = 0;
-    halo_row(a.h, G.first, G.width, t, b0, lu, w0, w1, w2);
+    T w0 = 0, w1 = 0, w2 = 0, d = 0;
+    halo_row_diag(a.h, G.first, G.width, t, b0, lu, (unsigned)(G.own0 + t), w0, w1, w2, d);
     double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (row) {
-        const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
-        st_rec_w(rec, ir, w0, w1, w2);
-        vals[0] = (double)r0 * u0; vals[1] = (double)r1 * u1; vals[2] = (double)r2 * u2;
+        const T di = (T)1 / d;
+        st3(P.w, ir, di * w0, di * w1, di * w2);
+        vals[0] = (double)d * u0 * u0; vals[1] = (double)d * u1 * u1; vals[2] = (double)d * u2 * u2;
         vals[3] = (double)w0 * u0; vals[4] = (double)w1 * u1; vals[5] = (double)w2 * u2;
         vals[6] = (double)u0 * u0; vals[7] = (double)u1 * u1; vals[8] = (double)u2 * u2;
     }
@@ -1040,7 +1075,7 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
 
 // One PCG iteration K(i), halo-staged.  The halo's row indices are requested
 // with the reduction stripes of the previous launch; once the scalars are
-// formed the halo records, the row's own record, p and x are requested
+// formed the halo planes, the row's own planes, p and x are requested
 // together, so the launch costs about two dependent memory round trips.  The
 // iteration that finds the solve converged returns after the scalars, before
 // any of those loads (the decision is the same in every wavefront).
@@ -1059,20 +1094,27 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     const bool row = t < G.rows;
     const size_t ir = (size_t)G.row0 + (row ? t : 0);
     const int rd = i & 1, wr = rd ^ 1;
-    const T* __restrict__ rec_in = a.rec[rd];
+    const Planes<T> Pi = planes(a.hrec[rd], a.nv), Po = planes(a.hrec[wr], a.nv);
     const uint32_t* __restrict__ hh = a.h.halo + (size_t)g * kHaloStride;
     uint32_t hidx[kHaloPer];
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) hidx[k] = hh[t + k * kTpb];  // in bounds: kHaloStride per workgroup
     const PcgState st = pcg_state(a, i);
     if (st.done) return;
-    Rec<T> hr[kHaloPer];
+    T hq[kHaloPer][9];  // R', W', S' of the staged halo rows
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) {
         const int e = t + k * kTpb;
-        if (e < G.hn) hr[k] = ld_rec(rec_in, hidx[k]);
+        if (e < G.hn) {
+            ld3(Pi.r, hidx[k], hq[k][0], hq[k][1], hq[k][2]);
+            ld3(Pi.w, hidx[k], hq[k][3], hq[k][4], hq[k][5]);
+            if (!kFirst) ld3(Pi.s, hidx[k], hq[k][6], hq[k][7], hq[k][8]);
+        }
     }
-    const Rec<T> own = ld_rec(rec_in, ir);
+    T r0, r1, r2, w0, w1, w2, so0 = 0, so1 = 0, so2 = 0;
+    ld3(Pi.r, ir, r0, r1, r2);
+    ld3(Pi.w, ir, w0, w1, w2);
+    if (!kFirst) ld3(Pi.s, ir, so0, so1, so2);
     T x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
     if (!kFirst) ld3(a.p, ir, p0, p1, p2);
     ld3(a.h.x, ir, x0, x1, x2);
@@ -1080,21 +1122,18 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     row_batch(a.h, G.first, G.width, t, 0, b0);
     const T al0 = (T)st.alpha[0], al1 = (T)st.alpha[1], al2 = (T)st.alpha[2];
     const T be0 = (T)st.beta[0], be1 = (T)st.beta[1], be2 = (T)st.beta[2];
-    // u_{i+1} of the halo rows: s_i = w_i + beta s_{i-1}, r_{i+1} = r_i - alpha s_i
+    // u_{i+1} of the halo rows: S'_i = W'_i + beta S'_{i-1}, R'_{i+1} = R'_i - alpha S'_i
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) {
         const int e = t + k * kTpb;
         if (e < G.hn) {
-            const Rec<T>& q = hr[k];
-            const T dq = q.v[kRecD];
-            T sq0 = q.v[kRecW], sq1 = q.v[kRecW + 1], sq2 = q.v[kRecW + 2];
+            T sq0 = hq[k][3], sq1 = hq[k][4], sq2 = hq[k][5];
             if (!kFirst) {
-                sq0 = fma(be0, q.v[kRecS], sq0); sq1 = fma(be1, q.v[kRecS + 1], sq1);
-                sq2 = fma(be2, q.v[kRecS + 2], sq2);
+                sq0 = fma(be0, hq[k][6], sq0); sq1 = fma(be1, hq[k][7], sq1); sq2 = fma(be2, hq[k][8], sq2);
             }
-            lu[0][e] = dq * fma(-al0, sq0, q.v[kRecR]);
-            lu[1][e] = dq * fma(-al1, sq1, q.v[kRecR + 1]);
-            lu[2][e] = dq * fma(-al2, sq2, q.v[kRecR + 2]);
+            lu[0][e] = fma(-al0, sq0, hq[k][0]);
+            lu[1][e] = fma(-al1, sq1, hq[k][1]);
+            lu[2][e] = fma(-al2, sq2, hq[k][2]);
         }
     }
     if (blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
@@ -1102,32 +1141,28 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
         for (int k = threadIdx.x; k < kSlotSize; k += kTpb) z[k] = 0.0;
     }
     __syncthreads();
-    T y0 = 0, y1 = 0, y2 = 0;  // w_{i+1} = A u_{i+1}
-    halo_row(a.h, G.first, G.width, t, b0, lu, y0, y1, y2);
+    T y0 = 0, y1 = 0, y2 = 0, d = 0;  // w_{i+1} = A u_{i+1}
+    halo_row_diag(a.h, G.first, G.width, t, b0, lu, (unsigned)(G.own0 + t), y0, y1, y2, d);
     double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (row) {
-        const T r0 = own.v[kRecR], r1 = own.v[kRecR + 1], r2 = own.v[kRecR + 2], dv = own.v[kRecD];
-        const T w0 = own.v[kRecW], w1 = own.v[kRecW + 1], w2 = own.v[kRecW + 2];
-        const T u0 = dv * r0, u1 = dv * r1, u2 = dv * r2;
+        // p_i = u_i + beta p_{i-1} (u_i = R'_i), S'_i, x_{i+1}, R'_{i+1} = u_{i+1}
         T s0, s1, s2;
         if (!kFirst) {
-            p0 = fma(be0, p0, u0); p1 = fma(be1, p1, u1); p2 = fma(be2, p2, u2);
-            s0 = fma(be0, own.v[kRecS], w0); s1 = fma(be1, own.v[kRecS + 1], w1); s2 = fma(be2, own.v[kRecS + 2], w2);
+            p0 = fma(be0, p0, r0); p1 = fma(be1, p1, r1); p2 = fma(be2, p2, r2);
+            s0 = fma(be0, so0, w0); s1 = fma(be1, so1, w1); s2 = fma(be2, so2, w2);
         } else {
-            p0 = u0; p1 = u1; p2 = u2;
+            p0 = r0; p1 = r1; p2 = r2;
             s0 = w0; s1 = w1; s2 = w2;
         }
         x0 = fma(al0, p0, x0); x1 = fma(al1, p1, x1); x2 = fma(al2, p2, x2);
-        const T rn0 = fma(-al0, s0, r0), rn1 = fma(-al1, s1, r1), rn2 = fma(-al2, s2, r2);
-        const T un0 = dv * rn0, un1 = dv * rn1, un2 = dv * rn2;
+        const T un0 = fma(-al0, s0, r0), un1 = fma(-al1, s1, r1), un2 = fma(-al2, s2, r2);
+        const T di = (T)1 / d;
         st3(a.p, ir, p0, p1, p2);
         st3(a.h.x, ir, x0, x1, x2);
-        Rec<T> out;
-        out.v[kRecR] = rn0; out.v[kRecR + 1] = rn1; out.v[kRecR + 2] = rn2; out.v[kRecD] = dv;
-        out.v[kRecW] = y0; out.v[kRecW + 1] = y1; out.v[kRecW + 2] = y2; out.v[kRecW + 3] = (T)0;
-        out.v[kRecS] = s0; out.v[kRecS + 1] = s1; out.v[kRecS + 2] = s2; out.v[kRecS + 3] = (T)0;
-        st_rec(a.rec[wr], ir, out);
-        vals[0] = (double)rn0 * un0; vals[1] = (double)rn1 * un1; vals[2] = (double)rn2 * un2;
+        st3(Po.r, ir, un0, un1, un2);
+        st3(Po.w, ir, di * y0, di * y1, di * y2);
+        st3(Po.s, ir, s0, s1, s2);
+        vals[0] = (double)d * un0 * un0; vals[1] = (double)d * un1 * un1; vals[2] = (double)d * un2 * un2;
         vals[3] = (double)y0 * un0; vals[4] = (double)y1 * un1; vals[5] = (double)y2 * un2;
         vals[6] = (double)un0 * un0; vals[7] = (double)un1 * un1; vals[8] = (double)un2 * un2;
     }
@@ -1448,6 +1483,7 @@ void PdEngine<T>::build_halo(const std::vector<int>& rowstart, const std::vector
     for (int g = 0; g < ng; ++g) {
         int* d = &grp[(size_t)g * kGrpInts];
         d[0] = h.g_row0[g]; d[1] = h.g_rows[g]; d[2] = h.g_halo_n[g]; d[3] = h.g_width[g]; d[4] = h.g_eptr[g];
+        d[5] = h.g_own0[g];
     }
     std::vector<int> src(h.src.begin(), h.src.end());
     h_grp_.alloc(grp.size());
@@ -1458,6 +1494,7 @@ void PdEngine<T>::build_halo(const std::vector<int>& rowstart, const std::vector
     h_src_.alloc(std::max<size_t>(1, src.size()));
     h_val_.alloc(std::max<size_t>(1, src.size()));
     cg_x_.alloc(3 * (size_t)nv_);
+    for (int k = 0; k < 2; ++k) cg_plane_[k].alloc(9 * (size_t)nv_);
     SBS_HIP(hipMemcpy(h_grp_.get(), grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(h_halo_.get(), h.halo.data(), h.halo.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(h_halo_v_.get(), h.halo_vertex.data(), h.halo_vertex.size() * sizeof(uint32_t),
@@ -1629,6 +1666,7 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
         a.h.val = h_val_.get();
         a.h.perm = h_perm_.get();
         a.h.x = cg_x_.get();
+        for (int k = 0; k < 2; ++k) a.hrec[k] = cg_plane_[k].get();
     }
     const dim3 grid(ceil_div(nv_, kTpb)), block(kTpb), hgrid(halo ? halo_groups_ : 1);
     const int max_iter = params.pcg_max_iter;

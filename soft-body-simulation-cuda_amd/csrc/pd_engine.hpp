@@ -222,6 +222,7 @@ private:
     DevBuf<uint32_t> h_halo_, h_halo_v_, h_perm_;
     DevBuf<uint16_t> h_col_;
     DevBuf<T> h_val_, cg_x_;
+    DevBuf<T> cg_plane_[2];             // halo PCG: D^-1-scaled r, w, s planes, double-buffered
     DevBuf<double> slots_;              // address-striped reduction slots
     DevBuf<unsigned char> ctl_;         // device control block
     void* host_ctl_ = nullptr;          // pinned copy of the control block (host checks)

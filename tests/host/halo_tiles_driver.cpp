@@ -44,6 +44,7 @@ int main() {
         put("g_halo_n", h.g_halo_n);
         put("g_width", h.g_width);
         put("g_eptr", h.g_eptr);
+        put("g_own0", h.g_own0);
         put("halo", h.halo);
         put("halo_vertex", h.halo_vertex);
         put("col", h.col);

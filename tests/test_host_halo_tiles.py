@@ -93,6 +93,8 @@ def check_tiles(h, rowstart, cols, X, rpg, stride):
         nb = np.unique(np.concatenate([r] + [row_of[cols[rowstart[v]:rowstart[v + 1]]] for v in verts]))
         assert np.array_equal(halo[g, :hn[g]], nb), g                      # exactly rows + neighbours, ascending
         assert (halo[g, hn[g]:] == row0[g]).all()
+        own = h["g_own0"][g] + np.arange(rows[g])                        # the rows' own halo-local columns
+        assert np.array_equal(halo[g, own], r)
         assert width[g] == rl[verts].max()
         base = eptr[g]
         for t in range(rpg):
