@@ -81,6 +81,7 @@ typedef struct SbsSolverData {
     const uint32_t* TriFathers; /* num_tris: body of each triangle (dev_TriFathers) */
     void* tIs;                  /* num_verts: 1, or 0.5 on hit vertices (dev_tIs) */
     float* Normals;             /* 3*num_verts: contact normals (dev_Normals)   */
+    struct SbsFixed* fixed_bodies; /* NULL or a fixed-body set (SolverData::pFixedBodies) */
 } SbsSolverData;
 
 /* Mirrors SolverParams<Scalar> (src/def.h:83-100) for the PD step. */
@@ -100,6 +101,8 @@ typedef struct SbsSolverParams {
     double dense_tol;     /* dense solver: max|D^-1 r| <= tol max|D^-1 b|      */
     int dense_max_iter;   /* dense solver: FP64 refinement corrections per solve */
     int pcg_halo;         /* PCG kernels: 1 halo-staged (LDS), 0 per-row gathers */
+    double mu_n;          /* fixed-body response: normal restitution (muN)  */
+    double mu_t;          /* fixed-body response: friction (muT)            */
 } SbsSolverParams;
 
 typedef struct SbsPd SbsPd;
@@ -202,6 +205,43 @@ int sbs_collision_stats(SbsCollision* c, long long* candidate_pairs, int* hits);
  * V = -((XTilde - X).n) n and keep X; the others take X = XTilde. */
 int sbs_ccd_response(int precision, int num_verts, void* d_X, const void* d_XTilde, void* d_V, const void* d_tI,
                      const float* d_normals);
+
+/* Fixed (kinematic) bodies: FixedBody (src/collision/rigid/rigid.h:14-28) as
+ * Sphere (sphere.h), Plane (plane.h) or Cylinder (cylinder.h), held as
+ * FixedBodyData (src/simulation/fixedBodyData.h:10-30):
+ *   FixedBodyData(int threadsPerBlock, const std::vector<FixedBody*>&)
+ *        fixedBodyData.h:15, fixedBodyData.cu:8-61    -> sbs_fixed_create
+ *   FixedBodyData::HandleCollisions(X, V, numVerts, muT, muN)
+ *        fixedBodyData.h:21, fixedBodyData.cu:136-148 -> sbs_fixed_handle_collisions
+ * A body is its type (BodyType order, rigid.h:8-12), its model matrix
+ * (FixedBody::m_model, glm column-major: model[4*col + row]) and its radius
+ * (Sphere::m_radius; Cylinder::m_radius, the x scale; unused by planes).
+ * Derived in float at creation: the plane normal m_floorUp (row 1 of the
+ * inverse of the model's 3x3 part, normalised; plane.cpp:9) and the cylinder
+ * axis normalize(model * (0,1,0,0)) (fixedBodyData.cu:121).  The response
+ * visits spheres, planes, then cylinders, each in creation order
+ * (fixedBodyData.cu:139-147).  Attach a set to a PD solve through
+ * SbsSolverData.fixed_bodies (SolverData::pFixedBodies, def.h:43): the
+ * update then applies it to XTilde and V after X is taken (pdSolver.cu:228-231),
+ * with SbsSolverParams.mu_t / mu_n. */
+#define SBS_FIXED_SPHERE 0
+#define SBS_FIXED_PLANE 1
+#define SBS_FIXED_CYLINDER 2
+typedef struct SbsFixedBody {
+    int type;          /* SBS_FIXED_*                                        */
+    float model[16];   /* model matrix, column-major                         */
+    float radius;      /* sphere / cylinder radius                           */
+} SbsFixedBody;
+typedef struct SbsFixed SbsFixed;
+int sbs_fixed_create(const SbsFixedBody* bodies, int num_bodies, SbsFixed** out);
+int sbs_fixed_destroy(SbsFixed* f);
+/* X, V: 3*num_verts device scalars of the precision (32 or 64), updated in
+ * place on the stream (NULL: the null stream). */
+int sbs_fixed_handle_collisions(SbsFixed* f, int precision, void* d_X, void* d_V, int num_verts, double mu_t,
+                                double mu_n, void* hip_stream);
+/* Derived parameters, 8 floats per body in creation order: p[3] (translation),
+ * u[3] (plane normal / cylinder axis, else 0), radius, 0. */
+int sbs_fixed_params(SbsFixed* f, float* out);
 
 /* ---------------------------------------------------------------- IPC --
  * The FP64 incremental-potential-contact solver of the reference:

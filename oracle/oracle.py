@@ -90,6 +90,11 @@ def lib(bench: bool = False) -> ctypes.CDLL:
     L.orc_col_min_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, up, up, vp, vp, ctypes.c_int]
     L.orc_col_proximity.restype = ctypes.c_longlong
     L.orc_col_proximity.argtypes = [ctypes.c_int, up, up, dp, ctypes.c_double, ctypes.c_int, up]
+    if hasattr(L, "orc_fixed_params"):
+        L.orc_fixed_params.restype = ctypes.c_int
+        L.orc_fixed_params.argtypes = [ctypes.c_int, ip, fp, fp, fp]
+        L.orc_fixed_handle.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ip, fp, ctypes.c_double, ctypes.c_double]
     if hasattr(L, "orc_ipc_create"):
         ucp = ctypes.POINTER(ctypes.c_ubyte)
         L.orc_ipc_distance_type.restype = ctypes.c_int
@@ -275,6 +280,32 @@ def ccd_response(X, XT, V, tI, vertex_normals, fp64=False):
     nrm = np.ascontiguousarray(vertex_normals, dtype=np.float32).reshape(-1)
     lib().orc_ccd_response(64 if fp64 else 32, X.size // 3, X.ctypes.data, XT.ctypes.data, V.ctypes.data,
                            tI.ctypes.data, _p(nrm, ctypes.c_float))
+    return X.reshape(-1, 3), V.reshape(-1, 3)
+
+
+# -------------------------------------------------------- fixed bodies --
+def fixed_params(types, models_colmajor, radii):
+    """Body parameters (8 floats each: translation, plane normal / cylinder
+    axis, radius, 0) from column-major float model matrices."""
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    m = np.ascontiguousarray(models_colmajor, dtype=np.float32).reshape(-1)
+    r = np.ascontiguousarray(radii, dtype=np.float32)
+    out = np.zeros((len(t), 8), np.float32)
+    rc = lib().orc_fixed_params(len(t), _p(t, ctypes.c_int), _p(m, ctypes.c_float), _p(r, ctypes.c_float),
+                                _p(out, ctypes.c_float))
+    if rc != 0:
+        raise ValueError("fixed body: unknown type or singular plane matrix")
+    return out
+
+
+def fixed_handle(X, V, types, params, mu_t, mu_n, fp64=False):
+    """HandleCollisions on the CPU; returns the new X and V."""
+    X = _vec(X, fp64).copy()
+    V = _vec(V, fp64).copy()
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    prm = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+    lib().orc_fixed_handle(64 if fp64 else 32, X.size // 3, X.ctypes.data, V.ctypes.data, len(t),
+                           _p(t, ctypes.c_int), _p(prm, ctypes.c_float), float(mu_t), float(mu_n))
     return X.reshape(-1, 3), V.reshape(-1, 3)
 
 

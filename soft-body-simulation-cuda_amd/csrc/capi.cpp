@@ -1,10 +1,12 @@
 // extern "C" entry points declared in include/sbs/sbs.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/sbs/sbs.h"
 #include "collision.hpp"
@@ -31,6 +33,10 @@ struct SbsPd {
 
 struct SbsIpc {
     std::unique_ptr<sbs::IpcEngine> e;
+};
+
+struct SbsFixed {
+    std::unique_ptr<sbs::FixedBodies> f;
 };
 
 struct SbsCollision {
@@ -79,6 +85,7 @@ template <class T> sbs::PdData<T> to_data(const SbsSolverData* d) {
     o.TriFathers = d->TriFathers;
     o.tIs = static_cast<T*>(d->tIs);
     o.Normals = d->Normals;
+    o.fixed = d->fixed_bodies ? d->fixed_bodies->f.get() : nullptr;
     return o;
 }
 
@@ -99,6 +106,8 @@ sbs::PdParams to_params(const SbsSolverParams* p) {
     o.dense_tol = p->dense_tol;
     o.dense_max_iter = p->dense_max_iter;
     o.pcg_halo = p->pcg_halo;
+    o.mu_n = p->mu_n;
+    o.mu_t = p->mu_t;
     return o;
 }
 
@@ -407,6 +416,48 @@ int sbs_device_count(int* count) {
         SBS_REQUIRE(count != nullptr, "null count");
         *count = 0;
         SBS_HIP(hipGetDeviceCount(count));
+    });
+}
+
+int sbs_fixed_create(const SbsFixedBody* bodies, int num_bodies, SbsFixed** out) {
+    return guarded([&] {
+        SBS_REQUIRE(out != nullptr, "sbs_fixed_create: null argument");
+        SBS_REQUIRE(num_bodies >= 0 && (num_bodies == 0 || bodies != nullptr), "sbs_fixed_create: bad body list");
+        *out = nullptr;
+        std::vector<sbs::FixedBodySpec> specs(num_bodies);
+        for (int k = 0; k < num_bodies; ++k) {
+            specs[k].type = bodies[k].type;
+            std::memcpy(specs[k].model, bodies[k].model, sizeof(specs[k].model));
+            specs[k].radius = bodies[k].radius;
+        }
+        auto h = std::make_unique<SbsFixed>();
+        h->f = std::make_unique<sbs::FixedBodies>(specs);
+        *out = h.release();
+    });
+}
+
+int sbs_fixed_destroy(SbsFixed* f) {
+    return guarded([&] { delete f; });
+}
+
+int sbs_fixed_handle_collisions(SbsFixed* f, int precision, void* d_X, void* d_V, int num_verts, double mu_t,
+                                double mu_n, void* hip_stream) {
+    return guarded([&] {
+        SBS_REQUIRE(f != nullptr, "null SbsFixed handle");
+        SBS_REQUIRE(precision == 32 || precision == 64, "sbs_fixed_handle_collisions: precision must be 32 or 64");
+        const hipStream_t s = static_cast<hipStream_t>(hip_stream);
+        if (precision == 64)
+            f->f->handle(static_cast<double*>(d_X), static_cast<double*>(d_V), num_verts, mu_t, mu_n, s);
+        else
+            f->f->handle(static_cast<float*>(d_X), static_cast<float*>(d_V), num_verts, (float)mu_t, (float)mu_n, s);
+    });
+}
+
+int sbs_fixed_params(SbsFixed* f, float* out) {
+    return guarded([&] {
+        SBS_REQUIRE(f != nullptr && out != nullptr, "sbs_fixed_params: null argument");
+        const auto& p = f->f->params();
+        std::copy(p.begin(), p.end(), out);
     });
 }
 

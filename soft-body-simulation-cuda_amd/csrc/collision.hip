@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstring>
 
@@ -551,7 +552,134 @@ __global__ void ccd_response_kernel(int nv, T* X, const T* XT, T* V, const T* tI
 
 inline unsigned blocks_for(unsigned long long n) { return (unsigned)((n + kTpbC - 1) / kTpbC); }
 
+// --------------------------------------------------------- fixed bodies --
+// The response shared by the three body types (fixedBodyData.cu:80-85,
+// 101-106, 126-131): normal velocity reflected and scaled by muN, tangential
+// velocity shortened by muT (1 + muN) |v_N| (Coulomb-like), clamped at 0.
+template <class T> __device__ __forceinline__ V3<T> fixed_response(V3<T> vel, V3<T> n, T mu_t, T mu_n) {
+    const V3<T> vn = dot(vel, n) * n;
+    const V3<T> vt = vel - vn;
+    const T mag = length(vt);
+    const T t = (T)1 - mu_t * ((T)1 + mu_n) * length(vn) / mag;
+    const T a = mag == (T)0 ? (T)0 : (t < (T)0 ? (T)0 : t);  // glm::max(t, 0)
+    return (-mu_n) * vn + a * vt;
+}
+
+// HandleCollisions (fixedBodyData.cu:136-148): one thread per vertex visits
+// the spheres, then the planes, then the cylinders, in the order of the
+// reference's three kernels, each body seeing the position and velocity left
+// by the previous one:
+//   sphere (:91-109):   d = |x - c| < r  -> x += (r - d) n, n = (x - c)/d;
+//   plane (:68-88):     d = (x - p).u < 0 and v.u < 0 -> x -= d u;
+//   cylinder (:112-134): n = (I - a a^T)(x - c), d = |n| < r -> x += (r - d) n/|n|
+// (an infinite cylinder: the reference has no height test), each followed by
+// the response above.  Body parameters are float and widened to T.
+template <class T>
+__global__ void fixed_bodies_kernel(int nv, T* X, T* V, const float* __restrict__ body, int ns, int np, int nc,
+                                    T mu_t, T mu_n) {
+    const int v = blockIdx.x * kTpbC + threadIdx.x;
+    if (v >= nv) return;
+    V3<T> x = ldv(X, (uint32_t)v), vel = ldv(V, (uint32_t)v);
+    auto p_of = [&](int j) {
+        const float* b = body + kFixedFloats * (size_t)j;
+        return V3<T>{(T)b[0], (T)b[1], (T)b[2]};
+    };
+    auto u_of = [&](int j) {
+        const float* b = body + kFixedFloats * (size_t)j;
+        return V3<T>{(T)b[3], (T)b[4], (T)b[5]};
+    };
+    for (int j = 0; j < ns; ++j) {
+        const V3<T> c = p_of(j);
+        const T r = (T)body[kFixedFloats * (size_t)j + 6];
+        const V3<T> tc = x - c;
+        const T d = length(tc);
+        if (d < r) {
+            const V3<T> n = normalize(tc);
+            x = x + (r - d) * n;
+            vel = fixed_response(vel, n, mu_t, mu_n);
+        }
+    }
+    for (int j = ns; j < ns + np; ++j) {
+        const V3<T> p = p_of(j), u = u_of(j);
+        const T d = dot(x - p, u);
+        if (d < (T)0 && dot(vel, u) < (T)0) {
+            x = x - d * u;
+            vel = fixed_response(vel, u, mu_t, mu_n);
+        }
+    }
+    for (int j = ns + np; j < ns + np + nc; ++j) {
+        const V3<T> c = p_of(j), a = u_of(j);
+        const T r = (T)body[kFixedFloats * (size_t)j + 6];
+        // nnT = I - outer(a, a), entries 1 - a_i a_j on the diagonal, 0 - a_i a_j off it
+        const T m00 = (T)1 - a.x * a.x, m01 = (T)0 - a.x * a.y, m02 = (T)0 - a.x * a.z;
+        const T m11 = (T)1 - a.y * a.y, m12 = (T)0 - a.y * a.z, m22 = (T)1 - a.z * a.z;
+        const V3<T> w = x - c;
+        const V3<T> n = {m00 * w.x + m01 * w.y + m02 * w.z, m01 * w.x + m11 * w.y + m12 * w.z,
+                         m02 * w.x + m12 * w.y + m22 * w.z};
+        const T d = length(n);
+        if (d < r) {
+            const V3<T> nn = normalize(n);
+            x = x + (r - d) * nn;
+            vel = fixed_response(vel, nn, mu_t, mu_n);
+        }
+    }
+    X[3 * (size_t)v] = x.x;
+    X[3 * (size_t)v + 1] = x.y;
+    X[3 * (size_t)v + 2] = x.z;
+    V[3 * (size_t)v] = vel.x;
+    V[3 * (size_t)v + 1] = vel.y;
+    V[3 * (size_t)v + 2] = vel.z;
+}
+
 }  // namespace
+
+FixedBodies::FixedBodies(const std::vector<FixedBodySpec>& bodies) {
+    std::vector<float> grouped;
+    params_.assign(kFixedFloats * bodies.size(), 0.f);
+    for (int type : {kFixedSphere, kFixedPlane, kFixedCylinder}) {
+        for (size_t k = 0; k < bodies.size(); ++k) {
+            const FixedBodySpec& b = bodies[k];
+            SBS_REQUIRE(b.type >= kFixedSphere && b.type <= kFixedCylinder, "fixed body: unknown type");
+            if (b.type != type) continue;
+            const float* m = b.model;  // m[4 * col + row]
+            float q[kFixedFloats] = {m[12], m[13], m[14], 0.f, 0.f, 0.f, b.radius, 0.f};
+            if (type == kFixedPlane) {
+                // row 1 of A^-1 = row 1 of adj(A) / det(A), A the model's 3x3 part
+                auto A = [&](int r, int c) { return m[4 * c + r]; };
+                const float det = A(0, 0) * (A(1, 1) * A(2, 2) - A(2, 1) * A(1, 2)) -
+                                  A(0, 1) * (A(1, 0) * A(2, 2) - A(2, 0) * A(1, 2)) +
+                                  A(0, 2) * (A(1, 0) * A(2, 1) - A(2, 0) * A(1, 1));
+                SBS_REQUIRE(det != 0.f, "fixed body: singular plane model matrix");
+                const float w[3] = {-(A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0)) / det,
+                                    (A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0)) / det,
+                                    -(A(0, 0) * A(1, 2) - A(0, 2) * A(1, 0)) / det};
+                const float s = 1.f / std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+                q[3] = w[0] * s; q[4] = w[1] * s; q[5] = w[2] * s;
+            } else if (type == kFixedCylinder) {
+                const float s = 1.f / std::sqrt(m[4] * m[4] + m[5] * m[5] + m[6] * m[6] + m[7] * m[7]);
+                q[3] = m[4] * s; q[4] = m[5] * s; q[5] = m[6] * s;
+            }
+            for (int i = 0; i < kFixedFloats; ++i) {
+                SBS_REQUIRE(std::isfinite(q[i]), "fixed body: non-finite parameter");
+                grouped.push_back(q[i]);
+                params_[kFixedFloats * k + i] = q[i];
+            }
+            (type == kFixedSphere ? ns_ : type == kFixedPlane ? np_ : nc_)++;
+        }
+    }
+    if (!grouped.empty()) {
+        dev_.alloc(grouped.size());
+        SBS_HIP(hipMemcpy(dev_.get(), grouped.data(), grouped.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+}
+
+template <class T> void FixedBodies::handle(T* X, T* V, int nv, T mu_t, T mu_n, hipStream_t stream) const {
+    SBS_REQUIRE(nv >= 0 && (nv == 0 || (X && V)), "HandleCollisions: null arrays");
+    if (nv == 0 || ns_ + np_ + nc_ == 0) return;
+    hipLaunchKernelGGL(fixed_bodies_kernel<T>, dim3(blocks_for((unsigned long long)nv)), dim3(kTpbC), 0, stream, nv,
+                       X, V, (const float*)dev_.get(), ns_, np_, nc_, mu_t, mu_n);
+    SBS_HIP(hipGetLastError());
+}
 
 template <class T>
 CollisionDetector<T>::CollisionDetector(int num_verts, int num_tris, const uint32_t* tri, const uint32_t* fathers,
@@ -751,5 +879,7 @@ template class CollisionDetector<float>;
 template class CollisionDetector<double>;
 template void ccd_response<float>(int, float*, const float*, float*, const float*, const float*, hipStream_t);
 template void ccd_response<double>(int, double*, const double*, double*, const double*, const float*, hipStream_t);
+template void FixedBodies::handle<float>(float*, float*, int, float, float, hipStream_t) const;
+template void FixedBodies::handle<double>(double*, double*, int, double, double, hipStream_t) const;
 
 }  // namespace sbs

@@ -87,6 +87,43 @@ private:
     unsigned long long* host_ = nullptr;  // pinned scratch for counts
 };
 
+// Fixed (kinematic) bodies: FixedBody and its subclasses Sphere, Plane and
+// Cylinder (src/collision/rigid/rigid.h:8-28, sphere.h, plane.h, cylinder.h)
+// as FixedBodyData holds them (src/simulation/fixedBodyData.cu:8-61), with
+// HandleCollisions (fixedBodyData.cu:68-148).
+enum FixedType : int { kFixedSphere = 0, kFixedPlane = 1, kFixedCylinder = 2 };  // BodyType order
+struct FixedBodySpec {
+    int type = kFixedPlane;
+    float model[16] = {};  // FixedBody::m_model, glm column-major: model[4 * col + row]
+    float radius = 0.f;    // Sphere::m_radius; Cylinder::m_radius (= scale[0]); unused for planes
+};
+constexpr int kFixedFloats = 8;  // per body: p[3], u[3], r, 0
+
+class FixedBodies {
+public:
+    // Derives each body's parameters on the host, in float as the reference
+    // holds them: p = the model's translation (the sphere/cylinder centre, a
+    // point of the plane); plane: u = m_floorUp = normalize(row 1 of the
+    // inverse of the model's 3x3 part) (plane.cpp:9, the y column of the
+    // inverse transpose); cylinder: u = normalize(model * (0,1,0,0))
+    // (fixedBodyData.cu:121); r = the radius.  Bodies are kept grouped by
+    // type in the order HandleCollisions visits them: spheres, planes,
+    // cylinders, each group in the caller's order.
+    explicit FixedBodies(const std::vector<FixedBodySpec>& bodies);
+    // HandleCollisions on positions X and velocities V (3*nv each, device).
+    template <class T> void handle(T* X, T* V, int nv, T mu_t, T mu_n, hipStream_t stream) const;
+    int spheres() const { return ns_; }
+    int planes() const { return np_; }
+    int cylinders() const { return nc_; }
+    // kFixedFloats per body, in the caller's order (tests).
+    const std::vector<float>& params() const { return params_; }
+
+private:
+    int ns_ = 0, np_ = 0, nc_ = 0;
+    std::vector<float> params_;
+    DevBuf<float> dev_;  // kFixedFloats per body, grouped by type
+};
+
 // CCDKernel (collisionUtil.cu:50-70): hit vertices (tI < 1) get
 // V = -(d.n) n with d = XTilde - X and keep X; the others take X = XTilde.
 template <class T>

@@ -1844,8 +1844,29 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
         if (!collision_)
             collision_ = std::make_unique<CollisionDetector<T>>(nv_, data.num_tris, data.Tri, data.TriFathers, stream_);
         collision_->set_stream(stream_);
+        if (perf_) SBS_HIP(hipEventRecord(ev[0], stream_));
         collision_->detect(data.X, data.XTilde, true, data.tIs, data.Normals);
         ccd_response<T>(nv_, data.X, data.XTilde, data.V, data.tIs, data.Normals, stream_);
+        if (perf_) {
+            SBS_HIP(hipEventRecord(ev[1], stream_));
+            SBS_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            SBS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            perf_data_[3].ms += ms;
+        }
+    }
+    // FixedBodyData::HandleCollisions on XTilde and V, after X was taken
+    // (pdSolver.cu:228-231)
+    if (data.fixed) {
+        if (perf_) SBS_HIP(hipEventRecord(ev[0], stream_));
+        data.fixed->handle(data.XTilde, data.V, nv_, (T)params.mu_t, (T)params.mu_n, stream_);
+        if (perf_) {
+            SBS_HIP(hipEventRecord(ev[1], stream_));
+            SBS_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            SBS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            perf_data_[2].ms += ms;
+        }
     }
     // counters of the step (PCG iterations, unconverged solves, max per solve,
     // solves, last, per-solve log) land in this step's pinned slot

@@ -57,7 +57,11 @@ struct PdParams {
     double dense_tol = 1e-10;   // DenseCholesky: max|D^-1 r| <= tol max|D^-1 b| (FP64 refinement)
     int dense_max_iter = 8;     // DenseCholesky: refinement corrections per solve
     int pcg_halo = 1;           // 1: halo-staged PCG kernels (when the tiles could be built); 0: row gathers
+    double mu_n = 0.5;          // fixed-body response: normal restitution (SolverParams::muN)
+    double mu_t = 0.5;          // fixed-body response: friction (SolverParams::muT)
 };
+
+class FixedBodies;
 
 // Caller-owned device arrays: the reference's SolverData<Scalar> (src/def.h:22-56).
 template <class T> struct PdData {
@@ -78,6 +82,7 @@ template <class T> struct PdData {
     const uint32_t* TriFathers = nullptr;  // num_tris: body of each triangle
     T* tIs = nullptr;                   // N_v: 1, or 0.5 on hit vertices
     float* Normals = nullptr;           // 3*N_v contact normals
+    const FixedBodies* fixed = nullptr; // SolverData::pFixedBodies (def.h:43); null: none
 };
 
 struct PerfEntry {

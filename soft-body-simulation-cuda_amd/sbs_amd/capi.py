@@ -41,7 +41,7 @@ class SbsSolverData(ctypes.Structure):
                 ("mass", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("DBC", ctypes.c_void_p),
                 ("moreDBC", ctypes.c_void_p), ("V0", ctypes.c_void_p), ("num_tris", ctypes.c_int),
                 ("Tri", ctypes.c_void_p), ("TriFathers", ctypes.c_void_p), ("tIs", ctypes.c_void_p),
-                ("Normals", ctypes.c_void_p)]
+                ("Normals", ctypes.c_void_p), ("fixed_bodies", ctypes.c_void_p)]
 
 
 class SbsSolverParams(ctypes.Structure):
@@ -49,7 +49,8 @@ class SbsSolverParams(ctypes.Structure):
                 ("tol", ctypes.c_double), ("rho", ctypes.c_double), ("handle_collision", ctypes.c_int),
                 ("pcg_tol", ctypes.c_double), ("pcg_floor", ctypes.c_double), ("pcg_max_iter", ctypes.c_int),
                 ("pcg_chebyshev", ctypes.c_int), ("pcg_adaptive", ctypes.c_int), ("dense_precision", ctypes.c_int),
-                ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int), ("pcg_halo", ctypes.c_int)]
+                ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int), ("pcg_halo", ctypes.c_int),
+                ("mu_n", ctypes.c_double), ("mu_t", ctypes.c_double)]
 
 
 class SbsPdStats(ctypes.Structure):
@@ -57,6 +58,13 @@ class SbsPdStats(ctypes.Structure):
                 ("pcg_cap_hits", ctypes.c_longlong), ("nnz_offdiag", ctypes.c_longlong),
                 ("pcg_iterations_total", ctypes.c_longlong), ("pcg_solves_total", ctypes.c_longlong),
                 ("halo_groups", ctypes.c_int)]
+
+
+FIXED_SPHERE, FIXED_PLANE, FIXED_CYLINDER = 0, 1, 2
+
+
+class SbsFixedBody(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("model", ctypes.c_float * 16), ("radius", ctypes.c_float)]
 
 
 class SbsIpcData(ctypes.Structure):
@@ -111,6 +119,10 @@ SIGNATURES = {
     "sbs_collision_get_queries": (_I, [_VP, _VP, _VP, _VP]),
     "sbs_collision_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)]),
     "sbs_ccd_response": (_I, [_I, _I, _VP, _VP, _VP, _VP, _VP]),
+    "sbs_fixed_create": (_I, [ctypes.POINTER(SbsFixedBody), _I, ctypes.POINTER(_VP)]),
+    "sbs_fixed_destroy": (_I, [_VP]),
+    "sbs_fixed_handle_collisions": (_I, [_VP, _I, _VP, _VP, _I, _D, _D, _VP]),
+    "sbs_fixed_params": (_I, [_VP, _VP]),
     "sbs_ipc_create": (_I, [ctypes.POINTER(SbsIpcData), ctypes.POINTER(_VP)]),
     "sbs_ipc_destroy": (_I, [_VP]),
     "sbs_ipc_update": (_I, [_VP, ctypes.POINTER(SbsIpcData), ctypes.POINTER(SbsIpcParams), ctypes.POINTER(_I)]),

@@ -45,13 +45,16 @@ class SolverParams:
     dense_tol: float = 1e-10
     dense_max_iter: int = 8
     pcg_halo: bool = True         # halo-staged PCG kernels (LDS) where the mesh can be tiled
+    muN: float = 0.5              # fixed-body response: normal restitution
+    muT: float = 0.5              # fixed-body response: friction
 
     def to_c(self) -> capi.SbsSolverParams:
         return capi.SbsSolverParams(int(self.numIterations), float(self.dt), float(self.gravity), float(self.tol),
                                     float(self.rho), int(self.handleCollision), float(self.pcg_tol),
                                     float(self.pcg_floor), int(self.pcg_max_iter), int(self.pcg_chebyshev),
                                     int(self.pcg_adaptive), int(self.dense_precision), float(self.dense_tol),
-                                    int(self.dense_max_iter), int(self.pcg_halo))
+                                    int(self.dense_max_iter), int(self.pcg_halo), float(self.muN),
+                                    float(self.muT))
 
 
 class SolverData:
@@ -96,6 +99,11 @@ class SolverData:
             self.c.TriFathers = put("TriFathers", np.ascontiguousarray(scene.tri_fathers, dtype=np.uint32))
         self.c.tIs = put("tIs", np.ones(self.nv, dtype=dt))
         self.c.Normals = put("Normals", np.zeros(3 * self.nv, dtype=np.float32))
+
+    def set_fixed_bodies(self, fixed) -> None:
+        """SolverData::pFixedBodies (def.h:43): a FixedBodyData, or None."""
+        self.fixed_bodies = fixed
+        self.c.fixed_bodies = fixed.h if fixed is not None else None
 
     def get(self, name: str) -> np.ndarray:
         if name in ("X", "V", "X0", "XTilde", "DBCX"):

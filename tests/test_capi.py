@@ -37,3 +37,28 @@ def test_errors_are_reported_not_raised():
     assert b"precision" in L.sbs_last_error()
     assert L.sbs_pd_set_global_solver(None, 0) == -1
     assert L.sbs_version() >= 1
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct of the header has the same size and field offsets in C (gcc)
+    as in the ctypes binding."""
+    import subprocess
+    structs = {"SbsSolverData": capi.SbsSolverData, "SbsSolverParams": capi.SbsSolverParams,
+               "SbsPdStats": capi.SbsPdStats, "SbsFixedBody": capi.SbsFixedBody, "SbsIpcData": capi.SbsIpcData,
+               "SbsIpcParams": capi.SbsIpcParams, "SbsIpcStats": capi.SbsIpcStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{capi.HEADER_PATH}"', "int main(void) {"]
+    expect = []
+    for name, cls in structs.items():
+        lines.append(f'printf("%zu\\n", sizeof({name}));')
+        expect.append(ctypes.sizeof(cls))
+        for field, _ in cls._fields_:
+            cfield = "lambda" if field == "lambda_" else field
+            lines.append(f'printf("%zu\\n", offsetof({name}, {cfield}));')
+            expect.append(getattr(cls, field).offset)
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == expect
