@@ -243,6 +243,38 @@ int sbs_fixed_handle_collisions(SbsFixed* f, int precision, void* d_X, void* d_V
  * u[3] (plane normal / cylinder axis, else 0), radius, 0. */
 int sbs_fixed_params(SbsFixed* f, float* out);
 
+/* ------------------------------------------------------- scene loading --
+ * DataLoader (src/simulation/dataLoader.h:10-37) for one soft body from
+ * TetGen files, on the host:
+ *   CollectData(nodeFile, eleFile, faceFile, pos, scale, rot, centralize,
+ *               startIndex, attrib)  dataLoader.h:17, dataLoader.cu:204-237 -> sbs_mesh_load_tetgen
+ * with loadNodeFile (:130-170: coordinates read as float, optional centring
+ * with a y/z swap), loadEleFaceFile (:39-127: the .face triangles, or when
+ * there is none the tet faces seen an odd number of times, ascending by
+ * sorted vertex triple, with the winding of their last occurrence) and
+ * CollectEdges (:172-199), then the model transform T(pos) S(scale) Rx Ry Rz
+ * (rotations in degrees) of TransformVertices (utilities.cu:57-64) in the
+ * chosen precision.  Indices come back 0-based.  Comment lines ('#') are
+ * skipped; an unreadable node or ele file, a malformed record or a vertex
+ * index outside the node list is an error (the reference prints and goes
+ * on); a face file that is NULL, empty or does not open means "extract". */
+typedef struct SbsTetgenOptions {
+    int start_index;      /* the files' first vertex index (0 or 1)        */
+    int centralize;       /* subtract the centroid, then swap y and z      */
+    float pos[3];
+    float scale[3];
+    float rot[3];         /* degrees about x, then y, then z               */
+    int precision;        /* 32: float arithmetic, 64: double              */
+} SbsTetgenOptions;
+typedef struct SbsMesh SbsMesh;
+int sbs_mesh_load_tetgen(const char* node_file, const char* ele_file, const char* face_file,
+                         const SbsTetgenOptions* opt, SbsMesh** out);
+int sbs_mesh_counts(const SbsMesh* m, int* num_verts, int* num_tets, int* num_tris, int* num_edges,
+                    int* tris_from_file);
+/* Host buffers of the sizes sbs_mesh_counts gives (3*nv, 4*nt, 3*ntri, 2*nedge); any may be NULL. */
+int sbs_mesh_copy(const SbsMesh* m, double* X, uint32_t* tet, uint32_t* tri, uint32_t* edges);
+int sbs_mesh_destroy(SbsMesh* m);
+
 /* ---------------------------------------------------------------- IPC --
  * The FP64 incremental-potential-contact solver of the reference:
  *   IPCSolver(int threadsPerBlock, const SolverData<double>&)

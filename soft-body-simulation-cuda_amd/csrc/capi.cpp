@@ -13,6 +13,7 @@
 #include "ipc_engine.hpp"
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
+#include "scene_loader.hpp"
 
 namespace sbs {
 
@@ -33,6 +34,10 @@ struct SbsPd {
 
 struct SbsIpc {
     std::unique_ptr<sbs::IpcEngine> e;
+};
+
+struct SbsMesh {
+    sbs::BodyMesh m;
 };
 
 struct SbsFixed {
@@ -417,6 +422,57 @@ int sbs_device_count(int* count) {
         *count = 0;
         SBS_HIP(hipGetDeviceCount(count));
     });
+}
+
+int sbs_mesh_load_tetgen(const char* node_file, const char* ele_file, const char* face_file,
+                         const SbsTetgenOptions* opt, SbsMesh** out) {
+    return guarded([&] {
+        SBS_REQUIRE(out != nullptr && node_file != nullptr && ele_file != nullptr && opt != nullptr,
+                    "sbs_mesh_load_tetgen: null argument");
+        *out = nullptr;
+        sbs::TetgenOptions o;
+        o.start_index = opt->start_index;
+        o.centralize = opt->centralize != 0;
+        for (int k = 0; k < 3; ++k) {
+            o.pos[k] = opt->pos[k];
+            o.scale[k] = opt->scale[k];
+            o.rot[k] = opt->rot[k];
+        }
+        o.precision = opt->precision;
+        auto h = std::make_unique<SbsMesh>();
+        try {
+            h->m = sbs::load_tetgen(node_file, ele_file, face_file ? face_file : "", o);
+        } catch (const std::invalid_argument& e) {
+            throw sbs::Error(SBS_E_INVALID, std::string("sbs_mesh_load_tetgen: ") + e.what());
+        }
+        *out = h.release();
+    });
+}
+
+int sbs_mesh_counts(const SbsMesh* m, int* num_verts, int* num_tets, int* num_tris, int* num_edges,
+                    int* tris_from_file) {
+    return guarded([&] {
+        SBS_REQUIRE(m != nullptr, "null SbsMesh handle");
+        if (num_verts) *num_verts = (int)(m->m.X.size() / 3);
+        if (num_tets) *num_tets = (int)(m->m.tet.size() / 4);
+        if (num_tris) *num_tris = (int)(m->m.tri.size() / 3);
+        if (num_edges) *num_edges = (int)(m->m.edges.size() / 2);
+        if (tris_from_file) *tris_from_file = m->m.tri_from_file ? 1 : 0;
+    });
+}
+
+int sbs_mesh_copy(const SbsMesh* m, double* X, uint32_t* tet, uint32_t* tri, uint32_t* edges) {
+    return guarded([&] {
+        SBS_REQUIRE(m != nullptr, "null SbsMesh handle");
+        if (X) std::copy(m->m.X.begin(), m->m.X.end(), X);
+        if (tet) std::copy(m->m.tet.begin(), m->m.tet.end(), tet);
+        if (tri) std::copy(m->m.tri.begin(), m->m.tri.end(), tri);
+        if (edges) std::copy(m->m.edges.begin(), m->m.edges.end(), edges);
+    });
+}
+
+int sbs_mesh_destroy(SbsMesh* m) {
+    return guarded([&] { delete m; });
 }
 
 int sbs_fixed_create(const SbsFixedBody* bodies, int num_bodies, SbsFixed** out) {

@@ -67,6 +67,11 @@ class SbsFixedBody(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("model", ctypes.c_float * 16), ("radius", ctypes.c_float)]
 
 
+class SbsTetgenOptions(ctypes.Structure):
+    _fields_ = [("start_index", ctypes.c_int), ("centralize", ctypes.c_int), ("pos", ctypes.c_float * 3),
+                ("scale", ctypes.c_float * 3), ("rot", ctypes.c_float * 3), ("precision", ctypes.c_int)]
+
+
 class SbsIpcData(ctypes.Structure):
     _fields_ = [("num_verts", ctypes.c_int), ("num_tets", ctypes.c_int), ("num_tris", ctypes.c_int),
                 ("X", ctypes.c_void_p), ("V", ctypes.c_void_p), ("XTilde", ctypes.c_void_p),
@@ -119,6 +124,10 @@ SIGNATURES = {
     "sbs_collision_get_queries": (_I, [_VP, _VP, _VP, _VP]),
     "sbs_collision_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)]),
     "sbs_ccd_response": (_I, [_I, _I, _VP, _VP, _VP, _VP, _VP]),
+    "sbs_mesh_load_tetgen": (_I, [_CP, _CP, _CP, ctypes.POINTER(SbsTetgenOptions), ctypes.POINTER(_VP)]),
+    "sbs_mesh_counts": (_I, [_VP] + [ctypes.POINTER(_I)] * 5),
+    "sbs_mesh_copy": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "sbs_mesh_destroy": (_I, [_VP]),
     "sbs_fixed_create": (_I, [ctypes.POINTER(SbsFixedBody), _I, ctypes.POINTER(_VP)]),
     "sbs_fixed_destroy": (_I, [_VP]),
     "sbs_fixed_handle_collisions": (_I, [_VP, _I, _VP, _VP, _I, _D, _D, _VP]),
