@@ -301,6 +301,8 @@ typedef struct SbsIpcData {
     const double* mu;           /* num_tets                                      */
     const double* lambda;       /* num_tets                                      */
     const double* DBC;          /* num_verts: nonzero = fixed vertex (DBCIdx)    */
+    struct SbsFixed* fixed_bodies; /* NULL or the bodies of the implicit barrier
+                                      (SolverData::pFixedBodies, ImplicitBarrierEnergy) */
 } SbsIpcData;
 
 /* The IPC fields of SolverParams<double> (src/def.h:83-100) plus the linear solve. */

@@ -35,8 +35,18 @@
 //     of the identity (the reference's per-entry elimination leaves a
 //     diagonal equal to the number of contributing entries, which does not
 //     change the solution since its right-hand side is 0);
-//   * no fixed-body (implicit barrier) terms: the scenes have no fixed bodies,
-//     for which the reference's implicit barrier contributes 0 and step 1.
+//   * contact areas (computeInvDmV0, solverUtil.cuh:110-137, divided by the
+//     degree, femSolver.cu:13-16) start from zero (the reference accumulates
+//     into uninitialised memory).
+//
+// Fixed bodies (ImplicitBarrierEnergy, src/simulation/energy/implicitBarrier.cu):
+// per vertex and body, planes then cylinders then spheres, the signed distance
+// d (plane: (x - p).u; cylinder: |(I - a a^T)(x - c)| - r; sphere: |x - c| - r)
+// and, when d < dhat, area * dhat * kappa/2 (s - 1) ln s with s = d/dhat and
+// kappa = 100 (FixedBody::kappa, rigid.h:25; solverUtil.cuh:33-47 for the
+// value, gradient and Hessian, the Hessian made positive definite per body);
+// the first step is bounded by 0.9 of each vertex's distance to each body
+// along -p (InitStepSize, implicitBarrier.cu:352-407).
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -351,6 +361,11 @@ struct Ipc {
     int max_iter = 100, lin_max_iter = 20000;
     int ignore_self = 0;  // the detector's flag: UpdateQueries uses the last value set
     std::vector<Query> qs;
+    // fixed bodies: type (0 sphere, 1 plane, 2 cylinder) and 8 floats each
+    // (p[3], u[3], r, 0), and each vertex's contact area
+    std::vector<int> fb_type;
+    std::vector<float> fb_prm;
+    std::vector<double> area;
     // statistics of the last step
     int newton_iters = 0, line_halvings = 0, lin_iters = 0, ok = 1;
     double last_alpha0 = 1.0;
@@ -393,6 +408,94 @@ struct Ipc {
         orc_svd3_f64(&F[0][0], &U[0][0], s, &Vm[0][0]);
     }
 
+    // Fixed-body barrier of vertex v at x: energy, gradient and the sum over
+    // bodies of the made-PD 3x3 Hessians (any may be null), unscaled.
+    void fixed_terms(const std::vector<double>& x, int v, double* e, double* g, double* H) const {
+        const double kap = 1e2;
+        const double* xv = &x[3 * (size_t)v];
+        for (int pass : {1, 2, 0}) {  // planes, cylinders, spheres
+            for (size_t j = 0; j < fb_type.size(); ++j) {
+                if (fb_type[j] != pass) continue;
+                const float* q = &fb_prm[8 * j];
+                const double p[3] = {q[0], q[1], q[2]}, u[3] = {q[3], q[4], q[5]}, r = q[6];
+                double n[3], d;
+                if (pass == 1) {
+                    d = 0.0;
+                    for (int k = 0; k < 3; ++k) d += (xv[k] - p[k]) * u[k];
+                    for (int k = 0; k < 3; ++k) n[k] = u[k];
+                } else {
+                    double w[3] = {xv[0] - p[0], xv[1] - p[1], xv[2] - p[2]};
+                    if (pass == 2) {  // remove the axial part
+                        const double au = w[0] * u[0] + w[1] * u[1] + w[2] * u[2];
+                        for (int k = 0; k < 3; ++k) w[k] -= au * u[k];
+                    }
+                    const double len = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+                    d = len - r;
+                    for (int k = 0; k < 3; ++k) n[k] = w[k] / len;
+                }
+                if (!(d < dhat)) continue;
+                const double s_ = d / dhat, A = area[v];
+                if (e) *e += A * dhat * kap * 0.5 * (s_ - 1) * std::log(s_);
+                if (g) {
+                    const double c = A * dhat * (kap / 2 * (std::log(s_) / dhat + (s_ - 1) / d));
+                    for (int k = 0; k < 3; ++k) g[k] += c * n[k];
+                }
+                if (H) {
+                    const double c = A * dhat * kap / (2 * d * d * dhat) * (d + dhat);
+                    double M[9];
+                    for (int a = 0; a < 3; ++a)
+                        for (int b2 = 0; b2 < 3; ++b2) M[3 * a + b2] = c * (n[a] * n[b2]);
+                    make_pd(3, M);
+                    for (int k = 0; k < 9; ++k) H[k] += M[k];
+                }
+            }
+        }
+    }
+
+    // InitStepSize of the fixed bodies: the largest alpha <= 1 that keeps
+    // 10% of every vertex's distance to every body it approaches along -p.
+    double fixed_step(const std::vector<double>& x, const std::vector<double>& pdir) const {
+        double alpha = 1.0;
+        for (int v = 0; v < nv; ++v) {
+            const double* xv = &x[3 * (size_t)v];
+            const double lp[3] = {-pdir[3 * (size_t)v], -pdir[3 * (size_t)v + 1], -pdir[3 * (size_t)v + 2]};
+            for (size_t j = 0; j < fb_type.size(); ++j) {
+                const float* q = &fb_prm[8 * j];
+                const double p[3] = {q[0], q[1], q[2]}, u[3] = {q[3], q[4], q[5]}, r = q[6];
+                double cand = NAN;
+                if (fb_type[j] == 1) {
+                    const double pn = lp[0] * u[0] + lp[1] * u[1] + lp[2] * u[2];
+                    if (pn < 0) {
+                        double dist = 0.0;
+                        for (int k = 0; k < 3; ++k) dist += u[k] * (xv[k] - p[k]);
+                        cand = 0.9 * dist / -pn;
+                    }
+                } else {
+                    double w[3] = {xv[0] - p[0], xv[1] - p[1], xv[2] - p[2]}, pp[3] = {lp[0], lp[1], lp[2]};
+                    if (fb_type[j] == 2) {
+                        const double aw = w[0] * u[0] + w[1] * u[1] + w[2] * u[2];
+                        const double ap = pp[0] * u[0] + pp[1] * u[1] + pp[2] * u[2];
+                        for (int k = 0; k < 3; ++k) {
+                            w[k] -= aw * u[k];
+                            pp[k] -= ap * u[k];
+                        }
+                    }
+                    // the reference tests the approach with the unprojected p for cylinders
+                    const double pn = (fb_type[j] == 2 ? lp[0] * w[0] + lp[1] * w[1] + lp[2] * w[2]
+                                                       : pp[0] * w[0] + pp[1] * w[1] + pp[2] * w[2]);
+                    if (pn < 0) {
+                        const double a2 = pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2];
+                        const double b = w[0] * pp[0] + w[1] * pp[1] + w[2] * pp[2];
+                        const double c = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] - r * r;
+                        cand = 0.9 * (-b - std::sqrt(b * b - a2 * c)) / a2;
+                    }
+                }
+                alpha = std::fmin(alpha, cand);  // a ray that misses (NaN) leaves alpha
+            }
+        }
+        return alpha;
+    }
+
     double energy(const std::vector<double>& x) const {
         const double h2 = dt * dt;
         double inertia = 0.0, grav = 0.0, elastic = 0.0, bar = 0.0;
@@ -416,7 +519,10 @@ struct Ipc {
         }
         for (const Query& q : qs)
             if (q.d < dhat * dhat) bar += barrier(q.d, dhat, kappa);
-        return inertia + h2 * (grav + elastic + bar);
+        double fixedb = 0.0;
+        if (!fb_type.empty())
+            for (int v = 0; v < nv; ++v) fixed_terms(x, v, &fixedb, nullptr, nullptr);
+        return inertia + h2 * (grav + elastic + fixedb + bar);
     }
 
     // gradient of the tet's elastic energy (scaled by coef) and, optionally, its
@@ -515,6 +621,12 @@ struct Ipc {
             for (int j = 0; j < 4; ++j)
                 for (int k = 0; k < 3; ++k) g[3 * (size_t)tet[4 * (size_t)t + j] + k] += g12[3 * j + k];
         }
+        if (!fb_type.empty())
+            for (int v = 0; v < nv; ++v) {
+                double gf[3] = {0, 0, 0};
+                fixed_terms(x, v, nullptr, gf, nullptr);
+                for (int k = 0; k < 3; ++k) g[3 * (size_t)v + k] += h2 * gf[k];
+            }
         double ge = 0.0;
         for (double a : g) ge = std::max(ge, std::fabs(a));
         std::fill(g.begin(), g.end(), 0.0);
@@ -558,6 +670,16 @@ struct Ipc {
             elastic_local(x, t, h2, g12, H);
             scatter(&tet[4 * (size_t)t]);
         }
+        if (!fb_type.empty())
+            for (int v = 0; v < nv; ++v) {
+                double gf[3] = {0, 0, 0}, Hf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                fixed_terms(x, v, nullptr, gf, Hf);
+                for (int k = 0; k < 3; ++k) {
+                    g[3 * (size_t)v + k] += h2 * gf[k];
+                    for (int l = 0; l < 3; ++l)
+                        coo.push_back({(long long)(3 * v + k) * n + (3 * v + l), h2 * Hf[3 * k + l]});
+                }
+            }
         for (const Query& q : qs) {
             if (q.d > dhat * dhat) continue;
             barrier_local(x, q, h2, kappa, g12, H);
@@ -662,7 +784,8 @@ struct Ipc {
             if (++newton_iters > max_iter) return ok = 0, false;
             for (size_t i = 0; i < X.size(); ++i) xt[i] = X[i] - p[i];
             const double toi = min_step(X, xt);
-            double alpha = 0.95 * std::min(1.0, 0.9 * toi);
+            const double af = fb_type.empty() ? 1.0 : fixed_step(X, p);
+            double alpha = 0.95 * std::min(af, 0.9 * toi);
             alpha = alpha < 1e-12 ? 0.0 : std::min(1.0, alpha);
             if (newton_iters == 1) last_alpha0 = alpha;
             while (true) {
@@ -714,6 +837,33 @@ void precompute(Ipc& s) {
         m[8] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
         s.V0[t] = std::fabs(det) / 6.0;
     }
+    // contact area per vertex: over its tets, a third of the areas of the
+    // tet's three faces that contain it, averaged over the tets
+    s.area.assign(s.nv, 0.0);
+    std::vector<double> deg(s.nv, 0.0);
+    for (int t = 0; t < s.nt; ++t) {
+        const uint32_t* tv = &s.tet[4 * (size_t)t];
+        double fa[4];  // faces (0 1 2), (1 2 3), (2 3 0), (3 0 1)
+        for (int f = 0; f < 4; ++f) {
+            const double* a = &s.X[3 * (size_t)tv[f]];
+            const double* b = &s.X[3 * (size_t)tv[(f + 1) % 4]];
+            const double* c = &s.X[3 * (size_t)tv[(f + 2) % 4]];
+            const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+            const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                         cz = e1[0] * e2[1] - e1[1] * e2[0];
+            fa[f] = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        }
+        for (int j = 0; j < 4; ++j) {  // vertex j lies on every face but face j+1
+            double sum = 0.0;
+            for (int f = 0; f < 4; ++f)
+                if (f != (j + 1) % 4) sum += fa[f];
+            s.area[tv[j]] += sum / 3.0;
+            deg[tv[j]] += 1.0;
+        }
+    }
+    for (int v = 0; v < s.nv; ++v)
+        if (deg[v] > 0) s.area[v] /= deg[v];
 }
 
 }  // namespace
@@ -815,6 +965,12 @@ double orc_ipc_energy(void* h, const double* X, double* grad) {
             for (int j = 0; j < 4; ++j)
                 for (int k = 0; k < 3; ++k) g[3 * (size_t)s->tet[4 * (size_t)t + j] + k] += g12[3 * j + k];
         }
+        if (!s->fb_type.empty())
+            for (int v = 0; v < s->nv; ++v) {
+                double gf[3] = {0, 0, 0};
+                s->fixed_terms(x, v, nullptr, gf, nullptr);
+                for (int k = 0; k < 3; ++k) g[3 * (size_t)v + k] += h2 * gf[k];
+            }
         for (const Query& q : s->qs) {
             if (q.d > s->dhat * s->dhat) continue;
             s->barrier_local(x, q, h2, s->kappa, g12, nullptr);
@@ -824,6 +980,35 @@ double orc_ipc_energy(void* h, const double* X, double* grad) {
         std::memcpy(grad, g.data(), g.size() * sizeof(double));
     }
     return e;
+}
+// Fixed bodies of the implicit barrier: n bodies, type (0 sphere, 1 plane,
+// 2 cylinder) and 8 floats each (p[3], u[3], radius, 0).
+void orc_ipc_set_fixed(void* h, int n, const int* types, const float* params) {
+    auto* s = (Ipc*)h;
+    s->fb_type.assign(types, types + n);
+    s->fb_prm.assign(params, params + 8 * (size_t)n);
+}
+// Fixed-body barrier of one vertex at X (unscaled by h^2): energy, gradient
+// (3) and the made-PD Hessian summed over bodies (9, row-major).
+double orc_ipc_fixed_vertex(void* h, const double* X, int v, double* g, double* H) {
+    auto* s = (Ipc*)h;
+    std::vector<double> x(X, X + 3 * (size_t)s->nv);
+    double e = 0.0, gl[3] = {0, 0, 0}, Hl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    s->fixed_terms(x, v, &e, gl, Hl);
+    if (g) std::memcpy(g, gl, sizeof gl);
+    if (H) std::memcpy(H, Hl, sizeof Hl);
+    return e;
+}
+// Contact area per vertex (nv).
+void orc_ipc_contact_area(void* h, double* out) {
+    auto* s = (Ipc*)h;
+    std::memcpy(out, s->area.data(), s->area.size() * sizeof(double));
+}
+// Fixed-body step bound at X along -p.
+double orc_ipc_fixed_step(void* h, const double* X, const double* p) {
+    auto* s = (Ipc*)h;
+    std::vector<double> x(X, X + 3 * (size_t)s->nv), pd(p, p + 3 * (size_t)s->nv);
+    return s->fixed_step(x, pd);
 }
 // x~ and kappa for energy checks without a step.
 void orc_ipc_set_state(void* h, const double* XT, double kappa) {

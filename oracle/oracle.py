@@ -118,6 +118,12 @@ def lib(bench: bool = False) -> ctypes.CDLL:
         L.orc_ipc_set_state.argtypes = [vp, dp, ctypes.c_double]
         L.orc_ipc_queries.restype = ctypes.c_longlong
         L.orc_ipc_queries.argtypes = [vp, up, ip, dp]
+        L.orc_ipc_set_fixed.argtypes = [vp, ctypes.c_int, ip, fp]
+        L.orc_ipc_contact_area.argtypes = [vp, dp]
+        L.orc_ipc_fixed_vertex.restype = ctypes.c_double
+        L.orc_ipc_fixed_vertex.argtypes = [vp, dp, ctypes.c_int, dp, dp]
+        L.orc_ipc_fixed_step.restype = ctypes.c_double
+        L.orc_ipc_fixed_step.argtypes = [vp, dp, dp]
     _LIBS[name] = L
     return L
 
@@ -391,6 +397,32 @@ class IpcOracle:
         g = np.zeros(X.size) if grad else None
         e = self.L.orc_ipc_energy(self.h, _p(X, ctypes.c_double), None if g is None else _p(g, ctypes.c_double))
         return (e, g.reshape(-1, 3)) if grad else e
+
+    def set_fixed(self, types, params):
+        """Fixed bodies of the implicit barrier: types (0 sphere, 1 plane, 2
+        cylinder) and 8 floats each (the library's / fixed_params' layout)."""
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        prm = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        self._keep += [t, prm]
+        self.L.orc_ipc_set_fixed(self.h, len(t), _p(t, ctypes.c_int), _p(prm, ctypes.c_float))
+
+    def contact_area(self):
+        out = np.zeros(self.nv)
+        self.L.orc_ipc_contact_area(self.h, _p(out, ctypes.c_double))
+        return out
+
+    def fixed_vertex(self, X, v):
+        """Fixed-body barrier of vertex v (no h^2): energy, gradient (3), Hessian (3x3)."""
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        g, H = np.zeros(3), np.zeros(9)
+        e = self.L.orc_ipc_fixed_vertex(self.h, _p(X, ctypes.c_double), int(v), _p(g, ctypes.c_double),
+                                        _p(H, ctypes.c_double))
+        return e, g, H.reshape(3, 3)
+
+    def fixed_step(self, X, p):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        return self.L.orc_ipc_fixed_step(self.h, _p(X, ctypes.c_double), _p(p, ctypes.c_double))
 
     def queries(self):
         n = self.L.orc_ipc_queries(self.h, None, None, None)

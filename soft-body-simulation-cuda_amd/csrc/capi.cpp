@@ -535,6 +535,7 @@ sbs::IpcData to_ipc(const SbsIpcData* d) {
     o.mu = d->mu;
     o.lambda = d->lambda;
     o.DBC = d->DBC;
+    o.fixed_bodies = d->fixed_bodies ? d->fixed_bodies->f.get() : nullptr;
     return o;
 }
 }  // namespace

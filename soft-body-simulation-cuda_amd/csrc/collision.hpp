@@ -117,6 +117,8 @@ public:
     int cylinders() const { return nc_; }
     // kFixedFloats per body, in the caller's order (tests).
     const std::vector<float>& params() const { return params_; }
+    // the device records, grouped spheres, planes, cylinders
+    const float* device() const { return dev_.get(); }
 
 private:
     int ns_ = 0, np_ = 0, nc_ = 0;

@@ -179,10 +179,196 @@ __device__ __forceinline__ double barrier_d2(double d, double dhat, double kappa
 //   inertia 1/2 m |x - x~|^2 (inertia.cu:71-85)
 //   + h^2 (g m y (gravity.cu:23-34) + V0 (mu |S - I|^2 + lambda/2 tr(S - I)^2)
 //          (corotated.cu:196-214) + b(d) for d < dhat^2 (barrier.cu:137-155)).
+// ------------------------------------------------------ implicit barrier --
+// Fixed-body records (collision.hpp: p[3], u[3], r, grouped spheres, planes,
+// cylinders).  barrierFunc / Grad / Hess (solverUtil.cuh:33-47) with kappa =
+// FixedBody::kappa = 100 (rigid.h:25), over planes, then cylinders, then
+// spheres (implicitBarrier.cu:163-240), each body's Hessian made positive
+// definite on its own (makePD, solverUtil.cuh:157-232).
+struct Bodies {
+    const float* rec;
+    int ns, np, nc;
+};
+constexpr double kBodyKappa = 100.0;
+
+// signed distance of x to body j and the unit normal; cyl/sphere: |n| - r
+__device__ __forceinline__ double body_distance(const Bodies& b, int j, int kind, const double x[3], double n[3]) {
+    const float* q = b.rec + kFixedFloats * (size_t)j;
+    const double p[3] = {q[0], q[1], q[2]}, u[3] = {q[3], q[4], q[5]};
+    if (kind == kFixedPlane) {
+        for (int k = 0; k < 3; ++k) n[k] = u[k];
+        return (x[0] - p[0]) * u[0] + (x[1] - p[1]) * u[1] + (x[2] - p[2]) * u[2];
+    }
+    const double w[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+    double m[3];
+    if (kind == kFixedCylinder) {  // (I - a a^T) w, entries 1 - a_i a_j / 0 - a_i a_j
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int c = 0; c < 3; ++c) acc += ((r == c ? 1.0 : 0.0) - u[r] * u[c]) * w[c];
+            m[r] = acc;
+        }
+    } else {
+        for (int k = 0; k < 3; ++k) m[k] = w[k];
+    }
+    const double len = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    const double inv = 1.0 / len;
+    for (int k = 0; k < 3; ++k) n[k] = inv * m[k];
+    return len - (double)q[6];
+}
+
+__device__ void make_pd3(double (&M)[3][3]) {
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};  // columns: eigenvectors
+    for (int sweep = 0; sweep < 20; ++sweep) {
+        bool any = false;
+        for (int k = 0; k < 3; ++k) {
+            const int p = k < 2 ? 0 : 1, q = k == 0 ? 1 : 2;
+            const double apq = M[p][q];
+            if (fabs(apq) < 1e-9) continue;
+            any = true;
+            const double app = M[p][p], aqq = M[q][q];
+            const double theta = 0.5 * (aqq - app) / apq;
+            double t;
+            if (fabs(theta) > 1e10) t = 0.5 / theta;
+            else t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+            const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+            M[p][q] = M[q][p] = 0.0;
+            M[p][p] = c * c * app - 2.0 * s * c * apq + s * s * aqq;
+            M[q][q] = s * s * app + 2.0 * s * c * apq + c * c * aqq;
+            const int o = 3 - p - q;
+            const double aop = M[o][p], aoq = M[o][q];
+            M[o][p] = M[p][o] = c * aop - s * aoq;
+            M[o][q] = M[q][o] = s * aop + c * aoq;
+            for (int r = 0; r < 3; ++r) {
+                const double vp = V[r][p], vq = V[r][q];
+                V[r][p] = c * vp - s * vq;
+                V[r][q] = s * vp + c * vq;
+            }
+        }
+        if (!any) break;
+    }
+    double lam[3];
+    for (int k = 0; k < 3; ++k) lam[k] = M[k][k] < 1e-6 ? 1e-6 : M[k][k];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) acc += V[i][k] * lam[k] * V[j][k];
+            M[i][j] = acc;
+        }
+}
+
+// Energy, gradient and Hessian (any may be null) of vertex x over all bodies,
+// unscaled.
+__device__ void body_terms(const Bodies& b, const double x[3], double area, double dhat, double* e, double* g,
+                           double (*H)[3]) {
+    const int first[3] = {b.ns, b.ns + b.np, 0};           // planes, cylinders, spheres
+    const int count[3] = {b.np, b.nc, b.ns};
+    const int kind[3] = {kFixedPlane, kFixedCylinder, kFixedSphere};
+    for (int pass = 0; pass < 3; ++pass)
+        for (int j = first[pass]; j < first[pass] + count[pass]; ++j) {
+            double n[3];
+            const double d = body_distance(b, j, kind[pass], x, n);
+            if (!(d < dhat)) continue;
+            const double s = d / dhat, ls = log(s);
+            if (e) *e += area * dhat * kBodyKappa * 0.5 * (s - 1) * ls;
+            if (g) {
+                const double c = area * dhat * (kBodyKappa / 2 * (ls / dhat + (s - 1) / d));
+                for (int k = 0; k < 3; ++k) g[k] += c * n[k];
+            }
+            if (H) {
+                const double c = area * dhat * kBodyKappa / (2 * d * d * dhat) * (d + dhat);
+                double M[3][3];
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) M[r][q] = c * (n[r] * n[q]);
+                make_pd3(M);
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) H[r][q] += M[r][q];
+            }
+        }
+}
+
+// g += coef * gradient; with hv: hv = coef * Hessian block, diag += its diagonal.
+__global__ void ipc_body_grad_kernel(int nv, const double* X, Bodies b, const double* area, double dhat, double coef,
+                                     double* g, double* hv, double* diag) {
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v >= nv) return;
+    const double x[3] = {X[3 * (size_t)v], X[3 * (size_t)v + 1], X[3 * (size_t)v + 2]};
+    double gl[3] = {0, 0, 0}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    body_terms(b, x, area[v], dhat, nullptr, gl, hv ? H : nullptr);
+    for (int k = 0; k < 3; ++k) g[3 * (size_t)v + k] += coef * gl[k];
+    if (hv) {
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) hv[9 * (size_t)v + 3 * r + q] = coef * H[r][q];
+        for (int k = 0; k < 3; ++k) diag[3 * (size_t)v + k] += coef * H[k][k];
+    }
+}
+
+// InitStepSize of the bodies (implicitBarrier.cu:352-407): per block, the
+// smallest 0.9 * (distance to the body along the motion -p) over its
+// vertices, 1 when nothing is approached; a ray that misses a curved body
+// (NaN) changes nothing.
+__global__ void ipc_body_step_kernel(int nv, const double* X, const double* P, Bodies b, double* parts) {
+    __shared__ double red[kT];
+    double alpha = 1.0;
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v < nv) {
+        const double x[3] = {X[3 * (size_t)v], X[3 * (size_t)v + 1], X[3 * (size_t)v + 2]};
+        const double lp[3] = {-P[3 * (size_t)v], -P[3 * (size_t)v + 1], -P[3 * (size_t)v + 2]};
+        for (int j = 0; j < b.ns + b.np + b.nc; ++j) {
+            const float* q = b.rec + kFixedFloats * (size_t)j;
+            const double p[3] = {q[0], q[1], q[2]}, u[3] = {q[3], q[4], q[5]}, r = q[6];
+            double cand = NAN;
+            if (j >= b.ns && j < b.ns + b.np) {  // plane
+                const double pn = lp[0] * u[0] + lp[1] * u[1] + lp[2] * u[2];
+                if (pn < 0) cand = 0.9 * (u[0] * (x[0] - p[0]) + u[1] * (x[1] - p[1]) + u[2] * (x[2] - p[2])) / -pn;
+            } else {
+                const bool cyl = j >= b.ns + b.np;
+                const double w0[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+                double n[3], pp[3];
+                for (int rr = 0; rr < 3; ++rr) {
+                    double an = 0.0, ap = 0.0;
+                    for (int c = 0; c < 3; ++c) {
+                        const double m = cyl ? ((rr == c ? 1.0 : 0.0) - u[rr] * u[c]) : (rr == c ? 1.0 : 0.0);
+                        an += m * w0[c];
+                        ap += m * lp[c];
+                    }
+                    n[rr] = an;
+                    pp[rr] = ap;
+                }
+                const double pn = lp[0] * n[0] + lp[1] * n[1] + lp[2] * n[2];
+                if (pn < 0) {
+                    const double pp2 = pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2];
+                    const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+                    const double nd = n[0] * pp[0] + n[1] * pp[1] + n[2] * pp[2];
+                    cand = 0.9 * (-nd - sqrt(nd * nd - pp2 * (n2 - r * r))) / pp2;
+                }
+            }
+            alpha = fmin(alpha, cand);
+        }
+    }
+    red[threadIdx.x] = alpha;
+    __syncthreads();
+    for (int s = kT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = red[0];
+}
+__global__ void min_final_kernel(int nb, const double* parts, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double m = 1.0;
+    for (int k = 0; k < nb; ++k) m = fmin(m, parts[k]);
+    out[0] = m;
+}
+
+Bodies bodies_of(const FixedBodies* f) {
+    if (!f || f->spheres() + f->planes() + f->cylinders() == 0) return Bodies{nullptr, 0, 0, 0};
+    return Bodies{f->device(), f->spheres(), f->planes(), f->cylinders()};
+}
+
 __global__ void ipc_energy_kernel(int nv, int nt, long long nq, const double* X, const double* XT,
                                   const double* mass, const uint32_t* tet, const double* dm, const double* v0,
                                   const double* mu, const double* lam, const double* qd, double dhat, double kappa,
-                                  double h2, double* parts) {
+                                  double h2, Bodies bodies, const double* area, double* parts) {
     const long long i = (long long)blockIdx.x * kT + threadIdx.x;
     double e[1] = {0.0};
     if (i < nv) {
@@ -192,6 +378,12 @@ __global__ void ipc_energy_kernel(int nv, int nt, long long nq, const double* X,
             s += d * d;
         }
         e[0] += 0.5 * mass[i] * s + h2 * kG * X[3 * i + 1] * mass[i];
+        if (bodies.rec) {
+            const double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+            double eb = 0.0;
+            body_terms(bodies, x, area[i], dhat, &eb, nullptr, nullptr);
+            e[0] += h2 * eb;
+        }
     }
     if (i < nt) {
         double F[3][3], U[3][3], sg[3], Vm[3][3];
@@ -407,11 +599,19 @@ __global__ void ipc_fix_kernel(int nv, const unsigned char* fixed, double* g, do
 // y = H x, element by element: inertia diagonal (identity on fixed rows),
 // then each tet's and each active query's 12x12 block on free rows/columns.
 __global__ void ipc_mv_vertex_kernel(int nv, const double* mass, const unsigned char* fixed, const double* x,
-                                     double* y) {
+                                     double* y, const double* hv) {
     const int v = blockIdx.x * kT + threadIdx.x;
     if (v >= nv) return;
     const double m = fixed[v] ? 1.0 : mass[v];
-    for (int k = 0; k < 3; ++k) y[3 * (size_t)v + k] = m * x[3 * (size_t)v + k];
+    const double xv[3] = {x[3 * (size_t)v], x[3 * (size_t)v + 1], x[3 * (size_t)v + 2]};
+    for (int k = 0; k < 3; ++k) {
+        double yk = m * xv[k];
+        if (hv && !fixed[v]) {  // the implicit barrier's block on free vertices
+            const double* h = hv + 9 * (size_t)v + 3 * k;
+            yk += h[0] * xv[0] + h[1] * xv[1] + h[2] * xv[2];
+        }
+        y[3 * (size_t)v + k] = yk;
+    }
 }
 __device__ __forceinline__ void mv_block(const uint32_t (&vv)[4], const double* H, const unsigned char* fixed,
                                          const double* x, double* y) {
@@ -504,6 +704,35 @@ IpcEngine::IpcEngine(const IpcData& data) {
         std::vector<uint32_t> tet(4 * (size_t)nt_);
         SBS_HIP(hipMemcpy(tet.data(), data.Tet, tet.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
         for (uint32_t v : tet) SBS_REQUIRE(v < (uint32_t)nv_, "IPCSolver: tet index out of range");
+        // contact areas (computeInvDmV0, solverUtil.cuh:123-136, femSolver.cu:13-16):
+        // over a vertex's tets, a third of the summed areas of the tet's three
+        // faces through it, divided by the number of tets (accumulated from 0)
+        std::vector<double> X(3 * (size_t)nv_), area(nv_, 0.0), deg(nv_, 0.0);
+        SBS_HIP(hipMemcpy(X.data(), data.X, X.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int t = 0; t < nt_; ++t) {
+            const uint32_t* tv = &tet[4 * (size_t)t];
+            double fa[4];  // faces (0 1 2), (1 2 3), (2 3 0), (3 0 1)
+            for (int f = 0; f < 4; ++f) {
+                const double* a = &X[3 * (size_t)tv[f]];
+                const double* b = &X[3 * (size_t)tv[(f + 1) % 4]];
+                const double* c = &X[3 * (size_t)tv[(f + 2) % 4]];
+                const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+                const double cx = e1[1] * e2[2] - e2[1] * e1[2], cy = e1[2] * e2[0] - e2[2] * e1[0],
+                             cz = e1[0] * e2[1] - e2[0] * e1[1];
+                fa[f] = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            }
+            const double per[4] = {(fa[0] + fa[2] + fa[3]) / 3.0, (fa[0] + fa[1] + fa[3]) / 3.0,
+                                   (fa[0] + fa[1] + fa[2]) / 3.0, (fa[1] + fa[2] + fa[3]) / 3.0};
+            for (int j = 0; j < 4; ++j) {
+                area[tv[j]] += per[j];
+                deg[tv[j]] += 1.0;
+            }
+        }
+        for (int v = 0; v < nv_; ++v)
+            if (deg[v] > 0) area[v] /= deg[v];
+        area_.alloc(nv_);
+        SBS_HIP(hipMemcpy(area_.get(), area.data(), area.size() * sizeof(double), hipMemcpyHostToDevice));
         if (ntri_ > 0) {
             std::vector<uint32_t> tri(3 * (size_t)ntri_);
             SBS_HIP(hipMemcpy(tri.data(), data.Tri, tri.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -517,6 +746,7 @@ IpcEngine::IpcEngine(const IpcData& data) {
     xn_.alloc(n3); xtmp_.alloc(n3); p_dir_.alloc(n3); g_.alloc(n3); diag_.alloc(n3);
     r_.alloc(n3); z_.alloc(n3); q_.alloc(n3);
     ht_.alloc(144 * (size_t)nt_);
+    hv_.alloc(9 * (size_t)nv_);
     parts_.alloc(2 * (size_t)nblk(std::max<size_t>(n3, (size_t)nt_)) + 64);
     out_.alloc(8);
     SBS_HIP(hipHostMalloc((void**)&host_, 8 * sizeof(double), hipHostMallocDefault));
@@ -586,7 +816,8 @@ double IpcEngine::energy(const double* X) {
     const unsigned nb = nblk(n);
     hipLaunchKernelGGL(ipc_energy_kernel, dim3(nb), dim3(kT), 0, stream_, nv_, nt_, nq_, X, (const double*)d_.XTilde,
                        d_.mass, d_.Tet, (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda,
-                       (const double*)qd_.get(), p_.dhat, kappa_, p_.dt * p_.dt, parts_.get());
+                       (const double*)qd_.get(), p_.dhat, kappa_, p_.dt * p_.dt, bodies_of(d_.fixed_bodies),
+                       (const double*)area_.get(), parts_.get());
     SBS_HIP(hipGetLastError());
     return reduce_sum(parts_.get(), (int)nb);
 }
@@ -602,6 +833,7 @@ void IpcEngine::gradient(const double* X, bool hessian, double kap) {
                        (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda, h2,
                        (const unsigned char*)fixed_.get(), g_.get(), hessian ? ht_.get() : nullptr,
                        hessian ? diag_.get() : nullptr);
+    body_gradient(X, h2, hessian);
     if (nq_ > 0)
         hipLaunchKernelGGL(ipc_barrier_kernel, dim3((unsigned)((nq_ + kQB - 1) / kQB)), dim3(kQB), 0, stream_, nq_,
                            col_->queries(), (const int*)qtype_.get(), (const double*)qd_.get(), X, p_.dhat, kap, h2,
@@ -611,6 +843,33 @@ void IpcEngine::gradient(const double* X, bool hessian, double kap) {
         hipLaunchKernelGGL(ipc_fix_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_,
                            (const unsigned char*)fixed_.get(), g_.get(), diag_.get());
     SBS_HIP(hipGetLastError());
+}
+
+// ImplicitBarrierEnergy::GradientHessian (implicitBarrier.cu:150-240): adds
+// coef * gradient to g and, with `hessian`, stores the per-vertex blocks and
+// adds their diagonal to the Jacobi diagonal.
+void IpcEngine::body_gradient(const double* X, double coef, bool hessian) {
+    const Bodies b = bodies_of(d_.fixed_bodies);
+    if (hessian) hv_live_ = b.rec != nullptr;
+    if (!b.rec) return;
+    hipLaunchKernelGGL(ipc_body_grad_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, X, b,
+                       (const double*)area_.get(), p_.dhat, coef, g_.get(), hessian ? hv_.get() : nullptr,
+                       hessian ? diag_.get() : nullptr);
+    SBS_HIP(hipGetLastError());
+}
+
+// ImplicitBarrierEnergy::InitStepSize (implicitBarrier.cu:352-407).
+double IpcEngine::body_step(const double* X, const double* p) {
+    const Bodies b = bodies_of(d_.fixed_bodies);
+    if (!b.rec) return 1.0;
+    const unsigned nb = nblk(nv_);
+    hipLaunchKernelGGL(ipc_body_step_kernel, dim3(nb), dim3(kT), 0, stream_, nv_, X, p, b, parts_.get());
+    hipLaunchKernelGGL(min_final_kernel, dim3(1), dim3(64), 0, stream_, (int)nb, (const double*)parts_.get(),
+                       out_.get());
+    SBS_HIP(hipGetLastError());
+    SBS_HIP(hipMemcpyAsync(host_, out_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SBS_HIP(hipStreamSynchronize(stream_));
+    return host_[0];
 }
 
 // UpdateKappa (ip.cu:77-115): kappa = max(100, max|grad of the other terms| /
@@ -624,6 +883,7 @@ void IpcEngine::update_kappa(const double* X) {
                        (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda, h2,
                        (const unsigned char*)fixed_.get(), g_.get(), (double*)nullptr, (double*)nullptr);
     SBS_HIP(hipGetLastError());
+    body_gradient(X, h2, false);
     const double ge = max_abs(g_.get(), n3);
     SBS_HIP(hipMemsetAsync(g_.get(), 0, g_.bytes(), stream_));
     if (nq_ > 0)
@@ -653,7 +913,8 @@ int IpcEngine::solve() {
     int it = 0;
     for (; it < p_.lin_max_iter;) {
         hipLaunchKernelGGL(ipc_mv_vertex_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, d_.mass,
-                           (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get());
+                           (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get(),
+                           hv_live_ ? (const double*)hv_.get() : nullptr);
         hipLaunchKernelGGL(ipc_mv_tet_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet,
                            (const double*)ht_.get(), (const unsigned char*)fixed_.get(), (const double*)pvec,
                            q_.get());
@@ -722,10 +983,12 @@ bool IpcEngine::step(IpcData& data, const IpcParams& p) {
         if (++stats_.newton_iterations > p.max_iterations) return false;
         hipLaunchKernelGGL(ipc_axpy_kernel, dim3(vb), dim3(kT), 0, stream_, n3, (const double*)X,
                            (const double*)p_dir_.get(), 1.0, xt);
-        // InitStepSize (ip.cu:116-123, barrier.cu:225-228): CCD over X -> X - p
+        // InitStepSize (ip.cu:116-123, barrier.cu:225-228): CCD over X -> X - p,
+        // and the fixed bodies' bound (1 without bodies)
         const double toi = ntri_ >= 2 ? col_->min_step(X, xt, true) : 1.0;
         ignore_self_ = true;
-        double alpha = 0.95 * std::min(1.0, 0.9 * toi);
+        const double af = body_step(X, p_dir_.get());
+        double alpha = 0.95 * std::min(af, 0.9 * toi);
         alpha = alpha < 1e-12 ? 0.0 : std::min(1.0, alpha);
         if (stats_.newton_iterations == 1) stats_.alpha0 = alpha;
         while (true) {

@@ -7,6 +7,11 @@
 // line search started at 0.95 min(1, 0.9 toi) (CCD over x -> x - p) and halved
 // until E does not increase, until |p|_inf / h < tol.
 //
+// Fixed bodies (ImplicitBarrierEnergy, implicitBarrier.cu) add, per vertex
+// and body within dhat, area * dhat * 50 (s - 1) ln s (s = d/dhat, d the signed
+// distance to the body) with its gradient and its made-PD 3x3 Hessian, and
+// bound the first step of the line search (InitStepSize, :352-407).
+//
 // Layout: per-tet Dm^-1 (row-major) and V0; per Newton iteration the 12x12
 // Hessian of every tet and of every active query (row-major, the query's or
 // tet's 4 vertices x 3 coordinates) and the gradient; the linear system
@@ -35,6 +40,7 @@ struct IpcData {
     const double* mu = nullptr;      // N_t
     const double* lambda = nullptr;  // N_t
     const double* DBC = nullptr;     // N_v: nonzero = fixed vertex (DBCIdx of the reference)
+    const FixedBodies* fixed_bodies = nullptr;  // SolverData::pFixedBodies: the implicit barrier's bodies
 };
 
 struct IpcParams {
@@ -79,6 +85,8 @@ private:
     int solve();
     double reduce_sum(const double* parts, int n);
     double max_abs(const double* v, size_t n);
+    double body_step(const double* X, const double* p);
+    void body_gradient(const double* X, double coef, bool hessian);
 
     int nv_ = 0, nt_ = 0, ntri_ = 0;
     IpcData d_{};
@@ -99,6 +107,9 @@ private:
     long long nq_ = 0;
     DevBuf<double> r_, z_, q_;              // PCG vectors
     DevBuf<double> parts_, out_;            // reduction partials, results
+    DevBuf<double> area_;                   // contact area per vertex (the implicit barrier's weight)
+    DevBuf<double> hv_;                     // 9 per vertex: the implicit barrier's Hessian blocks
+    bool hv_live_ = false;                  // hv_ holds this Newton iteration's blocks
     double* host_ = nullptr;                // pinned scalars
 };
 

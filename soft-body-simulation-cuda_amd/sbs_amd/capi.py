@@ -77,7 +77,7 @@ class SbsIpcData(ctypes.Structure):
                 ("X", ctypes.c_void_p), ("V", ctypes.c_void_p), ("XTilde", ctypes.c_void_p),
                 ("Tet", ctypes.c_void_p), ("Tri", ctypes.c_void_p), ("TriFathers", ctypes.c_void_p),
                 ("mass", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("lambda_", ctypes.c_void_p),
-                ("DBC", ctypes.c_void_p)]
+                ("DBC", ctypes.c_void_p), ("fixed_bodies", ctypes.c_void_p)]
 
 
 class SbsIpcParams(ctypes.Structure):

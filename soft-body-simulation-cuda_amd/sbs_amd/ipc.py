@@ -42,8 +42,9 @@ class IpcParams:
 class IpcData:
     """Device arrays (FP64) of a scene for the IPC solver."""
 
-    def __init__(self, scene, V=None):
+    def __init__(self, scene, V=None, fixed=None):
         self.nv, self.nt, self.ntri = scene.nv, scene.nt, scene.ntri
+        self.fixed_bodies = fixed          # a fixed.FixedBodyData: the implicit barrier's bodies
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
         u32 = lambda a: np.ascontiguousarray(a, dtype=np.uint32).ravel()
         self.arrays = {
@@ -63,7 +64,7 @@ class IpcData:
         a = self.arrays
         return capi.SbsIpcData(self.nv, self.nt, self.ntri, a["X"].ptr, a["V"].ptr, a["XTilde"].ptr, a["Tet"].ptr,
                                a["Tri"].ptr, a["TriFathers"].ptr, a["mass"].ptr, a["mu"].ptr, a["lambda"].ptr,
-                               a["DBC"].ptr)
+                               a["DBC"].ptr, self.fixed_bodies.h if self.fixed_bodies is not None else None)
 
     def get(self, name: str) -> np.ndarray:
         return self.arrays[name].download(np.float64, (self.nv, 3))

@@ -147,14 +147,13 @@ class SimulationContext:
             self.solver.SetPerf(True)
         else:
             from .ipc import IpcData, IpcParams, IpcSolver
-            # the IPC energy's implicit fixed-body barrier is not built: fixed
-            # bodies of a double context are parsed (fixed_specs) but not applied
+            # fixed bodies enter the IPC energy as its implicit barrier
             self.params = IpcParams(dt=float(ctx.get("dt", 0.001)), damp=float(ctx.get("damp", 0.999)),
                                     dhat=float(ctx.get("dhat", 1e-2)), tol=float(ctx.get("tolerance", 1e-2)),
                                     maxIterations=int(ctx.get("maxIterations", 100)))
-            self.data = IpcData(self.scene)
+            self.fixed = F.FixedBodyData(self.fixed_specs) if self.fixed_specs else None
+            self.data = IpcData(self.scene, fixed=self.fixed)
             self.solver = IpcSolver(self.data)
-            self.fixed = None
         self.steps = 0
         self.wall_ms = 0.0
 

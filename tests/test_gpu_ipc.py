@@ -108,3 +108,64 @@ def test_failure_latch_and_reset(gpu):
         solver.SetLinearSolver(LinearSolverType.Jacobi)
     solver.close()
     data.free()
+
+
+# ------------------------------------------------------------ fixed bodies --
+def fixed_pair(s, bodies, p):
+    from sbs_amd import fixed as F
+    fb = F.FixedBodyData(bodies)
+    data = IpcData(s, fixed=fb)
+    solver = IpcSolver(data)
+    o = oracle_for(s, p)
+    o.set_fixed([b.type for b in bodies], fb.params())          # one set of float parameters
+    return fb, data, solver, o
+
+
+def test_fixed_barrier_energy_and_gradient_match_oracle(gpu):
+    from sbs_amd import fixed as F
+    # the energy entry point evaluates with the default dhat (1e-3) until a step
+    # sets the parameters: bodies 0.5 mm from the vertices (jitter 0.1 mm)
+    s = mesh.cube_scene(3, pin_top=False, jitter=1e-4)
+    bodies = [F.Plane((0, -5e-4, 0)), F.Sphere((0.1, 0.1, -0.0505), radius=0.05),
+              F.Cylinder((0.35, 0.0, 0.1), (90, 0, 0), (0.0495, 1, 0.0495))]
+    p = IpcParams(dt=0.01, dhat=1e-3)
+    fb, data, solver, o = fixed_pair(s, bodies, p)
+    try:
+        X = s.X.copy()
+        XT = s.X + 1e-3
+        solver.set_state(XT, 250.0)
+        o.set_state(XT, 250.0)
+        e_gpu, g_gpu = solver.energy(X, grad=True)
+        e_orc, g_orc = o.energy(X, grad=True)
+        active = sum(o.fixed_vertex(X, v)[0] != 0.0 for v in range(s.nv))
+        assert active >= 10
+        assert abs(e_gpu - e_orc) <= 1e-12 * abs(e_orc)
+        # at the rest shape F = I up to rounding, so the elastic term leaves
+        # ~1e-14 of rounding noise per vertex (2 mu V0 |Dm^-1| h^2 eps); the
+        # barrier terms themselves are ~4e-5 here
+        assert np.abs(g_gpu - g_orc).max() <= 1e-10 * np.abs(g_orc).max() + 2e-13
+    finally:
+        solver.close(), data.free(), fb.close()
+
+
+def test_cube_falling_on_a_floor_matches_oracle(gpu):
+    from sbs_amd import fixed as F
+    s = mesh.cube_scene(3, pin_top=False)
+    s.X[:, 1] += 4e-3 - s.X[:, 1].min()
+    bodies = [F.Plane((0, 0, 0)), F.Plane((-0.01, 0, 0), (0, 0, -90))]   # a floor and a side wall
+    p = IpcParams(dt=0.01, dhat=1e-2, tol=1e-3)
+    fb, data, solver, o = fixed_pair(s, bodies, p)
+    try:
+        for k in range(8):
+            assert solver.Update(data, p) and o.step(), k
+            Xg, Xo = data.get("X"), o.get()[0]
+            sg, so = solver.stats(), o.stats()
+            scale = max(np.abs(Xo - s.X).max(), 1e-6)
+            assert np.abs(Xg - Xo).max() <= 1e-10 * scale, (k, np.abs(Xg - Xo).max(), scale)
+            assert sg["newton_iterations"] == so["newton_iterations"]
+            assert sg["line_search_halvings"] == so["line_search_halvings"]
+            assert np.isclose(sg["alpha0"], so["alpha0"], rtol=1e-12)
+            assert Xg[:, 1].min() > 0.0
+        assert Xg[:, 1].min() < 4e-3                               # it came down onto the floor
+    finally:
+        solver.close(), data.free(), fb.close()

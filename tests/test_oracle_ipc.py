@@ -170,3 +170,98 @@ def test_contact_keeps_bodies_apart():
     up = s.body_vert_start[1]
     assert (s.X[up:, 1] - X[up:, 1]).min() > 1.5e-3      # the upper cube came down onto the lower one
     assert o.stats()["kappa"] > 100.0                      # and the barrier stiffness adapted to the contact
+
+
+# ------------------------------------------------------------ fixed bodies --
+def fixed_setup(s, bodies, **kw):
+    from sbs_amd import fixed as F
+    o = scene_oracle(s, **kw)
+    types = [b.type for b in bodies]
+    models = np.stack([np.asarray(b.model, np.float32).T.reshape(-1) for b in bodies])
+    o.set_fixed(types, O.fixed_params(types, models, [b.radius for b in bodies]))
+    return o
+
+
+def test_contact_area_closed_form():
+    # one unit right tet: vertex 0 touches three faces of area 1/2, vertex 1
+    # two of them and the slanted face (sqrt(3)/2)
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    o = O.IpcOracle(X, np.zeros_like(X), [[0, 1, 2, 3]], np.zeros((0, 3)), [], [1.0] * 4, [1.0], [1.0],
+                    np.zeros(4))
+    a = o.contact_area()
+    np.testing.assert_allclose(a, [0.5] + [(1 + np.sqrt(3) / 2) / 3] * 3, rtol=1e-15)
+
+
+def test_fixed_barrier_derivatives():
+    from sbs_amd import fixed as F
+    s = mesh.cube_scene(2, pin_top=False)
+    # every vertex stays outside the bodies (jitter 1e-3) and some are within dhat
+    bodies = [F.Plane((0, -2e-3, 0)), F.Sphere((0.1, 0.1, -0.053), radius=0.05),
+              F.Cylinder((0.25, 0.0, 0.1), (90, 0, 0), (0.045, 1, 0.045))]
+    o = fixed_setup(s, bodies, dt=0.01, dhat=1e-2)
+    X = s.X.copy()
+    hit = 0
+    for v in range(s.nv):
+        e, g, H = o.fixed_vertex(X, v)
+        if e == 0.0:
+            continue
+        hit += 1
+        x = X.ravel().copy()
+        h = 1e-8
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[3 * v + k] += h
+            xm[3 * v + k] -= h
+            num = (o.fixed_vertex(xp, v)[0] - o.fixed_vertex(xm, v)[0]) / (2 * h)
+            assert np.isclose(g[k], num, rtol=1e-5, atol=1e-9 * np.abs(g).max()), (v, k)
+        np.testing.assert_allclose(H, H.T, atol=1e-12 * np.abs(H).max())
+        assert np.linalg.eigvalsh(H).min() >= 1e-6 - 1e-12 * np.abs(H).max()   # made PD: clamped at 1e-6
+    assert hit >= 4
+    # a bottom vertex sees only the floor (normal +y): its Hessian is the exact
+    # second derivative along y, and 1e-6 (the clamp) across
+    v = int(np.argmin(X[:, 1]))
+    _, g0, H = o.fixed_vertex(X, v)
+    xp = X.ravel().copy()
+    xp[3 * v + 1] += 1e-10
+    _, g1, _ = o.fixed_vertex(xp, v)
+    assert np.isclose(H[1, 1], (g1[1] - g0[1]) / 1e-10, rtol=1e-4)
+    np.testing.assert_allclose([H[0, 0], H[2, 2]], [1e-6, 1e-6], rtol=1e-6)
+
+
+def test_fixed_step_closed_form():
+    from sbs_amd import fixed as F
+    X = np.array([[0.0, 1.0, 0.0], [3.0, 5.0, 0.0], [10.0, 7.0, 0.0]])
+    o = O.IpcOracle(X, np.zeros_like(X), np.zeros((0, 4)), np.zeros((0, 3)), [], [1.0] * 3, [], [], np.zeros(3))
+    bodies = [F.Plane((0, 0, 0)), F.Sphere((0, 5, 0), radius=1.0), F.Cylinder((10, 0, 3), (0, 0, 0), (1, 1, 1))]
+    types = [b.type for b in bodies]
+    models = np.stack([np.asarray(b.model, np.float32).T.reshape(-1) for b in bodies])
+    o.set_fixed(types, O.fixed_params(types, models, [b.radius for b in bodies]))
+    # vertex 0 falls onto the plane: x - alpha p reaches y = 0 at alpha = 1/2
+    p = np.array([[0, 2.0, 0], [0, 0, 0], [0, 0, 0]])
+    assert np.isclose(o.fixed_step(X, p), 0.45)
+    # vertex 1 moves towards the sphere: reaches its surface after 2 of 4 units
+    p = np.array([[0, 0, 0], [4.0, 0, 0], [0, 0, 0]])
+    assert np.isclose(o.fixed_step(X, p), 0.9 * 0.5)
+    # vertex 2 moves towards the y-axis cylinder (radial distance 3) with an
+    # axial part that does not change the hit: 2 of 3 radial units
+    p = np.array([[0, 0, 0], [0, 0, 0], [0, -5.0, -3.0]])
+    assert np.isclose(o.fixed_step(X, p), 0.9 * 2 / 3)
+    # moving away from everything, or on a ray that misses the sphere: 1
+    p = np.array([[0, -1.0, 0], [-1.0, 0, 5.0], [0, 0, -1.0]])
+    assert o.fixed_step(X, p) == 1.0
+
+
+def test_cube_rests_on_a_floor():
+    from sbs_amd import fixed as F
+    s = mesh.cube_scene(2, pin_top=False)
+    X0 = s.X.copy()
+    X0[:, 1] += 3e-3 - X0[:, 1].min()                 # 3 mm above the floor
+    s.X[:] = X0
+    o = fixed_setup(s, [F.Plane((0, 0, 0))], dt=0.01, dhat=1e-2, tol=1e-3)
+    for _ in range(15):
+        assert o.step()
+        X, _ = o.get()
+        assert X[:, 1].min() > 0.0                     # never through the floor
+    assert X[:, 1].min() < 3e-3                        # it came down onto it
+    e, g = o.energy(X, grad=True)
+    assert np.isfinite(e)
