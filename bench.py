@@ -48,13 +48,17 @@ def local_step_bytes(nt: int, nv: int) -> int:
 
 
 def pcg_iteration_bytes(nnz_off: int, nv: int) -> int:
-    # one PCG iteration (one launch): off-diagonal values + u32 columns, the
-    # diagonal and D^-1, read and write x, p, r, w, s (12-byte vertex vectors)
-    return 8 * nnz_off + 128 * nv
+    # the work unit of one Jacobi-PCG iteration as a CSR implementation moves
+    # it: matrix values + u32 columns of the scalar matrix (nnz_s = N_v +
+    # off-diagonal entries), row pointers, D^-1, and 11 passes over 12-byte
+    # vertex vectors (SpMV p -> q: 2; x += alpha p: 3; r -= alpha q: 3;
+    # p = D^-1 r + beta p: 3): 8 nnz_s + 4 (N_v + 1) + 4 N_v + 132 N_v
+    nnz_s = nnz_off + nv
+    return 8 * nnz_s + 4 * (nv + 1) + 4 * nv + 132 * nv
 
 
 TRAFFIC_FILE = "profiles/pd_step_64cube_fp32_kernels.json"
-_TRAFFIC_KERNELS = {"local_step": ["local_chunk_kernel<float>"],
+_TRAFFIC_KERNELS = {"local_step": ["local_chunk_kernel<float, true>", "local_chunk_kernel<float, false>"],
                     "pcg_iteration": ["pcg_iter_halo_kernel<float, false>", "pcg_iter_halo_kernel<float, true>"],
                     "pcg_iteration_gather": ["pcg_iter_kernel<float, false>", "pcg_iter_kernel<float, true>"]}
 

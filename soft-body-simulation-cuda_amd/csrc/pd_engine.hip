@@ -339,93 +339,6 @@ __global__ void begin_step_kernel(int nv, const T* __restrict__ X, const T* __re
     else st3(prev, v, (T)0, (T)0, (T)0);
 }
 
-// ---------------------------------------------------------------------------
-// computeLocal (src/simulation/solver/projective/pdUtil.cu:97-145), one
-// workgroup per chunk of kChunkTets Morton-ordered tets:
-//   1. the chunk's vertices (a sorted list) are staged from q into LDS, with
-//      the chunk's incidence list (per vertex, the (tet, corner) pairs that
-//      touch it, in tet order);
-//   2. one thread per tet: F = Ds Dm^-1, 3x3 SVD in registers, R (or R - F),
-//      the 4 corner forces of w R Dm^-T G^T written to LDS (plain stores);
-//   3. one thread per (vertex, component) sums its incidences in list order
-//      and writes the chunk's partial force for that vertex at the vertex's
-//      slot of the vertex-major partial array.
-// No atomics anywhere, so the result is deterministic; float atomics on LDS
-// serialise on the shared corners of neighbouring tets and were the
-// bottleneck of the scatter form.
-template <class T>
-__global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
-    int nt, const uint2* __restrict__ lidx, const int* __restrict__ chunk_vptr,
-    const uint32_t* __restrict__ chunk_verts, const uint32_t* __restrict__ entry_pos,
-    const uint16_t* __restrict__ inc_codes,
-    const uint16_t* __restrict__ inc_end, const T* __restrict__ dminv, const T* __restrict__ w,
-    const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip,
-    double* __restrict__ zero_slots, int zero_n) {
-    if (skip && *skip) return;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    if (blockIdx.x == 0 && zero_slots)  // the PCG solve that follows accumulates into these
-        for (int i = threadIdx.x; i < zero_n; i += kChunkTets) zero_slots[i] = 0.0;
-    const unsigned c = xcd_block(blockIdx.x, gridDim.x);
-    const int e0 = chunk_vptr[c], nu = chunk_vptr[c + 1] - e0;
-    T* fs = reinterpret_cast<T*>(smem_raw);                    // [12][kChunkTets] corner forces
-    T* qs = fs + 12 * kChunkTets;                              // [nu][3] staged q
-    uint16_t* codes = reinterpret_cast<uint16_t*>(qs + 3 * (size_t)nu);  // [4 kChunkTets] tet<<2 | corner
-    uint16_t* ends = codes + 4 * kChunkTets;                   // [nu] end of each vertex's incidences
-    for (int i = threadIdx.x; i < nu; i += kChunkTets) {
-        const uint32_t g = chunk_verts[e0 + i];
-        T x, y, z;
-        ld3(q, g, x, y, z);
-        qs[3 * i] = x; qs[3 * i + 1] = y; qs[3 * i + 2] = z;
-        ends[i] = inc_end[e0 + i];
-    }
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(inc_codes) + (size_t)c * (2 * kChunkTets);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(codes);
-        for (int k = threadIdx.x; k < 2 * kChunkTets; k += kChunkTets) dst[k] = src[k];
-    }
-    __syncthreads();
-    const unsigned t = c * kChunkTets + threadIdx.x;
-    if (t < (unsigned)nt) {
-        const uint2 li = lidx[t];
-        const unsigned l0 = li.x & 0xffffu, l1 = li.x >> 16, l2 = li.y & 0xffffu, l3 = li.y >> 16;
-        T m[9];
-        load_dminv(dminv, nt, t, m);
-        const T ww = w[t];
-        T F[3][3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const T x0 = qs[3 * l0 + r];
-            const T d0 = qs[3 * l1 + r] - x0, d1 = qs[3 * l2 + r] - x0, d2 = qs[3 * l3 + r] - x0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) F[r][k] = d0 * m[k] + d1 * m[3 + k] + d2 * m[6 + k];
-        }
-        T P[3][3];
-        pd_projection(F, residual_form != 0, P);
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            // column j of w P Dm^-T is w P (row j of Dm^-1); corner 0 gets minus their sum
-            const T f1 = ww * (P[r][0] * m[0] + P[r][1] * m[1] + P[r][2] * m[2]);
-            const T f2 = ww * (P[r][0] * m[3] + P[r][1] * m[4] + P[r][2] * m[5]);
-            const T f3 = ww * (P[r][0] * m[6] + P[r][1] * m[7] + P[r][2] * m[8]);
-            fs[(0 * 3 + r) * kChunkTets + threadIdx.x] = -(f1 + f2 + f3);
-            fs[(1 * 3 + r) * kChunkTets + threadIdx.x] = f1;
-            fs[(2 * 3 + r) * kChunkTets + threadIdx.x] = f2;
-            fs[(3 * 3 + r) * kChunkTets + threadIdx.x] = f3;
-        }
-    }
-    __syncthreads();
-    for (int wi = threadIdx.x; wi < 3 * nu; wi += kChunkTets) {
-        const int i = wi / 3, r = wi - 3 * i;
-        const int k1 = ends[i];
-        T acc = 0;
-        for (int k = i ? ends[i - 1] : 0; k < k1; ++k) {
-            const unsigned code = codes[k];
-            acc += fs[((code & 3u) * 3 + r) * kChunkTets + (code >> 2)];
-        }
-        partial[3 * (size_t)entry_pos[e0 + i] + r] = acc;  // vertex-major: the consumers read contiguously
-    }
-}
-
 // Right-hand side of one vertex: addM_h2Sn (pdUtil.cu:168-178) plus the
 // local-step partials of every chunk touching it, with the pinned overwrite
 // of computeDBCLocal (pdUtil.cu:147-166).
@@ -888,6 +801,132 @@ __global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, i
             a.prev[i] = s;
             a.sn[i] = nx;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// computeLocal (src/simulation/solver/projective/pdUtil.cu:97-145), one
+// workgroup per chunk of kChunkTets Morton-ordered tets:
+//   1. the chunk's vertices (a sorted list) are staged from q into LDS, with
+//      the chunk's incidence list (per vertex, the (tet, corner) pairs that
+//      touch it, in tet order);
+//   2. one thread per tet: F = Ds Dm^-1, 3x3 SVD in registers, R (or R - F),
+//      the 4 corner forces of w R Dm^-T G^T written to LDS (plain stores);
+//   3. one thread per (vertex, component) sums its incidences in list order
+//      and writes the chunk's partial force for that vertex at the vertex's
+//      slot of the vertex-major partial array.
+// No atomics anywhere, so the result is deterministic; float atomics on LDS
+// serialise on the shared corners of neighbouring tets and were the
+// bottleneck of the scatter form.
+//
+// With kFold the kernel also finishes the previous PCG solve (instead of a
+// finish launch of its own): block 0 evaluates that solve's last stop test and
+// bookkeeping (pcg_close) before it clears the reduction slots, and q is not
+// read but formed at staging from that solve's iterate x (internal row order)
+// by the Chebyshev recurrence (pdSolver.cu:196-205),
+//   q = (0.9 (x - sn) + sn - prev) omega + prev,
+// which every chunk holding the vertex evaluates identically (explicit fmas);
+// the vertex's first entry writes q to sn_out and sn to prev_out (double
+// buffers: other chunks still read sn_in / prev_in).
+template <class T> struct FoldIn {
+    const T* x;           // the previous solve's iterate, internal row order
+    const uint32_t* rows; // per chunk entry: internal row of its vertex, bit 31 on the vertex's first entry
+    const T* sn_in;
+    const T* prev_in;
+    T* sn_out;
+    T* prev_out;
+    T omega;
+};
+
+template <class T, bool kFold>
+__global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
+    int nt, const uint2* __restrict__ lidx, const int* __restrict__ chunk_vptr,
+    const uint32_t* __restrict__ chunk_verts, const uint32_t* __restrict__ entry_pos,
+    const uint16_t* __restrict__ inc_codes,
+    const uint16_t* __restrict__ inc_end, const T* __restrict__ dminv, const T* __restrict__ w,
+    const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip,
+    double* __restrict__ zero_slots, int zero_n, FoldIn<T> fold, PcgArgs<T> close, int close_L) {
+    if (skip && *skip) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    if (blockIdx.x == 0 && zero_slots) {  // the PCG solve that follows accumulates into these
+        if (kFold) {
+            pcg_close(close, close_L);     // reads the previous solve's last slot first
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < zero_n; i += kChunkTets) zero_slots[i] = 0.0;
+    }
+    const unsigned c = xcd_block(blockIdx.x, gridDim.x);
+    const int e0 = chunk_vptr[c], nu = chunk_vptr[c + 1] - e0;
+    T* fs = reinterpret_cast<T*>(smem_raw);                    // [12][kChunkTets] corner forces
+    T* qs = fs + 12 * kChunkTets;                              // [nu][3] staged q
+    uint16_t* codes = reinterpret_cast<uint16_t*>(qs + 3 * (size_t)nu);  // [4 kChunkTets] tet<<2 | corner
+    uint16_t* ends = codes + 4 * kChunkTets;                   // [nu] end of each vertex's incidences
+    for (int i = threadIdx.x; i < nu; i += kChunkTets) {
+        const uint32_t g = chunk_verts[e0 + i];
+        T x, y, z;
+        if (kFold) {
+            const uint32_t rw = fold.rows[e0 + i];
+            T v[3], sv[3], pv[3];
+            ld3(fold.x, rw & 0x7fffffffu, v[0], v[1], v[2]);
+            ld3(fold.sn_in, g, sv[0], sv[1], sv[2]);
+            ld3(fold.prev_in, g, pv[0], pv[1], pv[2]);
+            for (int k = 0; k < 3; ++k) v[k] = fma(fma((T)0.9, v[k] - sv[k], sv[k]) - pv[k], fold.omega, pv[k]);
+            if (rw >> 31) {
+                st3(fold.sn_out, g, v[0], v[1], v[2]);
+                st3(fold.prev_out, g, sv[0], sv[1], sv[2]);
+            }
+            x = v[0]; y = v[1]; z = v[2];
+        } else {
+            ld3(q, g, x, y, z);
+        }
+        qs[3 * i] = x; qs[3 * i + 1] = y; qs[3 * i + 2] = z;
+        ends[i] = inc_end[e0 + i];
+    }
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(inc_codes) + (size_t)c * (2 * kChunkTets);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(codes);
+        for (int k = threadIdx.x; k < 2 * kChunkTets; k += kChunkTets) dst[k] = src[k];
+    }
+    __syncthreads();
+    const unsigned t = c * kChunkTets + threadIdx.x;
+    if (t < (unsigned)nt) {
+        const uint2 li = lidx[t];
+        const unsigned l0 = li.x & 0xffffu, l1 = li.x >> 16, l2 = li.y & 0xffffu, l3 = li.y >> 16;
+        T m[9];
+        load_dminv(dminv, nt, t, m);
+        const T ww = w[t];
+        T F[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const T x0 = qs[3 * l0 + r];
+            const T d0 = qs[3 * l1 + r] - x0, d1 = qs[3 * l2 + r] - x0, d2 = qs[3 * l3 + r] - x0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) F[r][k] = d0 * m[k] + d1 * m[3 + k] + d2 * m[6 + k];
+        }
+        T P[3][3];
+        pd_projection(F, residual_form != 0, P);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            // column j of w P Dm^-T is w P (row j of Dm^-1); corner 0 gets minus their sum
+            const T f1 = ww * (P[r][0] * m[0] + P[r][1] * m[1] + P[r][2] * m[2]);
+            const T f2 = ww * (P[r][0] * m[3] + P[r][1] * m[4] + P[r][2] * m[5]);
+            const T f3 = ww * (P[r][0] * m[6] + P[r][1] * m[7] + P[r][2] * m[8]);
+            fs[(0 * 3 + r) * kChunkTets + threadIdx.x] = -(f1 + f2 + f3);
+            fs[(1 * 3 + r) * kChunkTets + threadIdx.x] = f1;
+            fs[(2 * 3 + r) * kChunkTets + threadIdx.x] = f2;
+            fs[(3 * 3 + r) * kChunkTets + threadIdx.x] = f3;
+        }
+    }
+    __syncthreads();
+    for (int wi = threadIdx.x; wi < 3 * nu; wi += kChunkTets) {
+        const int i = wi / 3, r = wi - 3 * i;
+        const int k1 = ends[i];
+        T acc = 0;
+        for (int k = i ? ends[i - 1] : 0; k < k1; ++k) {
+            const unsigned code = codes[k];
+            acc += fs[((code & 3u) * 3 + r) * kChunkTets + (code >> 2)];
+        }
+        partial[3 * (size_t)entry_pos[e0 + i] + r] = acc;  // vertex-major: the consumers read contiguously
     }
 }
 
@@ -1379,6 +1418,10 @@ template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h
     for (size_t e = 0; e < nentries_; ++e) vent[fill[cverts[e]]++] = (uint32_t)e;
     std::vector<uint32_t> epos(nentries_);  // entry -> its slot in vertex-major order
     for (size_t j = 0; j < nentries_; ++j) epos[vent[j]] = (uint32_t)j;
+    host_cverts_ = cverts;
+    host_first_.assign(nentries_, 0);
+    for (int v = 0; v < nv_; ++v)
+        if (vptr[v + 1] > vptr[v]) host_first_[vent[vptr[v]]] = 1;
     tet_perm_d_.alloc(nt_);
     tet_lidx_.alloc(nt_);
     chunk_vptr_.alloc(nchunks_ + 1);
@@ -1495,6 +1538,15 @@ void PdEngine<T>::build_halo(const std::vector<int>& rowstart, const std::vector
     h_val_.alloc(std::max<size_t>(1, src.size()));
     cg_x_.alloc(3 * (size_t)nv_);
     for (int k = 0; k < 2; ++k) cg_plane_[k].alloc(9 * (size_t)nv_);
+    {   // the folded finish reads x by internal row and writes sn/prev once per vertex
+        std::vector<uint32_t> rows(nentries_);
+        for (size_t e = 0; e < nentries_; ++e)
+            rows[e] = h.row_of[host_cverts_[e]] | (host_first_[e] ? 0x80000000u : 0u);
+        chunk_rows_.alloc(std::max<size_t>(1, nentries_));
+        SBS_HIP(hipMemcpy(chunk_rows_.get(), rows.data(), nentries_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+        sn_alt_.alloc(3 * (size_t)nv_);
+        prev_alt_.alloc(3 * (size_t)nv_);
+    }
     SBS_HIP(hipMemcpy(h_grp_.get(), grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(h_halo_.get(), h.halo.data(), h.halo.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(h_halo_v_.get(), h.halo_vertex.data(), h.halo_vertex.size() * sizeof(uint32_t),
@@ -1613,10 +1665,33 @@ template <class T> void PdEngine<T>::launch_local(const T* q, bool residual_form
     SBS_REQUIRE(lds <= 64 * 1024, "local step: chunk vertex list exceeds LDS");
     tick(kKLocal, true);
     // block 0 also clears the reduction slots of the global step that follows
-    hipLaunchKernelGGL(local_chunk_kernel<T>, dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_, tet_lidx_.get(),
-                       chunk_vptr_.get(), chunk_verts_.get(), entry_pos_.get(), inc_codes_.get(), inc_end_.get(),
-                       dminv_.get(), w_.get(),
-                       q, partial_f_.get(), (int)residual_form, skip, slots_.get(), kNumSlots * kSlotSize);
+    hipLaunchKernelGGL((local_chunk_kernel<T, false>), dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_,
+                       tet_lidx_.get(), chunk_vptr_.get(), chunk_verts_.get(), entry_pos_.get(), inc_codes_.get(),
+                       inc_end_.get(), dminv_.get(), w_.get(), q, partial_f_.get(), (int)residual_form, skip,
+                       slots_.get(), kNumSlots * kSlotSize, FoldIn<T>{}, PcgArgs<T>{}, 0);
+    tick(kKLocal, false);
+    SBS_HIP(hipGetLastError());
+}
+
+template <class T>
+void PdEngine<T>::launch_local_fold(const T* x_int, const T* sn_in, const T* prev_in, T* sn_out, T* prev_out,
+                                    T omega, int close_L, const PdParams& params) {
+    const size_t lds = local_lds_bytes();
+    SBS_REQUIRE(lds <= 64 * 1024, "local step: chunk vertex list exceeds LDS");
+    SBS_REQUIRE(close_L >= 1, "local step: folded finish without a solve");
+    FoldIn<T> f{x_int, chunk_rows_.get(), sn_in, prev_in, sn_out, prev_out, omega};
+    PcgArgs<T> c{};  // what the stop test of the previous solve reads
+    c.nv = nv_;
+    c.slots = slots_.get();
+    c.ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
+    c.max_iter = params.pcg_max_iter;
+    c.tol2 = params.pcg_tol * params.pcg_tol;
+    c.floor2 = params.pcg_floor * params.pcg_floor;
+    tick(kKLocal, true);
+    hipLaunchKernelGGL((local_chunk_kernel<T, true>), dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_,
+                       tet_lidx_.get(), chunk_vptr_.get(), chunk_verts_.get(), entry_pos_.get(), inc_codes_.get(),
+                       inc_end_.get(), dminv_.get(), w_.get(), (const T*)nullptr, partial_f_.get(), 0,
+                       (const int*)nullptr, slots_.get(), kNumSlots * kSlotSize, f, c, close_L);
     tick(kKLocal, false);
     SBS_HIP(hipGetLastError());
 }
@@ -1628,7 +1703,7 @@ template <class T> void PdEngine<T>::launch_local(const T* q, bool residual_form
 // Returns L.
 template <class T>
 int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool cheb, T omega, const PdParams& params,
-                            const int* skip, int solve_index, bool exact) {
+                            const int* skip, int solve_index, bool exact, T* sn, T* prev, bool defer_finish) {
     PcgArgs<T> a{};
     a.nv = nv_;
     a.gathered = gathered;
@@ -1647,8 +1722,8 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
         a.rec[k] = cg_rec_[k].get();
     }
     a.p = cg_p_.get();
-    a.sn = sn_.get();
-    a.prev = prev_.get();
+    a.sn = sn ? sn : sn_.get();
+    a.prev = prev ? prev : prev_.get();
     a.omega = omega;
     a.slots = slots_.get();
     a.ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
@@ -1706,7 +1781,8 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
         }
         if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L));
     }
-    if (halo) hipLaunchKernelGGL(pcg_finish_halo_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
+    if (defer_finish) SBS_REQUIRE(halo && cheb, "PCG: only the halo Chebyshev finish can be deferred");
+    else if (halo) hipLaunchKernelGGL(pcg_finish_halo_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
     else hipLaunchKernelGGL(pcg_finish_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
     tick(kKPcg, false);
     SBS_HIP(hipGetLastError());
@@ -1784,9 +1860,26 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     T omega = 1;
     const T rho = (T)params.rho;
     const int* skip = cheb ? nullptr : &ctl->outer_done;
+    // PCG + Chebyshev on the halo kernels: the finish of solve i (its stop test
+    // and the Chebyshev recurrence) runs inside the local step of iteration
+    // i+1, which forms q from solve i's iterate; sn/prev then alternate
+    // between two buffers.  The last solve keeps its finish launch.
+    const bool fold = pcg && cheb && params.pcg_halo && halo_groups_ > 0 && chunk_rows_.get() != nullptr;
+    T* sn_cur = sn_.get();
+    T* prev_cur = prev_.get();
+    T* sn_nxt = sn_alt_.get();
+    T* prev_nxt = prev_alt_.get();
+    int open_L = 0;  // launches of the solve whose finish is pending (0: none)
     for (int i = 0; i < params.num_iterations; ++i) {
         if (perf_) SBS_HIP(hipEventRecord(ev[0], stream_));
-        launch_local(sn_.get(), jacobi, skip);
+        if (open_L > 0) {
+            launch_local_fold(cg_x_.get(), sn_cur, prev_cur, sn_nxt, prev_nxt, omega, open_L, params);
+            std::swap(sn_cur, sn_nxt);
+            std::swap(prev_cur, prev_nxt);
+            open_L = 0;
+        } else {
+            launch_local(sn_cur, jacobi, skip);
+        }
         if (perf_) SBS_HIP(hipEventRecord(ev[1], stream_));
         if (cheb) {
             if (i <= 10) omega = 1;
@@ -1801,7 +1894,10 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
         } else if (pcg) {
             // PCG warm-started from the current iterate, then the Chebyshev
             // recurrence, or computeError and the early exit
-            launch_pcg(1, nullptr, sn_.get(), next_.get(), cheb, omega, params, skip, i, !params.pcg_adaptive);
+            const bool defer = fold && i + 1 < params.num_iterations;
+            const int L = launch_pcg(1, nullptr, sn_cur, next_.get(), cheb, omega, params, skip, i,
+                                     !params.pcg_adaptive, sn_cur, prev_cur, defer);
+            if (defer) open_L = L;
             if (!cheb) {
                 hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_,
                                    (const T*)next_.get(), sn_.get(), prev_.get(), slots_.get(), skip);
@@ -1832,7 +1928,7 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
         }
     }
     const bool collide = params.handle_collision != 0;
-    hipLaunchKernelGGL(end_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(), (T)1 / dt,
+    hipLaunchKernelGGL(end_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_cur, (T)1 / dt,
                        data.XTilde, data.V, collide ? (T*)nullptr : data.X, data.moreDBC);
     SBS_HIP(hipGetLastError());
     if (collide) {

@@ -170,9 +170,16 @@ private:
                     const std::vector<int>& sptr);
     void assemble(const PdParams& params);
     void launch_local(const T* q, bool residual_form, const int* skip);
+    // the local step that also finishes the previous PCG solve (its stop test
+    // and the Chebyshev recurrence from sn_in/prev_in into sn_out/prev_out)
+    void launch_local_fold(const T* x_int, const T* sn_in, const T* prev_in, T* sn_out, T* prev_out, T omega,
+                           int close_L, const PdParams& params);
     size_t local_lds_bytes() const;
+    // sn/prev: the Chebyshev state the finish updates (null: sn_, prev_);
+    // defer_finish: launch no finish (the next local step does it)
     int launch_pcg(int gathered, const T* b, const T* x0, T* x, bool cheb, T omega, const PdParams& params,
-                   const int* skip, int solve_index, bool exact);
+                   const int* skip, int solve_index, bool exact, T* sn = nullptr, T* prev = nullptr,
+                   bool defer_finish = false);
     void reset_caps_if_changed(const PdParams& p);
     static int cap_margin(int used);
     bool is_direct() const;
@@ -199,6 +206,10 @@ private:
     int nchunks_ = 0, chunk_max_verts_ = 0;
     DevBuf<int> chunk_vptr_;            // nchunks+1: first entry of each chunk
     DevBuf<uint32_t> chunk_verts_;      // global vertex of each entry
+    DevBuf<uint32_t> chunk_rows_;       // internal (halo) row of each entry's vertex, bit 31: its first entry
+    std::vector<uint32_t> host_cverts_;
+    std::vector<unsigned char> host_first_;  // entry is the first of its vertex (vertex-major order)
+    DevBuf<T> sn_alt_, prev_alt_;       // second Chebyshev buffers of the folded finish
     DevBuf<int> vent_ptr_;              // N_v+1: entries of each vertex
     DevBuf<uint32_t> entry_pos_;        // entry -> slot of its partial in vertex-major order
     size_t nentries_ = 0;
