@@ -1779,7 +1779,7 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
             // st[(L-1)&1] is the state launch L-1 saw: done there means no later iteration ran
             if ((skip && hc->outer_done) || hc->st[(L - 1) & 1].done) break;
         }
-        if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L));
+        if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L, 2));
     }
     if (defer_finish) SBS_REQUIRE(halo && cheb, "PCG: only the halo Chebyshev finish can be deferred");
     else if (halo) hipLaunchKernelGGL(pcg_finish_halo_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
@@ -1790,7 +1790,12 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
 }
 
 // Iterations of headroom over the last observed count of a solve.
-template <class T> int PdEngine<T>::cap_margin(int used) { return 2 + used / 8; }
+// Launches beyond the larger of the last two observed counts: one lets the
+// launch after the last iteration see the solve converged; the last change of
+// the index's count (capped at 3) is added, so a count that moves gets room
+// and a steady one costs a single extra launch; long solves get one more per
+// 16 iterations.
+template <class T> int PdEngine<T>::cap_margin(int used, int moved) { return 1 + std::min(moved, 3) + used / 16; }
 
 template <class T> void PdEngine<T>::fold_slot(int slot) {
     if (!step_pending_[slot]) return;
@@ -1802,7 +1807,7 @@ template <class T> void PdEngine<T>::fold_slot(int slot) {
     pcg_iters_total_ += c[0];
     pcg_solves_total_ += c[3];
     // learned launch caps: follow the larger of the last two observed counts of
-    // each solve index; a solve cut by its cap doubles that cap
+    // each solve index; a solve cut by its cap raises that cap by half
     if (step_caps_[slot] != cap_epoch_) return;  // counters of other settings
     const int* log = c + 6;
     const int nsol = std::min(std::min(c[3], kMaxSolveLog), (int)cap_.size());
@@ -1810,10 +1815,12 @@ template <class T> void PdEngine<T>::fold_slot(int slot) {
         const int v = log[i];
         if (v < 0) {
             const int its = -v - 1;
-            if (its < cap_max_iter_) cap_[i] = std::min(cap_max_iter_, std::max(2 * cap_[i], its + 8));
+            if (its < cap_max_iter_) cap_[i] = std::min(cap_max_iter_, std::max(cap_[i] + cap_[i] / 2, its + 4));
             seen_[i] = std::max(seen_[i], its);
+            moved_[i] = 3;
         } else {
-            cap_[i] = std::min(cap_max_iter_, std::max(v, seen_[i]) + cap_margin(std::max(v, seen_[i])));
+            moved_[i] = seen_[i] > 0 ? std::abs(v - seen_[i]) : 2;
+            cap_[i] = std::min(cap_max_iter_, std::max(v, seen_[i]) + cap_margin(std::max(v, seen_[i]), moved_[i]));
             seen_[i] = v;
         }
     }
@@ -1986,6 +1993,7 @@ template <class T> void PdEngine<T>::reset_caps_if_changed(const PdParams& p) {
     if (same) return;
     cap_.assign(p.num_iterations, 0);
     seen_.assign(p.num_iterations, 0);
+    moved_.assign(p.num_iterations, 2);
     cap_max_iter_ = p.pcg_max_iter;
     cap_tol_ = p.pcg_tol;
     cap_floor_ = p.pcg_floor;
