@@ -1779,7 +1779,7 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
             // st[(L-1)&1] is the state launch L-1 saw: done there means no later iteration ran
             if ((skip && hc->outer_done) || hc->st[(L - 1) & 1].done) break;
         }
-        if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L, 2));
+        if (learned) cap_[solve_index] = std::min(max_iter, L + cap_margin(L, 0, 0));
     }
     if (defer_finish) SBS_REQUIRE(halo && cheb, "PCG: only the halo Chebyshev finish can be deferred");
     else if (halo) hipLaunchKernelGGL(pcg_finish_halo_kernel<T>, grid, block, 0, stream_, a, L, (int)cheb);
@@ -1790,12 +1790,14 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
 }
 
 // Iterations of headroom over the last observed count of a solve.
-// Launches beyond the larger of the last two observed counts: one lets the
-// launch after the last iteration see the solve converged; the last change of
-// the index's count (capped at 3) is added, so a count that moves gets room
-// and a steady one costs a single extra launch; long solves get one more per
-// 16 iterations.
-template <class T> int PdEngine<T>::cap_margin(int used, int moved) { return 1 + std::min(moved, 3) + used / 16; }
+// Launches beyond the larger of the last two observed counts.  A count seen
+// unchanged three times running gets one (the launch after the last
+// iteration sees the solve converged) plus one per 16 iterations; otherwise
+// 2 + used/8 plus the last change of the count (up to 3), since counts move
+// during transients and the caps follow them with a lag of two steps.
+template <class T> int PdEngine<T>::cap_margin(int used, int moved, int steady) {
+    return steady >= 2 ? 1 + used / 16 : 2 + used / 8 + std::min(moved, 3);
+}
 
 template <class T> void PdEngine<T>::fold_slot(int slot) {
     if (!step_pending_[slot]) return;
@@ -1818,9 +1820,12 @@ template <class T> void PdEngine<T>::fold_slot(int slot) {
             if (its < cap_max_iter_) cap_[i] = std::min(cap_max_iter_, std::max(cap_[i] + cap_[i] / 2, its + 4));
             seen_[i] = std::max(seen_[i], its);
             moved_[i] = 3;
+            steady_[i] = 0;
         } else {
-            moved_[i] = seen_[i] > 0 ? std::abs(v - seen_[i]) : 2;
-            cap_[i] = std::min(cap_max_iter_, std::max(v, seen_[i]) + cap_margin(std::max(v, seen_[i]), moved_[i]));
+            moved_[i] = seen_[i] > 0 ? std::abs(v - seen_[i]) : 0;
+            steady_[i] = (seen_[i] > 0 && v == seen_[i]) ? steady_[i] + 1 : 0;
+            const int used = std::max(v, seen_[i]);
+            cap_[i] = std::min(cap_max_iter_, used + cap_margin(used, moved_[i], steady_[i]));
             seen_[i] = v;
         }
     }
@@ -1993,7 +1998,8 @@ template <class T> void PdEngine<T>::reset_caps_if_changed(const PdParams& p) {
     if (same) return;
     cap_.assign(p.num_iterations, 0);
     seen_.assign(p.num_iterations, 0);
-    moved_.assign(p.num_iterations, 2);
+    moved_.assign(p.num_iterations, 0);
+    steady_.assign(p.num_iterations, 0);
     cap_max_iter_ = p.pcg_max_iter;
     cap_tol_ = p.pcg_tol;
     cap_floor_ = p.pcg_floor;

@@ -181,7 +181,7 @@ private:
                    const int* skip, int solve_index, bool exact, T* sn = nullptr, T* prev = nullptr,
                    bool defer_finish = false);
     void reset_caps_if_changed(const PdParams& p);
-    static int cap_margin(int used, int moved);
+    static int cap_margin(int used, int moved, int steady);
     bool is_direct() const;
     void direct_factor(const PdParams& params);
     void direct_solve(const T* b, T* x, const PdParams& params);
@@ -243,7 +243,9 @@ private:
     DevBuf<unsigned char> ctl_;         // device control block
     void* host_ctl_ = nullptr;          // pinned copy of the control block (host checks)
     // learned PCG launch caps per solve index, for one set of PCG settings
-    std::vector<int> cap_, seen_, moved_;  // per solve index: launch cap, last count, last change of the count
+    // per solve index: launch cap, last count, its last change, how many
+    // observations in a row repeated it
+    std::vector<int> cap_, seen_, moved_, steady_;
     int cap_max_iter_ = 0, cap_cheb_ = -1;
     double cap_tol_ = -1, cap_floor_ = -1, cap_dt_ = -1;
     long long cap_epoch_ = 0;
