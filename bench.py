@@ -28,6 +28,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "soft-body-simulation-cuda_amd"))
 
 from sbs_amd import mesh  # noqa: E402
+from sbs_amd.capi import Stream  # noqa: E402
 from sbs_amd.pd import PdSolver, SolverData, SolverParams, SolverType  # noqa: E402
 
 METRIC = "PD steps/sec at 64^3-hex cube vs plain FP32 at stated error; HBM GB/s"
@@ -78,11 +79,15 @@ def measured_traffic(kernel: str):
     return int(sum(p["mean_mb"] * p["dispatches"] for p in parts) / n * 1e6)
 
 
-def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int, halo: bool = True):
+def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int, halo: bool = True,
+            graph: bool = False):
     data = SolverData(scene, precision)
     pd = PdSolver(128, data)
     pd.SetGlobalSolver(SolverType.PCG)
+    stream = Stream()            # the solver's own stream (graph replay needs a non-null one)
+    pd.set_stream(stream)
     p = workload_params(precision == 64, halo)
+    p.use_graph = graph
     for _ in range(warmup):
         pd.Update(data, p)
     pd.synchronize()
@@ -110,9 +115,12 @@ def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int, h
     out = dict(seconds=t1 - t0, X=X, stats=st1, kernel_times=kt, timing=tst, nnz_off=int(st1.nnz_offdiag),
                pcg_iters=int(st1.pcg_iterations_total - st0.pcg_iterations_total),
                pcg_solves=int(st1.pcg_solves_total - st0.pcg_solves_total),
-               unconverged=int(st1.pcg_cap_hits - st0.pcg_cap_hits))
+               unconverged=int(st1.pcg_cap_hits - st0.pcg_cap_hits),
+               graph_replays=int(st1.graph_replays - st0.graph_replays),
+               graph_captures=int(st1.graph_captures - st0.graph_captures))
     pd.close()
     data.free()
+    stream.close()
     return out
 
 
@@ -348,6 +356,7 @@ def main():
                          "batched: 1024 uneven bodies in one engine vs per-body engines; "
                          "ipc: the FP64 IPC step on two 16^3 cubes in contact")
     ap.add_argument("--per-body-sample", type=int, default=16, help="bodies timed as separate engines (batched)")
+    ap.add_argument("--graph", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--pcg-kernels", default="halo", choices=["halo", "gather"],
                     help="PCG kernel family: halo-staged (LDS) or per-row neighbour gathers")
     args = ap.parse_args()
@@ -368,7 +377,8 @@ def main():
     scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
     print(f"# scene: {scene.nv} vertices, {scene.nt} tets", file=sys.stderr, flush=True)
     halo = args.pcg_kernels == "halo"
-    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=args.timing_steps, halo=halo)
+    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=args.timing_steps, halo=halo,
+                graph=args.graph)
     print(f"# gpu fp32: {args.steps} steps in {g['seconds']:.3f} s", file=sys.stderr, flush=True)
     sps = args.steps / g["seconds"]
 
@@ -422,7 +432,8 @@ def main():
         "config": {"workload": f"{args.n}^3-hex cube PD step (Jacobi-PCG + Chebyshev, 20 it/step)",
                    "verts": scene.nv, "tets": scene.nt, "nnz_scalar": g["nnz_off"] + scene.nv,
                    "iterations_per_step": 20, "dt": 1.0 / 60.0, "pcg_tol": 1e-6, "pcg_floor": 1e-8,
-                   "pcg_kernels": args.pcg_kernels, "halo_groups": int(g["stats"].halo_groups)},
+                   "pcg_kernels": args.pcg_kernels, "halo_groups": int(g["stats"].halo_groups),
+                   "graph_replays": g["graph_replays"], "graph_captures": g["graph_captures"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "error_vs_fp64": err,

@@ -103,6 +103,12 @@ typedef struct SbsSolverParams {
     int pcg_halo;         /* PCG kernels: 1 halo-staged (LDS), 0 per-row gathers */
     double mu_n;          /* fixed-body response: normal restitution (muN)  */
     double mu_t;          /* fixed-body response: friction (muT)            */
+    int use_graph;        /* 1: on a non-null stream, capture the step's launches once into a
+                             HIP graph and replay it while launch sequence, arrays and
+                             parameters stay the same (Jacobi, or adaptive PCG with learned
+                             caps; no collision handling, perf or kernel timing).  On MI355X
+                             a replay costs what the eager launches cost (64^3 cube: 248 vs
+                             256 steps/s; 8^3: 660 vs 654), so 0 is the usual choice */
 } SbsSolverParams;
 
 typedef struct SbsPd SbsPd;
@@ -116,6 +122,8 @@ typedef struct SbsPdStats {
     long long pcg_iterations_total; /* PCG iterations over all completed steps  */
     long long pcg_solves_total;     /* PCG solves over all completed steps      */
     int halo_groups;                /* workgroups of the halo-staged PCG (0: mesh not tiled) */
+    long long graph_captures;       /* steps captured into a HIP graph          */
+    long long graph_replays;        /* steps run by replaying a captured graph  */
 } SbsPdStats;
 
 const char* sbs_last_error(void);
@@ -361,6 +369,11 @@ int sbs_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int sbs_memset(void* dst, int value, size_t bytes);
 int sbs_device_synchronize(void);
 int sbs_device_count(int* count);
+/* A non-blocking HIP stream at the default priority, for callers without one
+ * (a PD solver replays its step as a captured HIP graph only on a non-null
+ * stream, see SbsSolverParams.use_graph). */
+int sbs_stream_create(void** stream);
+int sbs_stream_destroy(void* stream);
 
 #ifdef __cplusplus
 }

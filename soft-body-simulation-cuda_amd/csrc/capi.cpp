@@ -113,6 +113,7 @@ sbs::PdParams to_params(const SbsSolverParams* p) {
     o.pcg_halo = p->pcg_halo;
     o.mu_n = p->mu_n;
     o.mu_t = p->mu_t;
+    o.use_graph = p->use_graph;
     return o;
 }
 
@@ -217,6 +218,8 @@ int sbs_pd_stats(SbsPd* pd, SbsPdStats* out) {
             out->pcg_iterations_total = e.pcg_iterations_total();
             out->pcg_solves_total = e.pcg_solves_total();
             out->halo_groups = e.halo_groups();
+            out->graph_captures = e.graph_captures();
+            out->graph_replays = e.graph_replays();
         });
     });
 }
@@ -326,6 +329,17 @@ int sbs_memset(void* dst, int value, size_t bytes) {
 }
 int sbs_device_synchronize(void) {
     return guarded([&] { SBS_HIP(hipDeviceSynchronize()); });
+}
+int sbs_stream_create(void** stream) {
+    return guarded([&] {
+        SBS_REQUIRE(stream != nullptr, "sbs_stream_create: null argument");
+        hipStream_t s = nullptr;
+        SBS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+    });
+}
+int sbs_stream_destroy(void* stream) {
+    return guarded([&] { if (stream) SBS_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream))); });
 }
 int sbs_pd_dense_info(SbsPd* pd, int* bodies, int* max_body, long long* inverse_bytes, int* last_iterations,
                       double* last_ratio) {

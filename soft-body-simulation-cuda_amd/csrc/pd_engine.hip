@@ -246,50 +246,55 @@ template <class T> __device__ __forceinline__ void load_dminv(const T* __restric
 // Assembly of L_s = sum_t w_t G Dm^-1 Dm^-T G^T (computeSiTSi, pdUtil.cu:9-40),
 // G rows (-1,-1,-1),(1,0,0),(0,1,0),(0,0,1) (pdUtil.cu:16): diagonal into
 // diagL (the reference's matrix_diag), off-diagonal entries into the
-// sliced-ELL values found by a search of the (short) row.  One-time prepare.
+// sliced-ELL values found by a search of the (short) row.  One-time prepare:
+// one thread per row sums its tets' contributions in ascending internal tet
+// order (inc: the row's (tet << 2 | corner) list), so the matrix is the same
+// in every run (float atomics would make it depend on their order).
 template <class T>
-__global__ void __launch_bounds__(kTpb) assemble_kernel(int nt, const uint4* __restrict__ tets,
-                                                        const int* __restrict__ perm, const T* __restrict__ dminv,
-                                                        const T* __restrict__ w, const int* __restrict__ slice_ptr,
-                                                        const int* __restrict__ rowlen,
-                                                        const uint32_t* __restrict__ ecol, T* __restrict__ eval,
-                                                        T* __restrict__ diagL) {
-    const unsigned i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= (unsigned)nt) return;
-    const uint4 tv = tets[perm[i]];
-    const unsigned v[4] = {tv.x, tv.y, tv.z, tv.w};
-    T m[9];
-    load_dminv(dminv, nt, i, m);
-    T g[4][3];
+__global__ void __launch_bounds__(kTpb) assemble_rows_kernel(int nv, int nt, const int* __restrict__ inc_ptr,
+                                                             const uint32_t* __restrict__ inc, const uint4* __restrict__ tets,
+                                                             const int* __restrict__ perm, const T* __restrict__ dminv,
+                                                             const T* __restrict__ w, const int* __restrict__ slice_ptr,
+                                                             const int* __restrict__ rowlen,
+                                                             const uint32_t* __restrict__ ecol, T* __restrict__ eval,
+                                                             T* __restrict__ diagL) {
+    const unsigned row = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (row >= (unsigned)nv) return;
+    const size_t base = (size_t)slice_ptr[row / kSlice] + (row % kSlice);
+    const int len = rowlen[row];
+    T diag = 0;
+    for (int k = inc_ptr[row]; k < inc_ptr[row + 1]; ++k) {
+        const uint32_t code = inc[k];
+        const unsigned i = code >> 2, a = code & 3u;
+        const uint4 tv = tets[perm[i]];
+        const unsigned v[4] = {tv.x, tv.y, tv.z, tv.w};
+        T m[9];
+        load_dminv(dminv, nt, i, m);
+        T g[4][3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        g[0][k] = -(m[k] + m[3 + k] + m[6 + k]);
-        g[1][k] = m[k];
-        g[2][k] = m[3 + k];
-        g[3][k] = m[6 + k];
-    }
-    const T ww = w[i];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const unsigned row = v[a];
-        const size_t base = (size_t)slice_ptr[row / kSlice] + (row % kSlice);
-        const int len = rowlen[row];
-#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            g[0][c] = -(m[c] + m[3 + c] + m[6 + c]);
+            g[1][c] = m[c];
+            g[2][c] = m[3 + c];
+            g[3][c] = m[6 + c];
+        }
+        const T ww = w[i];
         for (int bb = 0; bb < 4; ++bb) {
             const T kab = ww * (g[a][0] * g[bb][0] + g[a][1] * g[bb][1] + g[a][2] * g[bb][2]);
-            if (a == bb) {
-                atomicAdd(&diagL[row], kab);
+            if ((unsigned)bb == a) {
+                diag += kab;
             } else {
                 for (int e = 0; e < len; ++e) {
                     const size_t pos = base + (size_t)e * kSlice;
                     if (ecol[pos] == v[bb]) {
-                        atomicAdd(&eval[pos], kab);
+                        eval[pos] += kab;
                         break;
                     }
                 }
             }
         }
     }
+    diagL[row] = diag;
 }
 
 // c = (m + DBC*weight)/dt^2 (setMDt_2, pdUtil.cu:42-54), diag(A_s) = diagL + c.
@@ -1311,6 +1316,7 @@ template <class T> PdEngine<T>::PdEngine(int threads_per_block, const PdData<T>&
 }
 
 template <class T> PdEngine<T>::~PdEngine() {
+    drop_graph();
     for (auto& e : step_event_)
         if (e) {
             (void)hipEventSynchronize(e);
@@ -1563,10 +1569,24 @@ void PdEngine<T>::build_halo(const std::vector<int>& rowstart, const std::vector
 template <class T> void PdEngine<T>::assemble(const PdParams& params) {
     const T dt2 = (T)params.dt * (T)params.dt;
     SBS_HIP(hipMemsetAsync(eval_.get(), 0, eval_.bytes(), stream_));
-    SBS_HIP(hipMemsetAsync(diagL_.get(), 0, diagL_.bytes(), stream_));
-    hipLaunchKernelGGL(assemble_kernel<T>, dim3(ceil_div(nt_, kTpb)), dim3(kTpb), 0, stream_, nt_,
-                       (const uint4*)data_.Tet, tet_perm_d_.get(), dminv_.get(), w_.get(), slice_ptr_.get(),
-                       rowlen_.get(), ecol_.get(), eval_.get(), diagL_.get());
+    if (!row_inc_ptr_.get()) {  // each row's (internal tet << 2 | corner) list, ascending tet
+        std::vector<int> ptr(nv_ + 1, 0);
+        for (int i = 0; i < nt_; ++i)
+            for (int a = 0; a < 4; ++a) ptr[host_tets_[4 * (size_t)tet_perm_[i] + a] + 1]++;
+        for (int v = 0; v < nv_; ++v) ptr[v + 1] += ptr[v];
+        std::vector<uint32_t> inc(ptr[nv_]);
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int i = 0; i < nt_; ++i)
+            for (int a = 0; a < 4; ++a) inc[fill[host_tets_[4 * (size_t)tet_perm_[i] + a]]++] = ((uint32_t)i << 2) | a;
+        row_inc_ptr_.alloc(nv_ + 1);
+        row_inc_.alloc(std::max<size_t>(1, inc.size()));
+        SBS_HIP(hipMemcpy(row_inc_ptr_.get(), ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        SBS_HIP(hipMemcpy(row_inc_.get(), inc.data(), inc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(assemble_rows_kernel<T>, dim3(ceil_div(nv_, kTpb)), dim3(kTpb), 0, stream_, nv_, nt_,
+                       (const int*)row_inc_ptr_.get(), (const uint32_t*)row_inc_.get(), (const uint4*)data_.Tet,
+                       tet_perm_d_.get(), dminv_.get(), w_.get(), slice_ptr_.get(), rowlen_.get(), ecol_.get(),
+                       eval_.get(), diagL_.get());
     SBS_HIP(hipGetLastError());
     hipLaunchKernelGGL(mass_diag_kernel<T>, dim3(ceil_div(nv_, kTpb)), dim3(kTpb), 0, stream_, nv_, data_.mass,
                        data_.DBC, (T)1e6, dt2, diagL_.get(), c_.get(), diagA_.get(), dinv_.get());
@@ -1851,14 +1871,124 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     // this step's pinned counter slot was last written two steps ago: fold it in
     const int slot = (int)(steps_ & 1);
     fold_slot(slot);
+    const bool pcg = solver_ == GlobalSolver::PCG;
+    if (pcg) reset_caps_if_changed(params);
+    step_caps_[slot] = cap_epoch_;
+    // A step whose launch sequence is known in advance (no host reads of the
+    // device state, no timing events) is captured once into a HIP graph and
+    // replayed while the sequence, its pointers and its parameters stay the
+    // same; the learned PCG caps are part of that key.
+    bool caps_known = true;
+    if (pcg)
+        for (int i = 0; i < params.num_iterations; ++i) caps_known = caps_known && cap_[i] > 0;
+    const bool graph = params.use_graph && stream_ != nullptr && !perf_ && timing_mask_ == 0 && !params.handle_collision &&
+                       (solver_ == GlobalSolver::Jacobi || (pcg && params.pcg_adaptive && caps_known));
+    if (graph) {
+        const std::vector<uint64_t> key = step_key(data, params);
+        if (!graph_exec_ || key != graph_key_ || graph_caps_stale(pcg ? params.num_iterations : 0)) {
+            drop_graph();
+            // the graph keeps its own caps, one launch above the learned ones,
+            // so that counts moving by a few launches replay the same graph
+            std::vector<int> learned;
+            if (pcg) {
+                learned = cap_;
+                graph_caps_.assign(cap_.begin(), cap_.end());
+                for (int& c : graph_caps_) c = std::min(cap_max_iter_, c + 1);
+                cap_ = graph_caps_;
+            }
+            hipGraph_t gr = nullptr;
+            SBS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_step(data, params);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream_, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                if (pcg) cap_ = learned;
+                throw;
+            }
+            if (pcg) cap_ = learned;
+            SBS_HIP(hipStreamEndCapture(stream_, &gr));
+            const hipError_t e = hipGraphInstantiate(&graph_exec_, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) {
+                graph_exec_ = nullptr;
+                throw Error(kHipError, std::string("PdSolver: graph instantiation failed: ") + hipGetErrorString(e));
+            }
+            graph_key_ = key;
+            ++graph_captures_;
+        }
+        SBS_HIP(hipGraphLaunch(graph_exec_, stream_));
+        ++graph_replays_;
+    } else {
+        enqueue_step(data, params);
+    }
+    // counters of the step (PCG iterations, unconverged solves, max per solve,
+    // solves, last, per-solve log) land in this step's pinned slot
+    CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
+    SBS_HIP(hipMemcpyAsync(host_counters_ + kCounterInts * slot, &ctl->pcg_iters, kCounterInts * sizeof(int),
+                           hipMemcpyDeviceToHost, stream_));
+    SBS_HIP(hipEventRecord(step_event_[slot], stream_));
+    step_pending_[slot] = true;
+    ++steps_;
+    if (perf_) SBS_HIP(hipStreamSynchronize(stream_));
+    last_iters_ = params.num_iterations;
+}
+
+// Everything a captured step depends on: the caller's arrays, the parameters,
+// the global solver and the learned launch caps.
+template <class T> std::vector<uint64_t> PdEngine<T>::step_key(const PdData<T>& d, const PdParams& p) const {
+    auto bits = [](double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, sizeof u);
+        return u;
+    };
+    std::vector<uint64_t> k = {(uint64_t)(uintptr_t)d.X, (uint64_t)(uintptr_t)d.V, (uint64_t)(uintptr_t)d.XTilde,
+                               (uint64_t)(uintptr_t)d.DBCX, (uint64_t)(uintptr_t)d.mass, (uint64_t)(uintptr_t)d.DBC,
+                               (uint64_t)(uintptr_t)d.moreDBC, (uint64_t)(uintptr_t)d.fixed,
+                               (uint64_t)(uintptr_t)stream_, (uint64_t)solver_, (uint64_t)p.num_iterations,
+                               bits(p.dt), bits(p.gravity), bits(p.tol), bits(p.rho), (uint64_t)p.pcg_max_iter,
+                               bits(p.pcg_tol), bits(p.pcg_floor), (uint64_t)p.pcg_chebyshev, (uint64_t)p.pcg_halo,
+                               bits(p.mu_n), bits(p.mu_t)};
+    k.push_back((uint64_t)cap_epoch_);
+    return k;
+}
+
+// A captured PCG step is replayed while every solve's last observed count
+// leaves it a spare launch under the graph's cap (count + 1 launches see the
+// solve converge); it is re-captured when a count comes closer than that (or
+// a solve was cut, which raises its count to the cap), or when the graph's
+// caps have carried more than max(2, 1/16 of the learned total) launches
+// above the learned caps for 16 steps running.
+template <class T> bool PdEngine<T>::graph_caps_stale(int n) {
+    if (n == 0) return false;
+    if ((int)graph_caps_.size() != n) return true;
+    int surplus = 0, learned = 0;
+    for (int i = 0; i < n; ++i) {
+        if (seen_[i] + 2 > graph_caps_[i] && graph_caps_[i] < cap_max_iter_) return true;
+        surplus += std::max(graph_caps_[i] - cap_[i], 0);
+        learned += cap_[i];
+    }
+    graph_slack_steps_ = surplus > std::max(2, learned / 16) ? graph_slack_steps_ + 1 : 0;
+    return graph_slack_steps_ >= 16;
+}
+
+template <class T> void PdEngine<T>::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+    graph_key_.clear();
+    graph_caps_.clear();
+    graph_slack_steps_ = 0;
+}
+
+// The launches of one step (begin, the local/global iterations, end, the
+// collision and fixed-body responses), on stream_.
+template <class T> void PdEngine<T>::enqueue_step(PdData<T>& data, const PdParams& params) {
     const T dt = (T)params.dt;
     const unsigned vb = ceil_div(nv_, kTpb);
     CtlBlock* ctl = reinterpret_cast<CtlBlock*>(ctl_.get());
     const bool jacobi = solver_ == GlobalSolver::Jacobi;
     const bool pcg = solver_ == GlobalSolver::PCG;
     const bool cheb = jacobi || (pcg && params.pcg_chebyshev);
-    if (pcg) reset_caps_if_changed(params);
-    step_caps_[slot] = cap_epoch_;
     const RhsGather<T> g{c_.get(), sn_old_.get(), vent_ptr_.get(), partial_f_.get(), data.DBC,
                          data.moreDBC, data.DBCX, pin_weight<T>(params.dt)};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -1976,18 +2106,10 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
             perf_data_[2].ms += ms;
         }
     }
-    // counters of the step (PCG iterations, unconverged solves, max per solve,
-    // solves, last, per-solve log) land in this step's pinned slot
-    SBS_HIP(hipMemcpyAsync(host_counters_ + kCounterInts * slot, &ctl->pcg_iters, kCounterInts * sizeof(int),
-                           hipMemcpyDeviceToHost, stream_));
-    SBS_HIP(hipEventRecord(step_event_[slot], stream_));
-    step_pending_[slot] = true;
-    ++steps_;
     if (perf_) {
         SBS_HIP(hipStreamSynchronize(stream_));
         for (auto& e : ev) (void)hipEventDestroy(e);
     }
-    last_iters_ = params.num_iterations;
 }
 
 // Learned launch caps are per solve index and valid for one set of PCG settings.

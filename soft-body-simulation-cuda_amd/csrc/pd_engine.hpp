@@ -57,6 +57,7 @@ struct PdParams {
     double dense_tol = 1e-10;   // DenseCholesky: max|D^-1 r| <= tol max|D^-1 b| (FP64 refinement)
     int dense_max_iter = 8;     // DenseCholesky: refinement corrections per solve
     int pcg_halo = 1;           // 1: halo-staged PCG kernels (when the tiles could be built); 0: row gathers
+    int use_graph = 0;          // replay a captured HIP graph of the step when its launch sequence is fixed
     double mu_n = 0.5;          // fixed-body response: normal restitution (SolverParams::muN)
     double mu_t = 0.5;          // fixed-body response: friction (SolverParams::muT)
 };
@@ -119,7 +120,12 @@ public:
     void set_global_solver(GlobalSolver s) { solver_ = s; }
     GlobalSolver global_solver() const { return solver_; }
     void set_perf(bool on) { perf_ = on; }
-    void set_stream(hipStream_t s) { stream_ = s; }
+    void set_stream(hipStream_t s) {
+        if (s != stream_) drop_graph();
+        stream_ = s;
+    }
+    long long graph_captures() const { return graph_captures_; }
+    long long graph_replays() const { return graph_replays_; }
     hipStream_t stream() const { return stream_; }
     const std::vector<PerfEntry>& perf() const { return perf_data_; }
     void reset();
@@ -181,6 +187,10 @@ private:
                    const int* skip, int solve_index, bool exact, T* sn = nullptr, T* prev = nullptr,
                    bool defer_finish = false);
     void reset_caps_if_changed(const PdParams& p);
+    void enqueue_step(PdData<T>& data, const PdParams& params);
+    std::vector<uint64_t> step_key(const PdData<T>& d, const PdParams& p) const;
+    void drop_graph();
+    bool graph_caps_stale(int n);
     static int cap_margin(int used, int moved, int steady);
     bool is_direct() const;
     void direct_factor(const PdParams& params);
@@ -207,6 +217,8 @@ private:
     DevBuf<int> chunk_vptr_;            // nchunks+1: first entry of each chunk
     DevBuf<uint32_t> chunk_verts_;      // global vertex of each entry
     DevBuf<uint32_t> chunk_rows_;       // internal (halo) row of each entry's vertex, bit 31: its first entry
+    DevBuf<int> row_inc_ptr_;           // N_v+1: each row's tet incidences (assembly)
+    DevBuf<uint32_t> row_inc_;          // internal tet << 2 | corner, ascending tet per row
     std::vector<uint32_t> host_cverts_;
     std::vector<unsigned char> host_first_;  // entry is the first of its vertex (vertex-major order)
     DevBuf<T> sn_alt_, prev_alt_;       // second Chebyshev buffers of the folded finish
@@ -268,6 +280,11 @@ private:
     long long pcg_iters_total_ = 0, pcg_solves_total_ = 0;
     // kernel timing
     unsigned timing_mask_ = 0;
+    hipGraphExec_t graph_exec_ = nullptr;  // the captured step
+    std::vector<uint64_t> graph_key_;
+    std::vector<int> graph_caps_;          // PCG launch caps the graph was captured with
+    int graph_slack_steps_ = 0;            // consecutive steps its caps sat well above the learned ones
+    long long graph_captures_ = 0, graph_replays_ = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev_[kKCount];
     size_t tev_used_[kKCount] = {};
     KernelTime ktime_[kKCount];

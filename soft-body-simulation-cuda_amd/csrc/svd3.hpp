@@ -125,15 +125,14 @@ template <class T> struct Svd3 {
     template <int I, int J, int N>
     __device__ __forceinline__ static void swap_neg(T (&B)[3][3], T (&V)[3][3], T (&rho)[3]) {
         const bool sw = rho[I] < rho[J];
-        const T sgn_i = (sw && N == I) ? (T)-1 : (T)1;
-        const T sgn_j = (sw && N == J) ? (T)-1 : (T)1;
+        // selects of (possibly negated) values: the same bits as a product by -1/+1
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const T bi = B[r][I], bj = B[r][J], vi = V[r][I], vj = V[r][J];
-            B[r][I] = sgn_i * (sw ? bj : bi);
-            B[r][J] = sgn_j * (sw ? bi : bj);
-            V[r][I] = sgn_i * (sw ? vj : vi);
-            V[r][J] = sgn_j * (sw ? vi : vj);
+            B[r][I] = sw ? (N == I ? -bj : bj) : bi;
+            B[r][J] = sw ? (N == J ? -bi : bi) : bj;
+            V[r][I] = sw ? (N == I ? -vj : vj) : vi;
+            V[r][J] = sw ? (N == J ? -vi : vi) : vj;
         }
         const T ri = rho[I], rj = rho[J];
         rho[I] = sw ? rj : ri;

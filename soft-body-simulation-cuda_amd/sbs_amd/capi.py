@@ -50,14 +50,15 @@ class SbsSolverParams(ctypes.Structure):
                 ("pcg_tol", ctypes.c_double), ("pcg_floor", ctypes.c_double), ("pcg_max_iter", ctypes.c_int),
                 ("pcg_chebyshev", ctypes.c_int), ("pcg_adaptive", ctypes.c_int), ("dense_precision", ctypes.c_int),
                 ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int), ("pcg_halo", ctypes.c_int),
-                ("mu_n", ctypes.c_double), ("mu_t", ctypes.c_double)]
+                ("mu_n", ctypes.c_double), ("mu_t", ctypes.c_double), ("use_graph", ctypes.c_int)]
 
 
 class SbsPdStats(ctypes.Structure):
     _fields_ = [("pcg_iterations_last_step", ctypes.c_int), ("pcg_cap", ctypes.c_int),
                 ("pcg_cap_hits", ctypes.c_longlong), ("nnz_offdiag", ctypes.c_longlong),
                 ("pcg_iterations_total", ctypes.c_longlong), ("pcg_solves_total", ctypes.c_longlong),
-                ("halo_groups", ctypes.c_int)]
+                ("halo_groups", ctypes.c_int), ("graph_captures", ctypes.c_longlong),
+                ("graph_replays", ctypes.c_longlong)]
 
 
 FIXED_SPHERE, FIXED_PLANE, FIXED_CYLINDER = 0, 1, 2
@@ -149,6 +150,8 @@ SIGNATURES = {
     "sbs_memset": (_I, [_VP, _I, _SZ]),
     "sbs_device_synchronize": (_I, []),
     "sbs_device_count": (_I, [ctypes.POINTER(_I)]),
+    "sbs_stream_create": (_I, [ctypes.POINTER(_VP)]),
+    "sbs_stream_destroy": (_I, [_VP]),
 }
 
 _lib = None
@@ -217,5 +220,25 @@ class DeviceArray:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream made through the library (sbs_stream_create)."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().sbs_stream_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().sbs_stream_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -47,6 +47,7 @@ class SolverParams:
     pcg_halo: bool = True         # halo-staged PCG kernels (LDS) where the mesh can be tiled
     muN: float = 0.5              # fixed-body response: normal restitution
     muT: float = 0.5              # fixed-body response: friction
+    use_graph: bool = False       # replay the step as a captured HIP graph (needs a non-null stream)
 
     def to_c(self) -> capi.SbsSolverParams:
         return capi.SbsSolverParams(int(self.numIterations), float(self.dt), float(self.gravity), float(self.tol),
@@ -54,7 +55,7 @@ class SolverParams:
                                     float(self.pcg_floor), int(self.pcg_max_iter), int(self.pcg_chebyshev),
                                     int(self.pcg_adaptive), int(self.dense_precision), float(self.dense_tol),
                                     int(self.dense_max_iter), int(self.pcg_halo), float(self.muN),
-                                    float(self.muT))
+                                    float(self.muT), int(self.use_graph))
 
 
 class SolverData:
@@ -223,6 +224,12 @@ class PdSolver:
         capi.check(self._L.sbs_pd_kernel_time(self.h, capi.KERNELS.index(kernel), None, ctypes.byref(n),
                                               ctypes.byref(ms)))
         return n.value, ms.value
+
+    def set_stream(self, stream) -> None:
+        """Queue the solver's work on `stream` (a capi.Stream, a raw hipStream_t or None)."""
+        ptr = getattr(stream, "ptr", stream)
+        self._stream = stream
+        capi.check(self._L.sbs_pd_set_stream(self.h, ptr))
 
     def synchronize(self) -> None:
         capi.check(self._L.sbs_device_synchronize())

@@ -167,21 +167,25 @@ def test_pcg_multi_body_halo_matches_row_gather_and_oracle(gpu, precision):
     g, sg = run_gpu(s, SolverType.PCG, params_for(SolverType.PCG, pcg_halo=False, **kw), 10, precision)
     r = run_oracle(s, SolverType.PCG, params_for(SolverType.PCG, **kw), 10, precision)
     assert sh.halo_groups >= 3 and sh.pcg_cap_hits == 0 and sg.pcg_cap_hits == 0
-    for out in (h, g):
-        assert rel_err(out["X"], r["X"]) < TOL[precision]
-        if precision == 64:
+    if precision == 64:
+        for out in (h, g):
+            assert rel_err(out["X"], r["X"]) < TOL[precision]
             assert max_rel(out["X"], r["X"]) < TOL[precision]
     assert rel_err(h["X"], g["X"]) < TOL[precision]
     if precision == 32:
-        # the stiff body amplifies FP32 rounding: the FP32 oracle itself sits
-        # ~1.8e-5 (max-norm) from the exact solution of this scene, so the
-        # max-norm bar is "as close to the exact solution as the FP32 oracle"
+        # the stiff bodies amplify FP32 rounding: the FP32 oracle itself sits
+        # ~1e-5 (L2) and ~1.8e-5 (max norm) from the exact solution of this
+        # scene, so two FP32 runs that round differently (here the summation
+        # order of the matrix assembly) can differ by about twice that; the bar
+        # is "as close to the exact solution as the FP32 oracle", in both norms
         px = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_tol=1e-13, pcg_floor=0.0,
                         pcg_max_iter=400)
         exact = run_oracle(s, SolverType.PCG, px, 10, 64)["X"]
-        own = max_rel(r["X"], exact)
+        own_l2, own_max = rel_err(r["X"], exact), max_rel(r["X"], exact)
         for out in (h, g):
-            assert max_rel(out["X"], exact) < 2.0 * own + 1e-6, (max_rel(out["X"], exact), own)
+            assert rel_err(out["X"], exact) < 2.0 * own_l2 + 1e-6, (rel_err(out["X"], exact), own_l2)
+            assert max_rel(out["X"], exact) < 2.0 * own_max + 1e-6, (max_rel(out["X"], exact), own_max)
+            assert rel_err(out["X"], r["X"]) < 3e-5       # and near the FP32 oracle itself
 
 
 def test_pcg_adaptive_launch_cap_matches_exact(gpu):
@@ -255,3 +259,57 @@ def test_full_size_fp32_tracks_fp64(gpu):
     top = s.dbc > 0
     assert np.abs(b["X"][top] - s.X[top]).max() < 1e-4
     assert (b["X"][~top, 1] - s.X[~top, 1]).max() < 1e-6
+
+
+def graph_pair(s, solver, p, nsteps, check):
+    """Two solvers on their own streams, one replaying captured step graphs and
+    one launching eagerly, stepped alternately; `check` compares the positions
+    after every step."""
+    from sbs_amd.capi import Stream
+    runs = []
+    for use_graph in (True, False):
+        data = SolverData(s, 32)
+        pd = PdSolver(128, data)
+        pd.SetGlobalSolver(solver)
+        st = Stream()
+        pd.set_stream(st)
+        runs.append((data, pd, st, SolverParams(**{**p.__dict__, "use_graph": use_graph})))
+    try:
+        for k in range(nsteps):
+            for data, pd, _, pp in runs:
+                pd.Update(data, pp)
+                pd.synchronize()                      # eager work and a sync between the replays
+            check(k, runs[0][0].get("X"), runs[1][0].get("X"))
+        return runs[0][1].stats(), runs[1][1].stats()
+    finally:
+        for data, pd, st, _ in runs:
+            pd.close()
+            data.free()
+            st.close()
+
+
+def test_graph_replay_matches_eager_jacobi(gpu):
+    s = mesh.cube_scene(8)
+
+    def check(k, xg, xe):
+        np.testing.assert_array_equal(xg, xe, err_msg=f"step {k}")   # Jacobi/Chebyshev is deterministic
+
+    sg, se = graph_pair(s, SolverType.Jacobi, params_for(SolverType.Jacobi), 12, check)
+    assert sg.graph_captures == 1 and sg.graph_replays == 12
+    assert se.graph_captures == 0 and se.graph_replays == 0
+
+
+def test_graph_replay_matches_eager_pcg(gpu):
+    s = mesh.cube_scene(16)
+    p = params_for(SolverType.PCG, n_iter=20, pcg_chebyshev=True, pcg_adaptive=True)
+
+    def check(k, xg, xe):
+        # dot products accumulate with atomics: a stop test can flip by one
+        # iteration between runs, so the bar is the PCG tolerance's reach
+        assert rel_err(xg, xe) < 1e-5, (k, rel_err(xg, xe))
+
+    sg, se = graph_pair(s, SolverType.PCG, p, 24, check)
+    assert sg.graph_replays >= 16 and sg.graph_captures >= 1      # once the learned caps are known
+    # caps moving by a launch between steps replay the same graph
+    assert sg.graph_captures * 2 <= sg.graph_replays, (sg.graph_captures, sg.graph_replays)
+    assert sg.pcg_cap_hits == 0 and se.graph_replays == 0
