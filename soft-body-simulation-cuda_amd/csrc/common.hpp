@@ -108,6 +108,25 @@ __device__ __forceinline__ double dpp_f64(double v) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// Sum of a double over each row of 16 lanes with DPP moves (pairs and quads
+// by quad_perm, 8 by row_half_mirror, 16 by row_mirror), returned to every
+// lane of the row.  Every lane of the wavefront must be active.
+__device__ __forceinline__ double row16_total(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return v;
+}
+
+// The double held by lane L (uniform over the wavefront).
+template <int L> __device__ __forceinline__ double readlane_f64(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), L);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Sum of a double over the 64 lanes of a wavefront with DPP moves (VALU only,
 // no LDS round trips), returned to every lane.  The steps: pairs and quads
 // (quad_perm), 8 (row_half_mirror), 16 (row_mirror), then row 1 and 3 take

@@ -112,36 +112,53 @@ template <class T> __device__ __forceinline__ void st_rec(T* __restrict__ base, 
     }
 }
 
-// Sums NQ per-thread values over the block; thread 0 adds them to the
-// block's stripe of `slot` (stripe = block id mod kStripes), so no address
-// sees more than gridDim/kStripes atomics.  Every thread of the block calls it.
+// Sums NQ per-thread values over the block (kTpb threads); thread k < NQ adds
+// quantity k to the block's stripe of `slot` (stripe = block id mod kStripes),
+// so no address sees more than gridDim/kStripes atomics.  Each row of 16
+// lanes reduces with DPP moves and leaves its total in LDS; the 16 row totals
+// are added in row order.  Every thread of the block calls it.
 template <int NQ>
 __device__ __forceinline__ void stripe_add(const double (&vals)[NQ], double* slot) {
-    __shared__ double red[kTpb / 64][NQ];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int kRows = kTpb / 16;
+    __shared__ double red[NQ][kRows];
+    const int lane = threadIdx.x & 63, rid = threadIdx.x >> 4;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-        const double s = wave_total(vals[k]);
-        if (lane == 0) red[wid][k] = s;
+        const double s = row16_total(vals[k]);
+        if ((lane & 15) == 0) red[k][rid] = s;
     }
     __syncthreads();
     if (threadIdx.x < NQ) {
         double s = 0;
-        for (int w = 0; w < kTpb / 64; ++w) s += red[w][threadIdx.x];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) s += red[threadIdx.x][r];
         atomicAdd(&slot[(blockIdx.x % kStripes) * kSlotQ + threadIdx.x], s);
     }
 }
 
 // Totals of the first NQ quantities of a slot (written by an earlier kernel),
-// computed by every wavefront on its own (no LDS, no barrier).
+// computed by every wavefront on its own (no LDS, no barrier): lane 16 r + s
+// adds stripes s and s + 16 of quantity 4 p + r, and one 16-lane reduction
+// per row finishes four quantities at once.  Every lane must be active.
 template <int NQ>
 __device__ __forceinline__ void slot_totals(const double* slot, double (&out)[NQ]) {
-    const int lane = threadIdx.x & 63;
-    double v[NQ];
+    static_assert(kStripes == 32, "two stripes per lane, 16 lanes per quantity");
+    constexpr int kPasses = (NQ + 3) / 4;
+    const int lane = threadIdx.x & 63, r = lane >> 4, s = lane & 15;
+    double v[kPasses];
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) v[k] = lane < kStripes ? slot[lane * kSlotQ + k] : 0.0;
+    for (int p = 0; p < kPasses; ++p) {
+        const int k = 4 * p + r;
+        v[p] = k < NQ ? slot[s * kSlotQ + k] + slot[(s + 16) * kSlotQ + k] : 0.0;
+    }
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) out[k] = wave_total(v[k]);
+    for (int p = 0; p < kPasses; ++p) {
+        const double t = row16_total(v[p]);
+        if (4 * p + 0 < NQ) out[4 * p + 0] = readlane_f64<0>(t);
+        if (4 * p + 1 < NQ) out[4 * p + 1] = readlane_f64<16>(t);
+        if (4 * p + 2 < NQ) out[4 * p + 2] = readlane_f64<32>(t);
+        if (4 * p + 3 < NQ) out[4 * p + 3] = readlane_f64<48>(t);
+    }
 }
 
 // ---------------------------------------------------------------------------
