@@ -969,15 +969,16 @@ template <class T> struct RowBatch {
     unsigned c[kEllBatch];
     T m[kEllBatch];
 };
+// (Uniform base pointers and 32-bit byte offsets: the loads take a scalar base.)
 template <class T>
 __device__ __forceinline__ void row_batch(const HaloView<T>& h, int first, int width, int t, int e0, RowBatch<T>& b) {
-    const uint16_t* __restrict__ c = h.col + first + t;
-    const T* __restrict__ m = h.val + first + t;
+    const char* __restrict__ c = reinterpret_cast<const char*>(h.col + first);
+    const char* __restrict__ m = reinterpret_cast<const char*>(h.val + first);
 #pragma unroll
     for (int j = 0; j < kEllBatch; ++j) {
-        const int e = min(e0 + j, width - 1);
-        b.c[j] = c[(size_t)e * kTpb];
-        const T mv = m[(size_t)e * kTpb];
+        const unsigned o = (unsigned)min(e0 + j, width - 1) * kTpb + (unsigned)t;
+        b.c[j] = *reinterpret_cast<const uint16_t*>(c + o * (unsigned)sizeof(uint16_t));
+        const T mv = *reinterpret_cast<const T*>(m + o * (unsigned)sizeof(T));
         b.m[j] = e0 + j < width ? mv : (T)0;
     }
 }
