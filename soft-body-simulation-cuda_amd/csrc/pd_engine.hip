@@ -488,7 +488,10 @@ __global__ void __launch_bounds__(kTpb) outer_error_kernel(int nv, const T* __re
 // memory latency instead of width.  Positions past the row's width are
 // clamped to its last entry (in bounds) and given the value 0, so every load
 // is unconditional.  f(u, m) accumulates m * x[u] in entry order.
-constexpr int kEllBatch = 8;
+#ifndef SBS_ELL_BATCH
+#define SBS_ELL_BATCH 8
+#endif
+constexpr int kEllBatch = SBS_ELL_BATCH;
 template <int B = kEllBatch, class T, class F>
 __device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const int* __restrict__ slice_w,
                                         const uint32_t* __restrict__ ecol, const T* __restrict__ eval, unsigned v,
@@ -651,60 +654,108 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_kernel(PcgArgs<T> a) {
     stripe_add<9>(vals, iter_slot(a.slots, -1));
 }
 
-// Stop test and scalars of iteration i, identical in every block; block 0
-// records them for the next launch.
-template <class T> __device__ __forceinline__ PcgState pcg_state(const PcgArgs<T>& a, int i) {
-    PcgState st;
+// Scalars of iteration i for a kernel body (uniform over the wavefront).
+struct PcgStep {
+    double alpha[3];
+    double beta[3];
+    int iters, done, unconverged;
+};
+
+// Row r < 3 of the wavefront gets the total of quantity K0 + r of a slot (a
+// lane adds two stripes, then the 16-lane row sum); row 3 gets 0.
+template <int K0> __device__ __forceinline__ double slot_rows3(const double* slot) {
+    static_assert(kStripes == 32, "two stripes per lane, 16 lanes per quantity");
+    const int lane = threadIdx.x & 63, r = lane >> 4, s = lane & 15;
+    const double v = r < 3 ? slot[s * kSlotQ + K0 + r] + slot[(s + 16) * kSlotQ + K0 + r] : 0.0;
+    return row16_total(v);
+}
+
+// Stop test and scalars of iteration i, identical in every wavefront; block 0
+// records the state for the next launch.  Component c of the three
+// right-hand sides is evaluated by row c of the wavefront (row 3 repeats
+// component 2 and is ignored), so a lane does one component's divisions;
+// alpha and beta are then read from lanes 0, 16 and 32.  Every lane of the
+// wavefront must be active.
+template <class T> __device__ __forceinline__ PcgStep pcg_state(const PcgArgs<T>& a, int i) {
+    const int lane = threadIdx.x & 63, row = lane >> 4;
+    const int c = row < 3 ? row : 2;
+    PcgStep out;
+    PcgState* rec = &a.ctl->st[i & 1];
+    const bool writer = blockIdx.x == 0 && threadIdx.x < 64;
+    bool act;
+    double gamma, thr, alpha = 0.0, beta = 0.0;
     if (i == 0) {
-        double t[9], dd[3];
-        slot_totals<9>(iter_slot(a.slots, -1), t);
-        slot_totals<3>(a.slots + kSlotInit * kSlotSize, dd);
-        bool any = false;
-        for (int c = 0; c < 3; ++c) {
-            st.thr[c] = fmax(a.tol2 * t[6 + c], a.floor2 * dd[c]);
-            const bool act = t[6 + c] > st.thr[c];
-            st.active[c] = act;
-            st.gamma[c] = t[c];
-            st.beta[c] = 0.0;
-            st.alpha[c] = (act && t[3 + c] != 0.0) ? t[c] / t[3 + c] : 0.0;
-            any |= act;
-        }
-        st.iters = 0;
-        st.done = !any || a.max_iter <= 0;
-        st.unconverged = any && a.max_iter <= 0;
+        const double* sl = iter_slot(a.slots, -1);
+        const double g = slot_rows3<0>(sl), dl = slot_rows3<3>(sl), uu = slot_rows3<6>(sl);
+        const double dd = slot_rows3<0>(a.slots + kSlotInit * kSlotSize);
+        thr = fmax(a.tol2 * uu, a.floor2 * dd);
+        act = uu > thr;
+        gamma = g;
+        alpha = (act && dl != 0.0) ? g / dl : 0.0;
     } else {
-        st = a.ctl->st[(i - 1) & 1];
-        if (!st.done) {
-            double t[9];
-            slot_totals<9>(iter_slot(a.slots, i - 1), t);
-            bool any = false;
-            for (int c = 0; c < 3; ++c) {
-                const bool act = st.active[c] && t[6 + c] > st.thr[c];
-                double beta = 0.0, alpha = 0.0;
-                if (act && st.gamma[c] != 0.0) {
-                    beta = t[c] / st.gamma[c];
-                    const double den = t[3 + c] - beta * t[c] / st.alpha[c];
-                    alpha = den != 0.0 ? t[c] / den : 0.0;
-                }
-                st.active[c] = act;
-                st.beta[c] = beta;
-                st.alpha[c] = alpha;
-                st.gamma[c] = t[c];
-                any |= act;
+        const PcgState* prev = &a.ctl->st[(i - 1) & 1];
+        if (prev->done) {  // stopped earlier: carry the state forward
+            for (int k = 0; k < 3; ++k) {
+                out.alpha[k] = prev->alpha[k];
+                out.beta[k] = prev->beta[k];
             }
-            st.iters = i;
-            st.done = !any || i >= a.max_iter;
-            st.unconverged = any && i >= a.max_iter;
+            out.iters = prev->iters;
+            out.done = 1;
+            out.unconverged = prev->unconverged;
+            if (writer && threadIdx.x == 0) *rec = *prev;
+            return out;
+        }
+        const double* sl = iter_slot(a.slots, i - 1);
+        const double g = slot_rows3<0>(sl), dl = slot_rows3<3>(sl), uu = slot_rows3<6>(sl);
+        thr = prev->thr[c];
+        act = prev->active[c] && uu > thr;
+        const double pg = prev->gamma[c], pa = prev->alpha[c];
+        if (act && pg != 0.0) {
+            beta = g / pg;
+            const double den = dl - beta * g / pa;
+            alpha = den != 0.0 ? g / den : 0.0;
+        }
+        gamma = g;
+    }
+    const bool any = __ballot(act && row < 3) != 0ull;
+    out.iters = i;
+    out.done = !any || i >= a.max_iter;
+    out.unconverged = any && i >= a.max_iter;
+    out.alpha[0] = readlane_f64<0>(alpha);
+    out.alpha[1] = readlane_f64<16>(alpha);
+    out.alpha[2] = readlane_f64<32>(alpha);
+    out.beta[0] = readlane_f64<0>(beta);
+    out.beta[1] = readlane_f64<16>(beta);
+    out.beta[2] = readlane_f64<32>(beta);
+    if (writer) {
+        if ((lane & 15) == 0 && row < 3) {
+            rec->gamma[c] = gamma;
+            rec->alpha[c] = alpha;
+            rec->beta[c] = beta;
+            rec->thr[c] = thr;
+            rec->active[c] = act;
+        }
+        if (threadIdx.x == 0) {
+            rec->iters = out.iters;
+            rec->done = out.done;
+            rec->unconverged = out.unconverged;
+            rec->pad = 0;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[i & 1] = st;
-    return st;
+    return out;
 }
 
 #ifndef SBS_ITER_BATCH
 #define SBS_ITER_BATCH 4
 #endif
 constexpr int kIterBatch = SBS_ITER_BATCH;  // neighbours in flight per row in the iteration kernel
+// Unroll of the local step's per-vertex incidence sums: the LDS reads of
+// several incidences are in flight together (the sum keeps its order).
+#ifndef SBS_GATHER_UNROLL
+#define SBS_GATHER_UNROLL 4
+#endif
+#define SBS_PRAGMA(x) _Pragma(#x)
+#define SBS_UNROLL(n) SBS_PRAGMA(unroll n)
 
 // One PCG iteration.  Everything that does not depend on alpha_i, beta_i is
 // issued first: the row's own values and the three neighbour sums
@@ -743,7 +794,7 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
         yw0 += md * nb.v[kRecW]; yw1 += md * nb.v[kRecW + 1]; yw2 += md * nb.v[kRecW + 2];
         if (!kFirst) { ys0 += md * nb.v[kRecS]; ys1 += md * nb.v[kRecS + 1]; ys2 += md * nb.v[kRecS + 2]; }
     });
-    const PcgState st = pcg_state(a, i);
+    const PcgStep st = pcg_state(a, i);
     if (st.done) return;
     if (blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
         double* z = iter_slot(a.slots, i + 1);
@@ -783,11 +834,11 @@ __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(5))) 
     stripe_add<9>(vals, iter_slot(a.slots, i));
 }
 
-// Solve log and counters of the control block; st is the state after the
-// L launched iterations.
-__device__ __forceinline__ void pcg_log_solve(CtlBlock* c, const PcgState& st, int L) {
-    const int its = st.done ? st.iters : L;
-    const bool cut = st.unconverged || !st.done;  // max_iter or the launch cap
+// Solve log and counters of the control block from the state after the L
+// launched iterations.
+__device__ __forceinline__ void pcg_log_solve(CtlBlock* c, int iters, int done, int unconverged, int L) {
+    const int its = done ? iters : L;
+    const bool cut = unconverged || !done;  // max_iter or the launch cap
     if (c->solves < kMaxSolveLog) c->log[c->solves] = cut ? -its - 1 : its;
     c->pcg_iters += its;
     c->solves += 1;
@@ -800,10 +851,13 @@ __device__ __forceinline__ void pcg_log_solve(CtlBlock* c, const PcgState& st, i
 // evaluated by the first wavefront of block 0 only).
 template <class T> __device__ __forceinline__ void pcg_close(const PcgArgs<T>& a, int L) {
     if (blockIdx.x != 0 || threadIdx.x >= 64) return;
-    PcgState st;
-    if (L > 0 && a.ctl->st[(L - 1) & 1].done) st = a.ctl->st[(L - 1) & 1];
-    else st = pcg_state(a, L);
-    if (threadIdx.x == 0) pcg_log_solve(a.ctl, st, L);
+    const PcgState* prev = L > 0 ? &a.ctl->st[(L - 1) & 1] : nullptr;
+    if (prev && prev->done) {
+        if (threadIdx.x == 0) pcg_log_solve(a.ctl, prev->iters, 1, prev->unconverged, L);
+        return;
+    }
+    const PcgStep st = pcg_state(a, L);
+    if (threadIdx.x == 0) pcg_log_solve(a.ctl, st.iters, st.done, st.unconverged, L);
 }
 
 // After K(L-1): the stop test of iteration L and bookkeeping (block 0), and with `cheb` the
@@ -944,6 +998,7 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
         const int i = wi / 3, r = wi - 3 * i;
         const int k1 = ends[i];
         T acc = 0;
+        SBS_UNROLL(SBS_GATHER_UNROLL)
         for (int k = i ? ends[i - 1] : 0; k < k1; ++k) {
             const unsigned code = codes[k];
             acc += fs[((code & 3u) * 3 + r) * kChunkTets + (code >> 2)];
@@ -1161,7 +1216,7 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     uint32_t hidx[kHaloPer];
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) hidx[k] = hh[t + k * kTpb];  // in bounds: kHaloStride per workgroup
-    const PcgState st = pcg_state(a, i);
+    const PcgStep st = pcg_state(a, i);
     if (st.done) return;
     T hq[kHaloPer][9];  // R', W', S' of the staged halo rows
 #pragma unroll
