@@ -102,7 +102,8 @@ def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int, h
     kt, tst = {}, None
     if timing_steps > 0:
         # kernel durations from HIP events on the solver's stream, in extra
-        # steps after the timed region (events between launches add gaps)
+        # steps after the timed region (events between launches add gaps: the
+        # PCG iterations of a solve are timed as one run of launches)
         names = ("local_step", "pcg_iteration", "pcg_solve")
         pd.set_kernel_timing(names)
         for _ in range(timing_steps):
@@ -288,8 +289,12 @@ def run_batched_workload(args):
 def run_ipc_workload(args):
     """The FP64 IPC step on two 16^3-hex cubes: the upper one falls 2 mm onto
     the lower one, which is pinned on its bottom face (h = 1/100, dhat = 1e-3,
-    Newton tol 5e-2).  Latency-bound (a host-driven Newton loop with one
-    reduction per PCG iteration), so it reports time per step, not a roofline."""
+    Newton tol 5e-2).  Latency-bound (a host-driven Newton loop; the PCG
+    launches four small kernels per iteration and reads its control word
+    every 16 iterations), so it reports time per step, not a roofline.  The
+    cost grows as the upper cube settles (the barrier stiffness kappa rises),
+    so the rate over the first three steps, the ones the CPU sample runs, is
+    reported as well."""
     from sbs_amd.ipc import IpcData, IpcParams, IpcSolver
     scene = mesh.stack_scene(16, spacing=0.1, gap=2e-3, jitter=5e-4, seed=0)
     print(f"# ipc workload: {scene.nv} vertices, {scene.nt} tets, {scene.ntri} surface triangles", file=sys.stderr,
@@ -299,8 +304,11 @@ def run_ipc_workload(args):
     solver = IpcSolver(data)
     steps = max(args.steps, 1)
     newton, lin, queries, kappa = 0, 0, 0, 0.0
-    for _ in range(args.warmup):
+    first, tf = min(3, args.warmup), time.perf_counter()   # the steps the CPU sample also runs
+    for k in range(args.warmup):
         assert solver.Update(data, p), solver.stats()
+        if k + 1 == first:
+            first_s = time.perf_counter() - tf
     t0 = time.perf_counter()
     for _ in range(steps):
         assert solver.Update(data, p), solver.stats()
@@ -336,7 +344,8 @@ def run_ipc_workload(args):
                        "verts": scene.nv, "tets": scene.nt, "tris": scene.ntri},
             "ipc": {"newton_iterations_per_step": round(newton / steps, 2),
                     "pcg_iterations_per_newton_solve": round(lin / max(newton, 1), 1),
-                    "max_queries": queries, "max_kappa": kappa},
+                    "max_queries": queries, "max_kappa": kappa,
+                    "first_steps": ({"steps": first, "steps_per_s": round(first / first_s, 3)} if first else None)},
             "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
@@ -350,7 +359,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
-    ap.add_argument("--timing-steps", type=int, default=3, help="extra steps timed kernel by kernel")
+    ap.add_argument("--timing-steps", type=int, default=None,
+                    help="steps timed kernel by kernel after the timed region (default: as many as --steps)")
     ap.add_argument("--workload", default="cube64", choices=["cube64", "dense256", "batched", "ipc"],
                     help="cube64: the headline 64^3 cube; dense256: 256 small bodies on the dense solver; "
                          "batched: 1024 uneven bodies in one engine vs per-body engines; "
@@ -377,7 +387,8 @@ def main():
     scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
     print(f"# scene: {scene.nv} vertices, {scene.nt} tets", file=sys.stderr, flush=True)
     halo = args.pcg_kernels == "halo"
-    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=args.timing_steps, halo=halo,
+    timing_steps = args.steps if args.timing_steps is None else args.timing_steps
+    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=timing_steps, halo=halo,
                 graph=args.graph)
     print(f"# gpu fp32: {args.steps} steps in {g['seconds']:.3f} s", file=sys.stderr, flush=True)
     sps = args.steps / g["seconds"]
@@ -416,8 +427,9 @@ def main():
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic,
                     traffic_source=TRAFFIC_FILE if traffic is not None else None,
                     algorithmic_bytes_per_launch=kinfo[dom]["bytes_per_launch"], kernels=kinfo,
-                    timing_sample=f"{tm['steps']} extra steps after the timed region, HIP events around every "
-                                  f"launch, {iters} PCG iterations in {tm['pcg_solves']} solves")
+                    timing_sample=f"{tm['steps']} steps after the timed region (as many as it has), HIP events "
+                                  f"around every local-step launch and around each solve's run of PCG iteration "
+                                  f"launches, {iters} PCG iterations in {tm['pcg_solves']} solves")
 
     cpu = None
     if not args.no_cpu:

@@ -40,8 +40,9 @@ template <int NQ> __device__ void block_sums(const double (&v)[NQ], double* part
     if ((int)threadIdx.x < NQ) parts[NQ * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out[k] = sum over blocks of parts[NQ * b + k], in block order (one block).
-template <int NQ> __global__ void sum_parts_kernel(int nb, const double* parts, double* out) {
+// Sums over blocks of parts[NQ * b + k] in a fixed order (one block of kT
+// threads, every thread calls it); thread 0 gets the totals.
+template <int NQ> __device__ void sum_parts(int nb, const double* parts, double (&tot)[NQ]) {
     __shared__ double red[NQ][kT];
     double acc[NQ];
     for (int k = 0; k < NQ; ++k) acc[k] = 0.0;
@@ -54,7 +55,14 @@ template <int NQ> __global__ void sum_parts_kernel(int nb, const double* parts, 
             for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
         __syncthreads();
     }
-    if ((int)threadIdx.x < NQ) out[threadIdx.x] = red[threadIdx.x][0];
+    for (int k = 0; k < NQ; ++k) tot[k] = red[k][0];
+}
+
+// out[k] = sum over blocks of parts[NQ * b + k], in block order (one block).
+template <int NQ> __global__ void sum_parts_kernel(int nb, const double* parts, double* out) {
+    double tot[NQ];
+    sum_parts<NQ>(nb, parts, tot);
+    if ((int)threadIdx.x < NQ) out[threadIdx.x] = tot[threadIdx.x];
 }
 
 __global__ void max_abs_partial_kernel(size_t n, const double* v, double* parts) {
@@ -598,10 +606,11 @@ __global__ void ipc_fix_kernel(int nv, const unsigned char* fixed, double* g, do
 
 // y = H x, element by element: inertia diagonal (identity on fixed rows),
 // then each tet's and each active query's 12x12 block on free rows/columns.
+// (`stop`: the PCG control word of the solve; a finished solve skips its launches.)
 __global__ void ipc_mv_vertex_kernel(int nv, const double* mass, const unsigned char* fixed, const double* x,
-                                     double* y, const double* hv) {
+                                     double* y, const double* hv, const int* stop) {
     const int v = blockIdx.x * kT + threadIdx.x;
-    if (v >= nv) return;
+    if (v >= nv || *stop) return;
     const double m = fixed[v] ? 1.0 : mass[v];
     const double xv[3] = {x[3 * (size_t)v], x[3 * (size_t)v + 1], x[3 * (size_t)v + 2]};
     for (int k = 0; k < 3; ++k) {
@@ -631,22 +640,6 @@ __device__ __forceinline__ void mv_block(const uint32_t (&vv)[4], const double* 
         }
     }
 }
-__global__ void ipc_mv_tet_kernel(int nt, const uint32_t* tet, const double* ht, const unsigned char* fixed,
-                                  const double* x, double* y) {
-    const int t = blockIdx.x * kT + threadIdx.x;
-    if (t >= nt) return;
-    const uint32_t vv[4] = {tet[4 * (size_t)t], tet[4 * (size_t)t + 1], tet[4 * (size_t)t + 2], tet[4 * (size_t)t + 3]};
-    mv_block(vv, ht + 144 * (size_t)t, fixed, x, y);
-}
-__global__ void ipc_mv_query_kernel(long long nq, const QueryKey* qk, const double* qd, double dhat2, const double* hq,
-                                    const unsigned char* fixed, const double* x, double* y) {
-    const long long i = (long long)blockIdx.x * kT + threadIdx.x;
-    if (i >= nq || qd[i] > dhat2) return;
-    const QueryKey k = qk[i];
-    const uint32_t vv[4] = {k.v0, k.v1, k.v2, k.v3};
-    mv_block(vv, hq + 144 * (size_t)i, fixed, x, y);
-}
-
 // Jacobi-PCG pieces (n = 3 N_v).
 __global__ void ipc_pcg_init_kernel(int n, const double* b, const double* diag, double* x, double* r, double* z,
                                     double* p, double* parts) {
@@ -663,17 +656,67 @@ __global__ void ipc_pcg_init_kernel(int n, const double* b, const double* diag, 
     }
     block_sums<2>(v, parts);
 }
-__global__ void ipc_dot_kernel(int n, const double* a, const double* b, double* parts) {
+__global__ void ipc_dot_kernel(int n, const double* a, const double* b, double* parts, const int* stop) {
+    if (*stop) return;
     const int i = blockIdx.x * kT + threadIdx.x;
     double v[1] = {i < n ? a[i] * b[i] : 0.0};
     block_sums<1>(v, parts);
 }
-__global__ void ipc_pcg_update_kernel(int n, const double* alpha, const double* p, const double* q,
-                                      const double* diag, double* x, double* r, double* z, double* parts) {
+
+// q += the element blocks (blocks [0, ntb)) and the active contact blocks
+// (the blocks after) times x.
+__global__ void ipc_mv_blocks_kernel(int nt, int ntb, const uint32_t* tet, const double* ht, long long nq,
+                                     const QueryKey* qk, const double* qd, double dhat2, const double* hq,
+                                     const unsigned char* fixed, const double* x, double* y, const int* stop) {
+    if (*stop) return;
+    if ((int)blockIdx.x < ntb) {
+        const int t = blockIdx.x * kT + threadIdx.x;
+        if (t >= nt) return;
+        const uint32_t vv[4] = {tet[4 * (size_t)t], tet[4 * (size_t)t + 1], tet[4 * (size_t)t + 2],
+                                tet[4 * (size_t)t + 3]};
+        mv_block(vv, ht + 144 * (size_t)t, fixed, x, y);
+    } else {
+        const long long i = (long long)(blockIdx.x - ntb) * kT + threadIdx.x;
+        if (i >= nq || qd[i] > dhat2) return;
+        const QueryKey k = qk[i];
+        const uint32_t vv[4] = {k.v0, k.v1, k.v2, k.v3};
+        mv_block(vv, hq + 144 * (size_t)i, fixed, x, y);
+    }
+}
+
+// The scalars of a solve stay on the device: out = [rz0, rr, pq, alpha, beta,
+// |b|, rz (even iterations), rz (odd iterations)], ctl = [finished, completed
+// iterations].  Every block of the two kernels below sums the previous
+// launch's block partials itself (the same fixed order in every block, so
+// every block takes the same decision); block 0 records the scalars.
+//
+// Iteration `it`, first half: pq = p.q, alpha = rz / pq, then x += alpha p,
+// r -= alpha q, z = D^-1 r and the partials of r.z, r.r.  H not positive
+// along p finishes the solve with the iterate kept.
+__global__ void ipc_pcg_step_kernel(int n, int nb_dot, const double* dot_parts, int it, const double* p,
+                                    const double* q, const double* diag, double* x, double* r, double* z,
+                                    double* upd_parts, double* out, int* ctl) {
+    if (ctl[0]) return;
+    double tot[1];
+    sum_parts<1>(nb_dot, dot_parts, tot);
+    const double pq = tot[0];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(pq > 0.0)) {
+        if (lead) {
+            out[2] = pq;
+            ctl[1] = it;
+            ctl[0] = 1;
+        }
+        return;
+    }
+    const double a = out[6 + (it & 1)] / pq;
+    if (lead) {
+        out[2] = pq;
+        out[3] = a;
+    }
     const int i = blockIdx.x * kT + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (i < n) {
-        const double a = alpha[0];
         x[i] += a * p[i];
         const double ri = r[i] - a * q[i], zi = ri / diag[i];
         r[i] = ri;
@@ -681,11 +724,53 @@ __global__ void ipc_pcg_update_kernel(int n, const double* alpha, const double* 
         v[0] = ri * zi;
         v[1] = ri * ri;
     }
-    block_sums<2>(v, parts);
+    block_sums<2>(v, upd_parts);
 }
-__global__ void ipc_pcg_dir_kernel(int n, const double* beta, const double* z, double* p) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta[0] * p[i];
+
+// Iteration `it`, second half (one thread per vertex): the iteration is
+// complete; the solve finishes when |r| <= tol |b| or at the iteration
+// limit, else beta = rz' / rz, p = z + beta p, and the next product starts
+// with q = M p (+ the implicit barrier's block on free vertices).
+__global__ void ipc_pcg_dir_kernel(int nv, int nb_upd, const double* upd_parts, int it, int max_iter, double tol,
+                                   const double* z, double* p, const double* mass, const unsigned char* fixed,
+                                   const double* hv, double* q, double* out, int* ctl) {
+    if (ctl[0]) return;
+    double tot[2];
+    sum_parts<2>(nb_upd, upd_parts, tot);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const int done_it = it + 1;
+    if (sqrt(tot[1]) <= tol * out[5] || done_it >= max_iter) {
+        if (lead) {
+            out[1] = tot[1];
+            ctl[1] = done_it;
+            ctl[0] = 1;
+        }
+        return;
+    }
+    const double beta = tot[0] / out[6 + (it & 1)];
+    if (lead) {
+        out[1] = tot[1];
+        out[4] = beta;
+        out[6 + (done_it & 1)] = tot[0];
+        ctl[1] = done_it;
+    }
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v >= nv) return;
+    double pv[3];
+    for (int k = 0; k < 3; ++k) {
+        const size_t j = 3 * (size_t)v + k;
+        pv[k] = z[j] + beta * p[j];
+        p[j] = pv[k];
+    }
+    const double m = fixed[v] ? 1.0 : mass[v];
+    for (int k = 0; k < 3; ++k) {
+        double yk = m * pv[k];
+        if (hv && !fixed[v]) {
+            const double* h = hv + 9 * (size_t)v + 3 * k;
+            yk += h[0] * pv[0] + h[1] * pv[1] + h[2] * pv[2];
+        }
+        q[3 * (size_t)v + k] = yk;
+    }
 }
 
 }  // namespace
@@ -749,7 +834,10 @@ IpcEngine::IpcEngine(const IpcData& data) {
     hv_.alloc(9 * (size_t)nv_);
     parts_.alloc(2 * (size_t)nblk(std::max<size_t>(n3, (size_t)nt_)) + 64);
     out_.alloc(8);
+    pcg_ctl_.alloc(2);
+    pcg_parts_.alloc(3 * (size_t)nblk(n3));
     SBS_HIP(hipHostMalloc((void**)&host_, 8 * sizeof(double), hipHostMallocDefault));
+    SBS_HIP(hipHostMalloc((void**)&host_ctl_, 2 * sizeof(int), hipHostMallocDefault));
     hipLaunchKernelGGL(ipc_precompute_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, data.Tet,
                        (const double*)data.X, dm_.get(), v0_.get());
     hipLaunchKernelGGL(ipc_fixed_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, data.DBC, fixed_.get());
@@ -760,6 +848,7 @@ IpcEngine::IpcEngine(const IpcData& data) {
 
 IpcEngine::~IpcEngine() {
     if (host_) (void)hipHostFree(host_);
+    if (host_ctl_) (void)hipHostFree(host_ctl_);
 }
 
 void IpcEngine::set_stream(hipStream_t s) {
@@ -895,59 +984,61 @@ void IpcEngine::update_kappa(const double* X) {
     kappa_ = gb > 1e-9 ? std::max(100.0, ge / gb) : 100.0;
 }
 
-// Jacobi-PCG on H p = g, element by element; returns the iterations.
+// Jacobi-PCG on H p = g, element by element; returns the iterations.  The
+// scalars stay on the device and an iteration is four launches (products,
+// p.q, the update with alpha, the direction with beta and the next product's
+// diagonal part); the host reads the control word once per kPcgChunk
+// iterations, and launches after the solve finished return at once.
 int IpcEngine::solve() {
+    constexpr int kPcgChunk = 16;
     const int n = 3 * nv_;
-    const unsigned nb = nblk(n);
-    double* parts = parts_.get();
-    double* out = out_.get();  // [0] rz, [1] rr, [2] pq, [3] alpha, [4] beta
+    const unsigned nb = nblk(n), nbv = nblk(nv_), ntb = nblk(nt_);
+    const unsigned nbq = nq_ > 0 ? nblk((size_t)nq_) : 0;
+    double* out = out_.get();  // [0] rz0, [1] rr, [2] pq, [3] alpha, [4] beta, [5] |b|, [6..7] rz by parity
+    int* ctl = pcg_ctl_.get();
+    double* dot_parts = pcg_parts_.get();
+    double* upd_parts = dot_parts + nb;
     hipLaunchKernelGGL(ipc_pcg_init_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)g_.get(),
-                       (const double*)diag_.get(), p_dir_.get(), r_.get(), z_.get(), xtmp_.get(), parts);
-    hipLaunchKernelGGL(sum_parts_kernel<2>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)parts, out);
+                       (const double*)diag_.get(), p_dir_.get(), r_.get(), z_.get(), xtmp_.get(), upd_parts);
+    hipLaunchKernelGGL(sum_parts_kernel<2>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)upd_parts, out);
     SBS_HIP(hipMemcpyAsync(host_, out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
     SBS_HIP(hipStreamSynchronize(stream_));
-    double rz = host_[0];
     const double bn = std::sqrt(host_[1]);
     if (bn == 0.0) return 0;
+    host_[5] = bn;
+    host_[6] = host_[0];
+    host_ctl_[0] = 0;
+    host_ctl_[1] = 0;
+    SBS_HIP(hipMemcpyAsync(out + 5, host_ + 5, 2 * sizeof(double), hipMemcpyHostToDevice, stream_));
+    SBS_HIP(hipMemcpyAsync(ctl, host_ctl_, 2 * sizeof(int), hipMemcpyHostToDevice, stream_));
     double* pvec = xtmp_.get();  // the PCG search direction (x_tmp is free during the solve)
-    int it = 0;
-    for (; it < p_.lin_max_iter;) {
-        hipLaunchKernelGGL(ipc_mv_vertex_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, d_.mass,
-                           (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get(),
-                           hv_live_ ? (const double*)hv_.get() : nullptr);
-        hipLaunchKernelGGL(ipc_mv_tet_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet,
-                           (const double*)ht_.get(), (const unsigned char*)fixed_.get(), (const double*)pvec,
-                           q_.get());
-        if (nq_ > 0)
-            hipLaunchKernelGGL(ipc_mv_query_kernel, dim3(nblk(nq_)), dim3(kT), 0, stream_, nq_, col_->queries(),
+    const double* hv = hv_live_ ? (const double*)hv_.get() : nullptr;
+    hipLaunchKernelGGL(ipc_mv_vertex_kernel, dim3(nbv), dim3(kT), 0, stream_, nv_, d_.mass,
+                       (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get(), hv, (const int*)ctl);
+    for (int launched = 0; launched < p_.lin_max_iter;) {
+        const int chunk = std::min(kPcgChunk, p_.lin_max_iter - launched);
+        for (int j = 0; j < chunk; ++j) {
+            const int it = launched + j;
+            hipLaunchKernelGGL(ipc_mv_blocks_kernel, dim3(ntb + nbq), dim3(kT), 0, stream_, nt_, (int)ntb, d_.Tet,
+                               (const double*)ht_.get(), nq_, nq_ > 0 ? col_->queries() : nullptr,
                                (const double*)qd_.get(), p_.dhat * p_.dhat, (const double*)hq_.get(),
-                               (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get());
-        hipLaunchKernelGGL(ipc_dot_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)pvec,
-                           (const double*)q_.get(), parts);
-        hipLaunchKernelGGL(sum_parts_kernel<1>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)parts,
-                           out + 2);
-        SBS_HIP(hipMemcpyAsync(host_ + 2, out + 2, sizeof(double), hipMemcpyDeviceToHost, stream_));
-        SBS_HIP(hipStreamSynchronize(stream_));
-        const double pq = host_[2];
-        if (!(pq > 0.0)) break;  // not positive definite along p: keep the iterate
-        host_[3] = rz / pq;
-        SBS_HIP(hipMemcpyAsync(out + 3, host_ + 3, sizeof(double), hipMemcpyHostToDevice, stream_));
-        hipLaunchKernelGGL(ipc_pcg_update_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)(out + 3),
-                           (const double*)pvec, (const double*)q_.get(), (const double*)diag_.get(), p_dir_.get(),
-                           r_.get(), z_.get(), parts);
-        hipLaunchKernelGGL(sum_parts_kernel<2>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)parts, out);
-        SBS_HIP(hipMemcpyAsync(host_, out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
-        SBS_HIP(hipStreamSynchronize(stream_));
-        ++it;
-        if (std::sqrt(host_[1]) <= p_.lin_tol * bn) break;
-        host_[4] = host_[0] / rz;
-        rz = host_[0];
-        SBS_HIP(hipMemcpyAsync(out + 4, host_ + 4, sizeof(double), hipMemcpyHostToDevice, stream_));
-        hipLaunchKernelGGL(ipc_pcg_dir_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)(out + 4),
-                           (const double*)z_.get(), pvec);
+                               (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get(), (const int*)ctl);
+            hipLaunchKernelGGL(ipc_dot_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)pvec,
+                               (const double*)q_.get(), dot_parts, (const int*)ctl);
+            hipLaunchKernelGGL(ipc_pcg_step_kernel, dim3(nb), dim3(kT), 0, stream_, n, (int)nb,
+                               (const double*)dot_parts, it, (const double*)pvec, (const double*)q_.get(),
+                               (const double*)diag_.get(), p_dir_.get(), r_.get(), z_.get(), upd_parts, out, ctl);
+            hipLaunchKernelGGL(ipc_pcg_dir_kernel, dim3(nbv), dim3(kT), 0, stream_, nv_, (int)nb,
+                               (const double*)upd_parts, it, p_.lin_max_iter, p_.lin_tol, (const double*)z_.get(),
+                               pvec, d_.mass, (const unsigned char*)fixed_.get(), hv, q_.get(), out, ctl);
+        }
         SBS_HIP(hipGetLastError());
+        launched += chunk;
+        SBS_HIP(hipMemcpyAsync(host_ctl_, ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        SBS_HIP(hipStreamSynchronize(stream_));
+        if (host_ctl_[0]) break;
     }
-    return it;
+    return host_ctl_[1];
 }
 
 // SearchDirection (ipc.cu:230-239): gradient and Hessian at X, elimination,

@@ -111,6 +111,9 @@ private:
     DevBuf<double> hv_;                     // 9 per vertex: the implicit barrier's Hessian blocks
     bool hv_live_ = false;                  // hv_ holds this Newton iteration's blocks
     double* host_ = nullptr;                // pinned scalars
+    DevBuf<int> pcg_ctl_;                   // PCG control: [finished, completed iterations]
+    DevBuf<double> pcg_parts_;              // PCG block partials: p.q, then r.z and r.r
+    int* host_ctl_ = nullptr;               // its pinned copy
 };
 
 }  // namespace sbs

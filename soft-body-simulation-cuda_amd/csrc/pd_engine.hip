@@ -1846,8 +1846,9 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
         hipLaunchKernelGGL(pcg_init1_kernel<T>, grid, block, 0, stream_, a);
         hipLaunchKernelGGL(pcg_init2_kernel<T>, grid, block, 0, stream_, a);
     }
+    // iterations are timed as runs of back-to-back launches (an event pair per
+    // launch would add its own gap to every launch)
     auto iteration = [&](int i) {
-        tick(kKPcgIter, true);
         if (halo) {
             if (i == 0) hipLaunchKernelGGL((pcg_iter_halo_kernel<T, true>), hgrid, block, 0, stream_, a, i);
             else hipLaunchKernelGGL((pcg_iter_halo_kernel<T, false>), hgrid, block, 0, stream_, a, i);
@@ -1855,18 +1856,21 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
             if (i == 0) hipLaunchKernelGGL((pcg_iter_kernel<T, true>), grid, block, 0, stream_, a, i);
             else hipLaunchKernelGGL((pcg_iter_kernel<T, false>), grid, block, 0, stream_, a, i);
         }
-        tick(kKPcgIter, false);
     };
     const bool learned = !exact && solve_index >= 0 && solve_index < (int)cap_.size();
     int L = 0;
     if (learned && cap_[solve_index] > 0) {
         L = std::min(cap_[solve_index], max_iter);
+        tick(kKPcgIter, true);
         for (int i = 0; i < L; ++i) iteration(i);
+        tick(kKPcgIter, false, L);
     } else {
         const CtlBlock* hc = reinterpret_cast<const CtlBlock*>(host_ctl_);
         while (L < max_iter) {
             const int n = std::min(kCheckChunk, max_iter - L);
+            tick(kKPcgIter, true);
             for (int j = 0; j < n; ++j) iteration(L++);
+            tick(kKPcgIter, false, n);
             SBS_HIP(hipMemcpyAsync(host_ctl_, a.ctl, sizeof(CtlBlock), hipMemcpyDeviceToHost, stream_));
             SBS_HIP(hipStreamSynchronize(stream_));
             // st[(L-1)&1] is the state launch L-1 saw: done there means no later iteration ran
@@ -2307,8 +2311,9 @@ template <class T> void PdEngine<T>::set_kernel_timing(unsigned mask) {
     timing_mask_ = mask;
 }
 
-// Records the start (or stop) event of kernel `id` when it is being timed.
-template <class T> void PdEngine<T>::tick(int id, bool start) {
+// Records the start (or stop) event of kernel `id` when it is being timed; a
+// stop event closes `launches` back-to-back launches of that kernel.
+template <class T> void PdEngine<T>::tick(int id, bool start, int launches) {
     if (!(timing_mask_ & (1u << id))) return;
     auto& pool = tev_[id];
     size_t& used = tev_used_[id];
@@ -2318,10 +2323,12 @@ template <class T> void PdEngine<T>::tick(int id, bool start) {
             SBS_HIP(hipEventCreate(&a));
             SBS_HIP(hipEventCreate(&b));
             pool.emplace_back(a, b);
+            tev_n_[id].push_back(0);
         }
         SBS_HIP(hipEventRecord(pool[used].first, stream_));
     } else {
         SBS_HIP(hipEventRecord(pool[used].second, stream_));
+        tev_n_[id][used] = launches;
         ++used;
         if (used >= 4096) flush_timing();  // bound the pool
     }
@@ -2337,7 +2344,7 @@ template <class T> void PdEngine<T>::flush_timing() {
             float ms = 0.f;
             SBS_HIP(hipEventElapsedTime(&ms, tev_[k][i].first, tev_[k][i].second));
             ktime_[k].ms += ms;
-            ktime_[k].launches += 1;
+            ktime_[k].launches += tev_n_[k][i];
         }
         tev_used_[k] = 0;
     }

@@ -98,7 +98,7 @@ enum KernelId : int {
     kKJacobi,
     kKDirect,
     kKRhs,
-    kKPcgIter,  // one PCG iteration (and the stop probe after the last one)
+    kKPcgIter,  // PCG iteration launches (the stop probe and launches after convergence included), timed per run
     kKCount
 };
 const char* kernel_name(int id);
@@ -195,7 +195,7 @@ private:
     bool is_direct() const;
     void direct_factor(const PdParams& params);
     void direct_solve(const T* b, T* x, const PdParams& params);
-    void tick(int id, bool start);
+    void tick(int id, bool start, int launches = 1);  // a stop covers `launches` launches
     void flush_timing();
 
     int tpb_ = 256;
@@ -286,6 +286,7 @@ private:
     int graph_slack_steps_ = 0;            // consecutive steps its caps sat well above the learned ones
     long long graph_captures_ = 0, graph_replays_ = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev_[kKCount];
+    std::vector<int> tev_n_[kKCount];  // launches between each start/stop pair
     size_t tev_used_[kKCount] = {};
     KernelTime ktime_[kKCount];
 };

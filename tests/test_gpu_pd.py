@@ -313,3 +313,36 @@ def test_graph_replay_matches_eager_pcg(gpu):
     # caps moving by a launch between steps replay the same graph
     assert sg.graph_captures * 2 <= sg.graph_replays, (sg.graph_captures, sg.graph_replays)
     assert sg.pcg_cap_hits == 0 and se.graph_replays == 0
+
+
+def test_kernel_timing_counts_launches(gpu):
+    """HIP-event kernel timing: one local step and one solve per outer
+    iteration; the PCG iteration launches (timed per run of launches) cover
+    every iteration that ran plus the stop probes; timing leaves the result
+    unchanged."""
+    s = mesh.cube_scene(12)
+    n_iter, nsteps = 8, 4
+    p = params_for(SolverType.PCG, n_iter=n_iter, pcg_chebyshev=True, pcg_adaptive=True)
+    ref, _ = run_gpu(s, SolverType.PCG, p, nsteps, 32)
+    data = SolverData(s, 32)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(SolverType.PCG)
+    try:
+        pd.set_kernel_timing(("local_step", "pcg_iteration", "pcg_solve"))
+        for _ in range(nsteps):
+            pd.Update(data, p)
+        pd.synchronize()
+        st = pd.stats()
+        nl, ml = pd.kernel_time("local_step")
+        ns, ms = pd.kernel_time("pcg_solve")
+        ni, mi = pd.kernel_time("pcg_iteration")
+        assert nl == nsteps * n_iter and ns == nsteps * n_iter
+        assert st.pcg_solves_total == ns
+        # every solve launches its iterations and, unless its cap cut it, the probe that finds it converged
+        lo = st.pcg_iterations_total + ns - st.pcg_cap_hits
+        assert lo <= ni <= ns * p.pcg_max_iter, (st.pcg_iterations_total, ns, st.pcg_cap_hits, ni)
+        assert ml > 0 and ms > mi > 0
+        assert rel_err(data.get("X"), ref["X"]) < 1e-5
+    finally:
+        pd.close()
+        data.free()
