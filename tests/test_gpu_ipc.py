@@ -169,3 +169,24 @@ def test_cube_falling_on_a_floor_matches_oracle(gpu):
         assert Xg[:, 1].min() < 4e-3                               # it came down onto the floor
     finally:
         solver.close(), data.free(), fb.close()
+
+
+def test_pcg_stops_at_the_iteration_limit(gpu):
+    """The PCG's stop logic lives on the device and the host reads it once per
+    chunk of 16 iterations: a solve limited to lin_max_iter iterations reports
+    exactly that many, inside a chunk, at its end and across a boundary, and
+    the unlimited solve needs more."""
+    s = mesh.stack_scene(4, gap=1.5e-3)
+    counts = {}
+    for cap in (7, 16, 17, 20000):
+        data = IpcData(s)
+        solver = IpcSolver(data)
+        try:
+            solver.Update(data, IpcParams(dt=0.01, dhat=1e-3, tol=1e-3, lin_tol=1e-13, lin_max_iter=cap))
+            counts[cap] = solver.stats()["linear_iterations"]
+        finally:
+            solver.close()
+            data.free()
+    assert counts[20000] > 17, counts
+    for cap in (7, 16, 17):
+        assert counts[cap] == cap, counts
