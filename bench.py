@@ -350,6 +350,104 @@ def run_ipc_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def run_solver_workload(args, solver, n: int, kernel: str, label: str):
+    """The 20-iteration PD step of an n^3 cube (the headline's body and
+    parameters) on another global solver: Jacobi + Chebyshev (the reference's
+    default) or the direct band Cholesky.  The global-step kernel is timed
+    with HIP events in as many steps again after the timed region."""
+    scene = mesh.cube_scene(n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)
+    print(f"# {label}: {scene.nv} vertices, {scene.nt} tets", file=sys.stderr, flush=True)
+    p = workload_params()
+    data = SolverData(scene, 32)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(solver)
+    for _ in range(args.warmup):
+        pd.Update(data, p)
+    pd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pd.Update(data, p)
+    pd.synchronize()
+    el = time.perf_counter() - t0
+    pd.set_kernel_timing(("local_step", kernel))
+    for _ in range(args.steps):
+        pd.Update(data, p)
+    pd.synchronize()
+    st = pd.stats()
+    kinfo = {}
+    for k in ("local_step", kernel):
+        nl, ms = pd.kernel_time(k)
+        kinfo[k] = dict(launches=nl, avg_us=round(ms / max(nl, 1) * 1e3, 2))
+    kinfo["local_step"]["gbs"] = round(local_step_bytes(scene.nt, scene.nv) / (kinfo["local_step"]["avg_us"] * 1e3), 1)
+    roof = None
+    if solver == SolverType.Jacobi:
+        # per vertex: pin flag, partial range, x~, m/h^2, the diagonal, sn and prev
+        # read and written; per (chunk, vertex) partial: 12 bytes
+        jb = 76 * scene.nv + 12 * st.partial_entries
+        kinfo[kernel]["bytes_per_launch"] = jb
+        kinfo[kernel]["gbs"] = round(jb / (kinfo[kernel]["avg_us"] * 1e3), 1)
+        roof = dict(bound="hbm", kernel=kernel, achieved=kinfo[kernel]["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(kinfo[kernel]["gbs"] / HBM_PEAK_GBS, 4))
+    pd.close()
+    data.free()
+    line = {"metric": f"PD steps/sec, {n}^3-hex cube, {label}", "value": round(args.steps / el, 3),
+            "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(1e3 * el / args.steps, 4), "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{n}^3-hex cube PD step ({label}, 20 it/step)", "verts": scene.nv,
+                       "tets": scene.nt}, "kernels": kinfo, "roofline": roof}
+    print(json.dumps(line), flush=True)
+
+
+def run_collide_workload(args):
+    """Two 24^3-hex cubes, the upper one falling onto the lower one (pinned on
+    its bottom face), with the PD update's collision handling (LBVH broad
+    phase over the step's motion, CCD narrow phase, the CCD response) and two
+    fixed bodies (a floor plane and a sphere beside the upper cube): the
+    collision rows of the step, timed per phase by the solver's own events in
+    as many steps again after the timed region."""
+    from sbs_amd import fixed as F
+    n = 24
+    scene = mesh.stack_scene(n, spacing=0.1, gap=2e-3, jitter=5e-4, seed=0)
+    print(f"# collide workload: {scene.nv} vertices, {scene.nt} tets, {scene.ntri} surface triangles",
+          file=sys.stderr, flush=True)
+    p = workload_params()
+    p.handleCollision = True
+    data = SolverData(scene, 32)
+    bodies = F.FixedBodyData([F.Plane(pos=(0.0, -0.05, 0.0)), F.Sphere(pos=(n * 0.1 + 0.35, n * 0.15, n * 0.05),
+                                                                       radius=0.3)])
+    data.set_fixed_bodies(bodies)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(SolverType.PCG)
+    for _ in range(args.warmup):
+        pd.Update(data, p)
+    pd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pd.Update(data, p)
+    pd.synchronize()
+    el = time.perf_counter() - t0
+    pd.SetPerf(True)
+    before = dict(pd.GetPerformanceData())
+    for _ in range(args.steps):
+        pd.Update(data, p)
+    pd.synchronize()
+    after = dict(pd.GetPerformanceData())
+    phases = {k: round((after[k] - before.get(k, 0.0)) / args.steps, 4) for k in after}
+    pd.close()
+    bodies.close()
+    data.free()
+    line = {"metric": "PD steps/sec, two 24^3-hex cubes colliding, with collision handling and fixed bodies",
+            "value": round(args.steps / el, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4), "higher_is_better": True,
+            "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "2 x 24^3-hex cubes, PD (Jacobi-PCG + Chebyshev, 20 it/step), collisions, "
+                                   "a plane and a sphere", "verts": scene.nv, "tets": scene.nt,
+                       "tris": scene.ntri},
+            "phases_ms_per_step": phases}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,10 +459,13 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--timing-steps", type=int, default=None,
                     help="steps timed kernel by kernel after the timed region (default: as many as --steps)")
-    ap.add_argument("--workload", default="cube64", choices=["cube64", "dense256", "batched", "ipc"],
+    ap.add_argument("--workload", default="cube64",
+                    choices=["cube64", "dense256", "batched", "ipc", "jacobi64", "direct14", "collide"],
                     help="cube64: the headline 64^3 cube; dense256: 256 small bodies on the dense solver; "
                          "batched: 1024 uneven bodies in one engine vs per-body engines; "
-                         "ipc: the FP64 IPC step on two 16^3 cubes in contact")
+                         "ipc: the FP64 IPC step on two 16^3 cubes in contact; jacobi64: the 64^3 cube on the "
+                         "Jacobi + Chebyshev global step; direct14: a 14^3 cube on the band Cholesky; collide: "
+                         "two 24^3 cubes colliding, with fixed bodies")
     ap.add_argument("--per-body-sample", type=int, default=16, help="bodies timed as separate engines (batched)")
     ap.add_argument("--graph", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--pcg-kernels", default="halo", choices=["halo", "gather"],
@@ -382,6 +483,16 @@ def main():
         return
     if args.workload == "ipc":
         run_ipc_workload(args)
+        return
+    if args.workload == "jacobi64":
+        run_solver_workload(args, SolverType.Jacobi, 64, "jacobi_chebyshev", "Jacobi + Chebyshev global step")
+        return
+    if args.workload == "direct14":
+        # the band factor of one system is held whole (at most 3,818 vertices in FP32)
+        run_solver_workload(args, SolverType.CuSolverCholesky, 14, "direct_solve", "band Cholesky global step")
+        return
+    if args.workload == "collide":
+        run_collide_workload(args)
         return
 
     scene = mesh.cube_scene(args.n, spacing=0.1, jitter=1e-3, seed=0, E=1e5, nu=0.3, density=1000.0, pin_top=True)

@@ -124,6 +124,7 @@ typedef struct SbsPdStats {
     int halo_groups;                /* workgroups of the halo-staged PCG (0: mesh not tiled) */
     long long graph_captures;       /* steps captured into a HIP graph          */
     long long graph_replays;        /* steps run by replaying a captured graph  */
+    long long partial_entries;      /* (chunk, vertex) pairs of the local step's partial forces */
 } SbsPdStats;
 
 const char* sbs_last_error(void);

@@ -220,6 +220,7 @@ int sbs_pd_stats(SbsPd* pd, SbsPdStats* out) {
             out->halo_groups = e.halo_groups();
             out->graph_captures = e.graph_captures();
             out->graph_replays = e.graph_replays();
+            out->partial_entries = (long long)e.partial_entries();
         });
     });
 }

@@ -126,6 +126,7 @@ public:
     }
     long long graph_captures() const { return graph_captures_; }
     long long graph_replays() const { return graph_replays_; }
+    size_t partial_entries() const { return nentries_; }
     hipStream_t stream() const { return stream_; }
     const std::vector<PerfEntry>& perf() const { return perf_data_; }
     void reset();

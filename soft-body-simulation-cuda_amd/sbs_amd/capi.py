@@ -58,7 +58,7 @@ class SbsPdStats(ctypes.Structure):
                 ("pcg_cap_hits", ctypes.c_longlong), ("nnz_offdiag", ctypes.c_longlong),
                 ("pcg_iterations_total", ctypes.c_longlong), ("pcg_solves_total", ctypes.c_longlong),
                 ("halo_groups", ctypes.c_int), ("graph_captures", ctypes.c_longlong),
-                ("graph_replays", ctypes.c_longlong)]
+                ("graph_replays", ctypes.c_longlong), ("partial_entries", ctypes.c_longlong)]
 
 
 FIXED_SPHERE, FIXED_PLANE, FIXED_CYLINDER = 0, 1, 2
