@@ -326,24 +326,57 @@ template <class T> __device__ __forceinline__ bool overlap(const T* a, const T* 
     return true;
 }
 
-// Every leaf walks the tree from the root and records the ordered pairs
+// Every leaf walks the tree (its walk split over the subtrees `depth` levels
+// below the root, one thread each) and records the ordered pairs
 // (its triangle, hit triangle) that pass the filters of traverseTree
 // (broadphase.cu:331-398): a different triangle, no shared vertex, and a
 // different body when ignore_self.  Pairs past `cap` are counted, not stored.
 template <class T>
-__global__ void traverse_kernel(int n, const uint32_t* order, const uint32_t* tri, const uint32_t* fathers,
+__global__ void traverse_kernel(int n, int depth, const uint32_t* order, const uint32_t* tri, const uint32_t* fathers,
                                 const int* left, const int* right, const T* node_box, int ignore_self, uint2* pairs,
                                 unsigned long long cap, unsigned long long* counters) {
-    const int k = blockIdx.x * kTpbC + threadIdx.x;
-    if (k >= n) return;
+    // thread g = s * n + k: leaf k (Morton order), subtree s of the 2^depth
+    // nodes `depth` levels below the root (consecutive threads take
+    // neighbouring leaves, so a wavefront's walks stay coherent)
+    const unsigned long long g = (unsigned long long)blockIdx.x * kTpbC + threadIdx.x;
+    if (g >= ((unsigned long long)n << depth)) return;
+    const int k = (int)(g % (unsigned long long)n);
+    const unsigned s = (unsigned)(g / (unsigned long long)n);
     const uint32_t ti = order[k];
     T qb[6];
     for (int c = 0; c < 6; ++c) qb[c] = node_box[6 * (size_t)(k + n - 1) + c];
     const uint32_t a0 = tri[3 * (size_t)ti], a1 = tri[3 * (size_t)ti + 1], a2 = tri[3 * (size_t)ti + 2];
     const uint32_t fa = fathers[ti];
+    auto leaf = [&](int m) {  // m overlaps the query box
+        const uint32_t tj = order[m - (n - 1)];
+        if (tj == ti) return;
+        if (ignore_self && fathers[tj] == fa) return;
+        const uint32_t b0 = tri[3 * (size_t)tj], b1 = tri[3 * (size_t)tj + 1], b2 = tri[3 * (size_t)tj + 2];
+        const bool adj = a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
+                         a2 == b1 || a2 == b2;
+        if (adj) return;
+        const unsigned long long slot = atomicAdd(&counters[0], 1ull);
+        if (slot < cap) pairs[slot] = make_uint2(ti, tj);
+    };
+    // descend along the bits of s; a subtree whose box misses the query box
+    // holds no overlapping leaf, and a leaf met above `depth` is taken by the
+    // one thread whose remaining bits are 0
+    int root = 0;
+    for (int l = 0; l < depth; ++l) {
+        const int rest = depth - 1 - l;
+        const int m = ((s >> rest) & 1u) ? right[root] : left[root];
+        T cb[6];
+        for (int e = 0; e < 6; ++e) cb[e] = node_box[6 * (size_t)m + e];
+        if (!overlap(qb, cb)) return;
+        if (m >= n - 1) {
+            if ((s & ((1u << rest) - 1u)) == 0) leaf(m);
+            return;
+        }
+        root = m;
+    }
     int stack[kStack];
     int sp = 0;
-    stack[sp++] = 0;
+    stack[sp++] = root;
     for (int visits = 0; sp > 0; ++visits) {
         if (visits >= n) {  // a tree has n - 1 internal nodes: more visits means a malformed tree
             atomicOr(&counters[1], 2ull);
@@ -357,15 +390,7 @@ __global__ void traverse_kernel(int n, const uint32_t* order, const uint32_t* tr
             for (int e = 0; e < 6; ++e) cb[e] = node_box[6 * (size_t)m + e];
             if (!overlap(qb, cb)) continue;
             if (m >= n - 1) {
-                const uint32_t tj = order[m - (n - 1)];
-                if (tj == ti) continue;
-                if (ignore_self && fathers[tj] == fa) continue;
-                const uint32_t b0 = tri[3 * (size_t)tj], b1 = tri[3 * (size_t)tj + 1], b2 = tri[3 * (size_t)tj + 2];
-                const bool adj = a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
-                                 a2 == b1 || a2 == b2;
-                if (adj) continue;
-                const unsigned long long slot = atomicAdd(&counters[0], 1ull);
-                if (slot < cap) pairs[slot] = make_uint2(ti, tj);
+                leaf(m);
             } else if (sp < kStack) {
                 stack[sp++] = m;
             } else {
@@ -741,10 +766,14 @@ template <class T> void CollisionDetector<T>::broad_phase(const T* X, const T* X
                        (const T*)leaf_box_.get(), (const int*)left_.get(), (const int*)right_.get(),
                        (const int*)parent_.get(), flags_.get(), node_box_.get());
     SBS_HIP(hipGetLastError());
+    // each leaf's walk is split over the 2^depth subtrees below the root, so
+    // that about 2^18 threads walk (one per leaf would leave most CUs idle)
+    int depth = 0;
+    while (depth < 6 && ((unsigned long long)n << depth) < (1ull << 18)) ++depth;
     for (int attempt = 0; attempt < 2; ++attempt) {
         SBS_HIP(hipMemsetAsync(counters_.get(), 0, counters_.bytes(), stream_));
-        hipLaunchKernelGGL(traverse_kernel<T>, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n,
-                           (const uint32_t*)order_.get(), tri_, fathers_, (const int*)left_.get(),
+        hipLaunchKernelGGL(traverse_kernel<T>, dim3(blocks_for((unsigned long long)n << depth)), dim3(kTpbC), 0,
+                           stream_, n, depth, (const uint32_t*)order_.get(), tri_, fathers_, (const int*)left_.get(),
                            (const int*)right_.get(), (const T*)node_box_.get(), (int)ignore_self, pairs_.get(),
                            (unsigned long long)pair_cap_, counters_.get());
         SBS_HIP(hipGetLastError());
