@@ -108,7 +108,10 @@ struct KernelTime {
     double ms = 0.0;
 };
 
-constexpr int kChunkTets = 256;  // tets per local-step chunk (one workgroup)
+#ifndef SBS_CHUNK_TETS
+#define SBS_CHUNK_TETS 512
+#endif
+constexpr int kChunkTets = SBS_CHUNK_TETS;  // tets per local-step chunk (one workgroup)
 constexpr int kMaxSolveLog = 64;  // per-step log of PCG iterations, by solve index
 constexpr int kCounterInts = 6 + kMaxSolveLog;  // step counters copied to the host
 
