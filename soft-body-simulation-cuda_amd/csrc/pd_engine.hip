@@ -17,7 +17,10 @@ namespace sbs {
 
 namespace {
 
-constexpr int kTpb = 256;      // threads per block of the streaming kernels (4 wavefronts)
+#ifndef SBS_TPB
+#define SBS_TPB 256
+#endif
+constexpr int kTpb = SBS_TPB;  // threads per block of the streaming kernels (and rows per halo tile)
 constexpr int kSlice = 64;     // sliced-ELL slice height = one wavefront
 constexpr int kStripes = 32;   // address stripes of each reduction slot
 constexpr int kSlotQ = 16;     // doubles per stripe
@@ -549,7 +552,10 @@ __device__ __forceinline__ void ell_row(const int* __restrict__ slice_ptr, const
 // (internal row i = vertex perm[i]) and grouped into workgroups of up to kTpb
 // consecutive internal rows; a workgroup's halo is the ascending list of the
 // internal rows its rows touch (their own and their neighbours').
-constexpr int kHaloStride = 768;              // halo entries (LDS slots) per workgroup
+#ifndef SBS_HALO_STRIDE
+#define SBS_HALO_STRIDE 768
+#endif
+constexpr int kHaloStride = SBS_HALO_STRIDE;  // halo entries (LDS slots) per workgroup
 constexpr int kHaloPer = kHaloStride / kTpb;  // halo entries staged per thread
 constexpr int kGrpInts = 8;                   // per workgroup: row0, rows, halo size, width, first entry, own0
 template <class T> struct HaloView {
