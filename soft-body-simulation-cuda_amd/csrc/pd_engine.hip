@@ -758,7 +758,7 @@ constexpr int kIterBatch = SBS_ITER_BATCH;  // neighbours in flight per row in t
 // Unroll of the local step's per-vertex incidence sums: the LDS reads of
 // several incidences are in flight together (the sum keeps its order).
 #ifndef SBS_GATHER_UNROLL
-#define SBS_GATHER_UNROLL 4
+#define SBS_GATHER_UNROLL 2
 #endif
 #define SBS_PRAGMA(x) _Pragma(#x)
 #define SBS_UNROLL(n) SBS_PRAGMA(unroll n)
