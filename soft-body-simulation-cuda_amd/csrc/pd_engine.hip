@@ -382,9 +382,13 @@ template <class T> struct RhsGather {
     // A vertex's partials are contiguous (the local step writes them in vertex
     // order); they are loaded kGatherBatch at a time, clamped in bounds.
     __device__ __forceinline__ void at(unsigned v, T& bx, T& by, T& bz) const {
+        at_range(v, vent_ptr[v], vent_ptr[v + 1], bx, by, bz);
+    }
+    // The same with the vertex's partial range [e0, e1) given (a caller that
+    // holds it per internal row need not wait for v to request it).
+    __device__ __forceinline__ void at_range(unsigned v, int e0, int e1, T& bx, T& by, T& bz) const {
         const T pin = dbc[v];
         const T more = more_dbc ? more_dbc[v] : (T)0;
-        const int e0 = vent_ptr[v], e1 = vent_ptr[v + 1];
         T sx, sy, sz;
         ld3(sn_old, v, sx, sy, sz);
         const T cv = c[v];
@@ -566,6 +570,7 @@ template <class T> struct HaloView {
     const uint16_t* col = nullptr;     // entry e of thread t at first + e * kTpb + t: halo-local column
     const T* val = nullptr;            // its value (diagonal included, padding 0)
     const uint32_t* perm = nullptr;    // internal row -> vertex
+    const uint2* vent = nullptr;       // internal row -> its vertex's partial range (null: none)
     T* x = nullptr;                    // the iterate in internal order
 };
 
@@ -1125,8 +1130,14 @@ __global__ void __launch_bounds__(kTpb) pcg_init1_halo_kernel(PcgArgs<T> a) {
     const size_t ir = (size_t)G.row0 + (row ? t : 0);
     const uint32_t v = a.h.perm[ir];
     T b0, b1, b2;
-    if (a.gathered) a.g.at(v, b0, b1, b2);
-    else ld3(a.b, v, b0, b1, b2);
+    if (a.gathered && a.h.vent) {
+        const uint2 rv = a.h.vent[ir];  // requested with v, not after it
+        a.g.at_range(v, (int)rv.x, (int)rv.y, b0, b1, b2);
+    } else if (a.gathered) {
+        a.g.at(v, b0, b1, b2);
+    } else {
+        ld3(a.b, v, b0, b1, b2);
+    }
     T ax0 = 0, ax1 = 0, ax2 = 0, x0 = 0, x1 = 0, x2 = 0;
     if (a.x0) {
         RowBatch<T> bt;
@@ -1504,6 +1515,7 @@ template <class T> void PdEngine<T>::build_chunks(const std::vector<double>& X0h
     std::vector<uint32_t> epos(nentries_);  // entry -> its slot in vertex-major order
     for (size_t j = 0; j < nentries_; ++j) epos[vent[j]] = (uint32_t)j;
     host_cverts_ = cverts;
+    host_vptr_ = vptr;
     host_first_.assign(nentries_, 0);
     for (int v = 0; v < nv_; ++v)
         if (vptr[v + 1] > vptr[v]) host_first_[vent[vptr[v]]] = 1;
@@ -1637,6 +1649,15 @@ void PdEngine<T>::build_halo(const std::vector<int>& rowstart, const std::vector
     SBS_HIP(hipMemcpy(h_halo_v_.get(), h.halo_vertex.data(), h.halo_vertex.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
     SBS_HIP(hipMemcpy(h_perm_.get(), h.row_perm.data(), nv_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if ((int)host_vptr_.size() == nv_ + 1) {
+        std::vector<uint2> rv(nv_);
+        for (int r = 0; r < nv_; ++r) {
+            const uint32_t v = h.row_perm[r];
+            rv[r] = make_uint2((unsigned)host_vptr_[v], (unsigned)host_vptr_[v + 1]);
+        }
+        h_vent_.alloc(nv_);
+        SBS_HIP(hipMemcpy(h_vent_.get(), rv.data(), nv_ * sizeof(uint2), hipMemcpyHostToDevice));
+    }
     if (!src.empty()) {
         SBS_HIP(hipMemcpy(h_col_.get(), h.col.data(), h.col.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
         SBS_HIP(hipMemcpy(h_src_.get(), src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1839,6 +1860,7 @@ int PdEngine<T>::launch_pcg(int gathered, const T* b, const T* x0, T* x, bool ch
         a.h.col = h_col_.get();
         a.h.val = h_val_.get();
         a.h.perm = h_perm_.get();
+        a.h.vent = h_vent_.get();
         a.h.x = cg_x_.get();
         for (int k = 0; k < 2; ++k) a.hrec[k] = cg_plane_[k].get();
     }

@@ -224,6 +224,7 @@ private:
     DevBuf<int> row_inc_ptr_;           // N_v+1: each row's tet incidences (assembly)
     DevBuf<uint32_t> row_inc_;          // internal tet << 2 | corner, ascending tet per row
     std::vector<uint32_t> host_cverts_;
+    std::vector<int> host_vptr_;             // vertex -> its partials [host_vptr_[v], host_vptr_[v+1])
     std::vector<unsigned char> host_first_;  // entry is the first of its vertex (vertex-major order)
     DevBuf<T> sn_alt_, prev_alt_;       // second Chebyshev buffers of the folded finish
     DevBuf<int> vent_ptr_;              // N_v+1: entries of each vertex
@@ -252,6 +253,7 @@ private:
     size_t halo_entries_ = 0;
     DevBuf<int> h_grp_, h_src_;
     DevBuf<uint32_t> h_halo_, h_halo_v_, h_perm_;
+    DevBuf<uint2> h_vent_;                   // internal row -> its vertex's partial range
     DevBuf<uint16_t> h_col_;
     DevBuf<T> h_val_, cg_x_;
     DevBuf<T> cg_plane_[2];             // halo PCG: D^-1-scaled r, w, s planes, double-buffered
