@@ -512,7 +512,8 @@ def main():
         err = dict(rel_l2=float(np.linalg.norm(g["X"] - r["X"]) / np.linalg.norm(r["X"])),
                    max_abs_m=float(np.abs(g["X"] - r["X"]).max()),
                    max_abs_over_max_displacement=float(np.abs(g["X"] - r["X"]).max() / disp),
-                   reference="FP64 run of the same steps on the GPU (FP64 path parity-tested vs the FP64 oracle)")
+                   reference="FP64 run of the same steps on the GPU (FP64 path parity-tested vs the FP64 oracle)",
+                   fp64_steps_per_s=round(args.steps / r["seconds"], 3))
 
     roof = None
     kt, tm = g["kernel_times"], g["timing"]
