@@ -68,12 +68,15 @@ typedef struct SbsSolverData {
     void* V;              /* 3*num_verts: velocities, written by update      */
     void* X0;             /* 3*num_verts: rest positions                      */
     void* XTilde;         /* 3*num_verts: end-of-step positions               */
-    void* DBCX;           /* 3*num_verts: pin targets (set from X0 at prepare) */
+    void* DBCX;           /* 3*num_verts: pin targets (set from X0 at prepare);
+                           * a dragged vertex (moreDBC > 0) is held at its
+                           * DBCX entry, which the caller sets between updates */
     const uint32_t* Tet;  /* 4*num_tets, 16-byte aligned                      */
     const void* mass;     /* num_verts                                        */
     const void* mu;       /* num_tets                                         */
     const void* DBC;      /* num_verts: 1 for pinned vertices, else 0         */
-    const void* moreDBC;  /* num_verts or NULL: extra soft pins (0 = none)    */
+    const void* moreDBC;  /* num_verts or NULL: mouse-drag weights (0 = none):
+                           * c = (m + w)/h^2, s_n = b/w = DBCX, V = 0       */
     void* V0;             /* num_tets or NULL: rest volumes, written at create */
     /* Surface for collision handling (used when handle_collision is set):   */
     int num_tris;

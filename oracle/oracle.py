@@ -69,6 +69,7 @@ def lib(bench: bool = False) -> ctypes.CDLL:
     L.orc_pd_nnz.restype = ctypes.c_int
     L.orc_pd_nnz.argtypes = [ctypes.c_int, ctypes.c_void_p]
     L.orc_pd_export.argtypes = [ctypes.c_int, ctypes.c_void_p, dp, dp, dp, dp, ip, ip, dp]
+    L.orc_pd_set_more_dbc.argtypes = [ctypes.c_int, ctypes.c_void_p, dp, dp]
     L.orc_pd_local.argtypes = [ctypes.c_int, ctypes.c_void_p, dp, dp, ctypes.c_int, ctypes.c_double,
                                ctypes.c_double, dp]
     L.orc_num_threads.restype = ctypes.c_int
@@ -189,6 +190,12 @@ class PdOracle:
         pcg = ctypes.c_int(0)
         it = self.L.orc_pd_step(self.fp64, self.h, ctypes.byref(self.params), n, ctypes.byref(pcg))
         return it, pcg.value
+
+    def set_more_dbc(self, weights, targets):
+        """moreDBC weights (nv) and the targets of the weighted vertices (nv, 3)."""
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1)
+        self.L.orc_pd_set_more_dbc(self.fp64, self.h, _p(w, ctypes.c_double), _p(t, ctypes.c_double))
 
     def get(self, which: str = "X") -> np.ndarray:
         out = np.zeros(3 * self.nv)

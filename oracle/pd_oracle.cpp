@@ -184,6 +184,7 @@ template <class S> struct Pd {
     std::vector<uint32_t> tet;
     std::vector<S> dminv, V0, w;      // 9*nt row-major, nt, nt
     std::vector<S> mass, dbc, c;      // nv
+    std::vector<S> more;              // nv moreDBC weights (empty: all zero)
     std::vector<S> diagL;             // nv: diagonal of L_s (matrix_diag)
     std::vector<int> rowptr, col;     // CSR of A_s = L_s + diag(c)
     std::vector<S> val;
@@ -350,55 +351,99 @@ template <class S> struct Pd {
         }
     }
 
+    // Sums of f(v) (3 components) over all vertices: fixed blocks of
+    // vertices summed in double in parallel, then the block sums added in
+    // block order, so the result does not depend on the thread count.
+    template <class F> void vsum3(F&& f, double out[3]) const {
+        constexpr int kBlocks = 256;
+        double part[kBlocks][3];
+#pragma omp parallel for schedule(static)
+        for (int bk = 0; bk < kBlocks; ++bk) {
+            const int v0 = (int)((long long)nv * bk / kBlocks), v1 = (int)((long long)nv * (bk + 1) / kBlocks);
+            double a[3] = {0, 0, 0};
+            for (int v = v0; v < v1; ++v) f(v, a);
+            for (int k = 0; k < 3; ++k) part[bk][k] = a[k];
+        }
+        for (int k = 0; k < 3; ++k) out[k] = 0;
+        for (int bk = 0; bk < kBlocks; ++bk)
+            for (int k = 0; k < 3; ++k) out[k] += part[bk][k];
+    }
+
     // Jacobi-preconditioned CG on the scalar system, 3 independent RHS,
-    // warm start from x.  Residuals are measured Jacobi-scaled, z = D^-1 r,
-    // which has units of displacement, so the pinned rows (b = x0*1e6/h^2)
-    // do not swamp the norm the way they swamp ||b||.  Component k stops when
-    // ||z_k|| <= max(tol*||z_0||, floor*||D^-1 b||).
+    // warm start from x (pcgJacobi.cu:89-173, with cuBLAS's dot products as
+    // double-accumulated block sums).  Residuals are measured Jacobi-scaled,
+    // z = D^-1 r, which has units of displacement, so the pinned rows
+    // (b = x0*1e6/h^2) do not swamp the norm the way they swamp ||b||.
+    // Component k stops when ||z_k|| <= max(tol*||z_0||, floor*||D^-1 b||).
     int pcg(const S* b, S* x, double tol, double floor_tol, int max_iter) const {
         const size_t n3 = 3 * (size_t)nv;
         std::vector<S> r(n3), z(n3), p(n3), q(n3), dinv(nv);
-        for (int v = 0; v < nv; ++v) dinv[v] = (S)1 / val[std::lower_bound(col.begin() + rowptr[v], col.begin() + rowptr[v + 1], v) - col.begin()];
+#pragma omp parallel for schedule(static)
+        for (int v = 0; v < nv; ++v)
+            dinv[v] = (S)1 / val[std::lower_bound(col.begin() + rowptr[v], col.begin() + rowptr[v + 1], v) - col.begin()];
         spmv3(x, q.data());
-        S rz[3] = {0, 0, 0}, bb[3] = {0, 0, 0}, rr[3] = {0, 0, 0};
-        for (size_t i = 0; i < n3; ++i) {
-            r[i] = b[i] - q[i];
-            z[i] = dinv[i / 3] * r[i];
-            p[i] = z[i];
-            rz[i % 3] += r[i] * z[i];
-            const S db = dinv[i / 3] * b[i];
-            bb[i % 3] += db * db;
-            rr[i % 3] += z[i] * z[i];
-        }
+        double rzd[3], bbd[3], rrd[3];
+        vsum3([&](int v, double* a) {
+            for (int k = 0; k < 3; ++k) {
+                const size_t i = 3 * (size_t)v + k;
+                r[i] = b[i] - q[i];
+                z[i] = dinv[v] * r[i];
+                p[i] = z[i];
+                a[k] += (double)(r[i] * z[i]);
+            }
+        }, rzd);
+        vsum3([&](int v, double* a) {
+            for (int k = 0; k < 3; ++k) {
+                const S db = dinv[v] * b[3 * (size_t)v + k];
+                a[k] += (double)(db * db);
+            }
+        }, bbd);
+        vsum3([&](int v, double* a) {
+            for (int k = 0; k < 3; ++k) a[k] += (double)(z[3 * (size_t)v + k] * z[3 * (size_t)v + k]);
+        }, rrd);
+        S rz[3], rr[3];
         bool active[3];
         S thr[3];
         for (int k = 0; k < 3; ++k) {
-            thr[k] = std::max((S)(tol * tol) * rr[k], (S)(floor_tol * floor_tol) * bb[k]);
+            rz[k] = (S)rzd[k];
+            rr[k] = (S)rrd[k];
+            thr[k] = std::max((S)(tol * tol) * rr[k], (S)(floor_tol * floor_tol) * (S)bbd[k]);
             active[k] = rr[k] > thr[k];
         }
         int it = 0;
         for (; it < max_iter && (active[0] || active[1] || active[2]); ++it) {
             spmv3(p.data(), q.data());
-            S pq[3] = {0, 0, 0};
-            for (size_t i = 0; i < n3; ++i) pq[i % 3] += p[i] * q[i];
+            double pqd[3];
+            vsum3([&](int v, double* a) {
+                for (int k = 0; k < 3; ++k) a[k] += (double)(p[3 * (size_t)v + k] * q[3 * (size_t)v + k]);
+            }, pqd);
             S alpha[3];
-            for (int k = 0; k < 3; ++k) alpha[k] = (active[k] && pq[k] != 0) ? rz[k] / pq[k] : (S)0;
-            S rz_new[3] = {0, 0, 0};
-            for (int k = 0; k < 3; ++k) rr[k] = 0;
-            for (size_t i = 0; i < n3; ++i) {
-                const int k = (int)(i % 3);
-                x[i] += alpha[k] * p[i];
-                r[i] -= alpha[k] * q[i];
-                z[i] = dinv[i / 3] * r[i];
-                rz_new[k] += r[i] * z[i];
-                rr[k] += z[i] * z[i];
-            }
+            for (int k = 0; k < 3; ++k) alpha[k] = (active[k] && pqd[k] != 0) ? rz[k] / (S)pqd[k] : (S)0;
+            double rzn[3], rrn[3];
+            vsum3([&](int v, double* a) {
+                for (int k = 0; k < 3; ++k) {
+                    const size_t i = 3 * (size_t)v + k;
+                    x[i] += alpha[k] * p[i];
+                    r[i] -= alpha[k] * q[i];
+                    z[i] = dinv[v] * r[i];
+                    a[k] += (double)(r[i] * z[i]);
+                }
+            }, rzn);
+            vsum3([&](int v, double* a) {
+                for (int k = 0; k < 3; ++k) a[k] += (double)(z[3 * (size_t)v + k] * z[3 * (size_t)v + k]);
+            }, rrn);
             S beta[3];
             for (int k = 0; k < 3; ++k) {
-                beta[k] = (active[k] && rz[k] != 0) ? rz_new[k] / rz[k] : (S)0;
-                rz[k] = rz_new[k];
+                beta[k] = (active[k] && rz[k] != 0) ? (S)rzn[k] / rz[k] : (S)0;
+                rz[k] = (S)rzn[k];
+                rr[k] = (S)rrn[k];
             }
-            for (size_t i = 0; i < n3; ++i) p[i] = z[i] + beta[i % 3] * p[i];
+#pragma omp parallel for schedule(static)
+            for (int v = 0; v < nv; ++v)
+                for (int k = 0; k < 3; ++k) {
+                    const size_t i = 3 * (size_t)v + k;
+                    p[i] = z[i] + beta[k] * p[i];
+                }
             for (int k = 0; k < 3; ++k) active[k] = active[k] && rr[k] > thr[k];
         }
         return it;
@@ -443,17 +488,27 @@ template <class S> struct Pd {
     }
 
     // One PdSolver::Update without mesh contact (pdSolver.cu:141-208, 227).
+    // moreDBC (the mouse-drag weights, empty: none) follows setMDt_2MoreDBC,
+    // computeSn, computeDBCLocal, getErrorKern and updateVelPos: a vertex with
+    // more > 0 (and not pinned) gets c = (m + more)/h^2, sn = its target
+    // (DBCX, which computeSn sets to the target), b = DBCX * more, no Jacobi
+    // move, and zero velocity.
     int step(const OrcParams& p, int* pcg_iters_total) {
         const S dt = (S)p.dt, dtInv = (S)1 / dt, dt2 = dt * dt, dt2Inv = dtInv * dtInv;
         const size_t n3 = 3 * (size_t)nv;
         std::vector<S> sn(n3), sn_old(n3), prev(n3), next(n3), b(n3), contrib(12 * (size_t)nt);
-        // massDt_2s for non-DBC vertices (setMDt_2MoreDBC; moreDBC = 0 in batch)
-        for (int v = 0; v < nv; ++v)
-            if (dbc[v] == 0) c[v] = mass[v] / dt2;
+        const bool has_more = !more.empty();
+        auto mw = [&](int v) { return has_more ? more[v] : (S)0; };
+#pragma omp parallel for schedule(static)
         for (int v = 0; v < nv; ++v) {
-            const S f[3] = {0, -(S)p.gravity * mass[v], 0};
-            const S inv = (S)1 / c[v];
-            for (int k = 0; k < 3; ++k) sn[3 * v + k] = X[3 * v + k] + dt * V[3 * v + k] + inv * f[k];
+            if (dbc[v] == 0) c[v] = mw(v) > 0 ? (mass[v] + mw(v)) / dt2 : mass[v] / dt2;  // setMDt_2MoreDBC
+            if (mw(v) > 0) {
+                for (int k = 0; k < 3; ++k) sn[3 * v + k] = DBCX[3 * v + k];
+            } else {
+                const S f[3] = {0, -(S)p.gravity * mass[v], 0};
+                const S inv = (S)1 / c[v];
+                for (int k = 0; k < 3; ++k) sn[3 * v + k] = X[3 * v + k] + dt * V[3 * v + k] + inv * f[k];
+            }
         }
         sn_old = sn;
         if (p.solver == 0) prev = sn; else std::fill(prev.begin(), prev.end(), (S)0);
@@ -461,21 +516,28 @@ template <class S> struct Pd {
         double err = 1;
         S omega = 1;
         int iters = 0;
+        const S wdbc = (S)p.positional_weight * dt2Inv;
         for (int i = 0; i < p.num_iterations && std::sqrt(err) >= p.tol; ++i) {
             ++iters;
+#pragma omp parallel for schedule(static)
             for (int v = 0; v < nv; ++v)
                 for (int k = 0; k < 3; ++k) b[3 * v + k] = c[v] * sn_old[3 * v + k];
             local_step(sn.data(), jacobi, b.data(), contrib);
-            const S wdbc = (S)p.positional_weight * dt2Inv;
-            for (int v = 0; v < nv; ++v)
+#pragma omp parallel for schedule(static)
+            for (int v = 0; v < nv; ++v) {  // computeDBCLocal
                 if (dbc[v] > 0)
                     for (int k = 0; k < 3; ++k) b[3 * v + k] = DBCX[3 * v + k] * wdbc;
+                else if (mw(v) > 0)
+                    for (int k = 0; k < 3; ++k) b[3 * v + k] = DBCX[3 * v + k] * mw(v);
+            }
             if (p.solver == 0 || (p.solver == 3 && p.pcg_chebyshev)) {
                 if (p.solver == 0) {
+#pragma omp parallel for schedule(static)
                     for (int v = 0; v < nv; ++v) {
                         const S cc = c[v], md = diagL[v];
                         for (int k = 0; k < 3; ++k)
-                            next[3 * v + k] = (b[3 * v + k] - cc * sn[3 * v + k]) / (cc + md) + sn[3 * v + k];
+                            next[3 * v + k] = mw(v) > 0 ? sn[3 * v + k]
+                                                        : (b[3 * v + k] - cc * sn[3 * v + k]) / (cc + md) + sn[3 * v + k];
                     }
                 } else {
                     next = sn;
@@ -485,8 +547,11 @@ template <class S> struct Pd {
                 if (i <= 10) omega = 1;
                 else if (i == 11) omega = (S)2 / ((S)2 - (S)p.rho * (S)p.rho);
                 else omega = (S)4 / ((S)4 - (S)p.rho * (S)p.rho * omega);
+                // chebyshevKern (pdUtil.cu:216-226): 0.9 is a double literal,
+                // so the damping is evaluated in double and rounded once
+#pragma omp parallel for schedule(static)
                 for (size_t j = 0; j < n3; ++j) {
-                    S nx = (S)0.9 * (next[j] - sn[j]) + sn[j];
+                    S nx = (S)(0.9 * (double)(next[j] - sn[j]) + (double)sn[j]);
                     nx = (nx - prev[j]) * omega + prev[j];
                     prev[j] = sn[j];
                     sn[j] = nx;
@@ -498,15 +563,21 @@ template <class S> struct Pd {
                 } else {
                     direct_solve(b.data(), sn.data());
                 }
-                double e = 0;
-                for (size_t j = 0; j < n3; ++j) { S d = prev[j] - sn[j]; e += (double)(d * d); }
-                err = e / (double)n3;
+                double e3[3];
+                vsum3([&](int v, double* a) {
+                    for (int k = 0; k < 3; ++k) {
+                        const S d = prev[3 * (size_t)v + k] - sn[3 * (size_t)v + k];
+                        a[k] += (double)(d * d);
+                    }
+                }, e3);
+                err = (e3[0] + e3[1] + e3[2]) / (double)n3;
                 prev = sn;
             }
         }
+#pragma omp parallel for schedule(static)
         for (int v = 0; v < nv; ++v)
             for (int k = 0; k < 3; ++k) {
-                V[3 * v + k] = (sn[3 * v + k] - XTilde[3 * v + k]) * dtInv;
+                V[3 * v + k] = mw(v) > 0 ? (S)0 : (sn[3 * v + k] - XTilde[3 * v + k]) * dtInv;
                 XTilde[3 * v + k] = sn[3 * v + k];
             }
         X = XTilde;
@@ -582,6 +653,19 @@ int orc_pd_step(int fp64, void* h, const OrcParams* p, int nsteps, int* pcg_iter
     for (int s = 0; s < nsteps; ++s)
         total += fp64 ? static_cast<Pd<double>*>(h)->step(*p, pcg_iters) : static_cast<Pd<float>*>(h)->step(*p, pcg_iters);
     return total;
+}
+
+// moreDBC weights (nv) and the targets of the vertices with a positive
+// weight (3*nv, read only there): computeSn writes the target into DBCX.
+void orc_pd_set_more_dbc(int fp64, void* h, const double* more, const double* target) {
+    auto set = [&](auto* pd) {
+        using S = std::remove_reference_t<decltype(pd->X[0])>;
+        pd->more.assign(more, more + pd->nv);
+        for (int v = 0; v < pd->nv; ++v)
+            if (more[v] > 0)
+                for (int k = 0; k < 3; ++k) pd->DBCX[3 * (size_t)v + k] = (S)target[3 * (size_t)v + k];
+    };
+    if (fp64) set(static_cast<Pd<double>*>(h)); else set(static_cast<Pd<float>*>(h));
 }
 
 // Copies out state: which 0 = X, 1 = V, 2 = XTilde (3*nv doubles).

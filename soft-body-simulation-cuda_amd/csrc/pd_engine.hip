@@ -41,6 +41,16 @@ template <class T> T pin_weight(double dt) {
     return (T)1e6 * dt2_inv;
 }
 
+// The Chebyshev step of the global solve (chebyshevKern, pdUtil.cu:216-226):
+// the damping 0.9 (x - s) + s uses the double literal 0.9, so it is evaluated
+// in double and rounded once to the solver precision; then
+// q = (damped - prev) omega + prev in the solver precision.  Every kernel that
+// applies the recurrence uses this one sequence.
+template <class T> __device__ __forceinline__ T cheb_update(T x, T s, T prev, T omega) {
+    const T damped = (T)fma(0.9, (double)(x - s), (double)s);
+    return fma(damped - prev, omega, prev);
+}
+
 template <class T> __device__ __forceinline__ void ld3(const T* __restrict__ p, size_t i, T& x, T& y, T& z) {
     x = p[3 * i];
     y = p[3 * i + 1];
@@ -335,33 +345,48 @@ __global__ void mass_diag_kernel(int nv, const T* __restrict__ mass, const T* __
 // Step prologue: gravity force (pdSolver.cu:154-155), setMDt_2MoreDBC
 // (pdUtil.cu:56-71), computeSn (pdUtil.cu:73-95), the sn_old copy and the prev
 // initialisation (pdSolver.cu:158-162; prev = 0 outside Jacobi mode, the
-// reference memsets bytes there).
+// reference memsets bytes there).  A vertex with a moreDBC weight (a dragged
+// vertex) is predicted at its target, which the caller keeps in DBCX (computeSn
+// writes the target into DBCX and sn).
 template <class T>
 __global__ void begin_step_kernel(int nv, const T* __restrict__ X, const T* __restrict__ V,
                                   const T* __restrict__ mass, const T* __restrict__ dbc,
-                                  const T* __restrict__ more_dbc, T dt, T gravity, T* __restrict__ c,
+                                  const T* __restrict__ more_dbc, const T* __restrict__ dbcx, T dt, T gravity,
+                                  T* __restrict__ c,
                                   T* __restrict__ sn, T* __restrict__ sn_old, T* __restrict__ prev,
-                                  int prev_from_sn) {
+                                  int prev_from_sn, T* __restrict__ sn2, T* __restrict__ prev2) {
     const unsigned v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= (unsigned)nv) return;
     const T m = mass[v];
     const T dt2 = dt * dt;
     T cv = c[v];
+    const T wi = more_dbc ? more_dbc[v] : (T)0;
     if (dbc[v] == (T)0) {
-        const T wi = more_dbc ? more_dbc[v] : (T)0;
         cv = wi > (T)0 ? (m + wi) / dt2 : m / dt2;
         c[v] = cv;
     }
-    T x, y, z, vx, vy, vz;
-    ld3(X, v, x, y, z);
-    ld3(V, v, vx, vy, vz);
-    const T inv = (T)1 / cv;
-    const T fy = -gravity * m;
-    const T sx = x + dt * vx, sy = y + dt * vy + inv * fy, sz = z + dt * vz;
+    T sx, sy, sz;
+    if (wi > (T)0) {
+        ld3(dbcx, v, sx, sy, sz);
+    } else {
+        T x, y, z, vx, vy, vz;
+        ld3(X, v, x, y, z);
+        ld3(V, v, vx, vy, vz);
+        const T inv = (T)1 / cv;
+        const T fy = -gravity * m;
+        sx = x + dt * vx; sy = y + dt * vy + inv * fy; sz = z + dt * vz;
+    }
     st3(sn, v, sx, sy, sz);
     st3(sn_old, v, sx, sy, sz);
     if (prev_from_sn) st3(prev, v, sx, sy, sz);
     else st3(prev, v, (T)0, (T)0, (T)0);
+    // the second sn/prev pair of the folded Chebyshev finish: the fold writes
+    // only vertices that some tet touches, and a vertex in no tet keeps
+    // sn = prev = sn_old through every iteration (its row is c x = c sn_old)
+    if (sn2) {
+        st3(sn2, v, sx, sy, sz);
+        st3(prev2, v, sx, sy, sz);
+    }
 }
 
 // Right-hand side of one vertex: addM_h2Sn (pdUtil.cu:168-178) plus the
@@ -442,12 +467,9 @@ __global__ void jacobi_chebyshev_kernel(int nv, RhsGather<T> g, const T* __restr
         ny = (by - cv * sy) / den + sy;
         nz = (bz - cv * sz) / den + sz;
     }
-    nx = (T)0.9 * (nx - sx) + sx;
-    ny = (T)0.9 * (ny - sy) + sy;
-    nz = (T)0.9 * (nz - sz) + sz;
-    nx = (nx - px) * omega + px;
-    ny = (ny - py) * omega + py;
-    nz = (nz - pz) * omega + pz;
+    nx = cheb_update(nx, sx, px, omega);
+    ny = cheb_update(ny, sy, py, omega);
+    nz = cheb_update(nz, sz, pz, omega);
     st3(prev, v, sx, sy, sz);
     st3(sn, v, nx, ny, nz);
 }
@@ -883,8 +905,7 @@ __global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, i
         for (int k = 0; k < 3; ++k) {
             const size_t i = 3 * (size_t)v + k;
             const T s = a.sn[i];
-            T nx = (T)0.9 * (a.x[i] - s) + s;
-            nx = (nx - a.prev[i]) * a.omega + a.prev[i];
+            const T nx = cheb_update(a.x[i], s, a.prev[i], a.omega);
             a.prev[i] = s;
             a.sn[i] = nx;
         }
@@ -912,7 +933,7 @@ __global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, i
 // read but formed at staging from that solve's iterate x (internal row order)
 // by the Chebyshev recurrence (pdSolver.cu:196-205),
 //   q = (0.9 (x - sn) + sn - prev) omega + prev,
-// which every chunk holding the vertex evaluates identically (explicit fmas);
+// which every chunk holding the vertex evaluates identically (cheb_update);
 // the vertex's first entry writes q to sn_out and sn to prev_out (double
 // buffers: other chunks still read sn_in / prev_in).
 template <class T> struct FoldIn {
@@ -957,7 +978,7 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
             ld3(fold.x, rw & 0x7fffffffu, v[0], v[1], v[2]);
             ld3(fold.sn_in, g, sv[0], sv[1], sv[2]);
             ld3(fold.prev_in, g, pv[0], pv[1], pv[2]);
-            for (int k = 0; k < 3; ++k) v[k] = fma(fma((T)0.9, v[k] - sv[k], sv[k]) - pv[k], fold.omega, pv[k]);
+            for (int k = 0; k < 3; ++k) v[k] = cheb_update(v[k], sv[k], pv[k], fold.omega);
             if (rw >> 31) {
                 st3(fold.sn_out, g, v[0], v[1], v[2]);
                 st3(fold.prev_out, g, sv[0], sv[1], sv[2]);
@@ -1322,8 +1343,7 @@ __global__ void __launch_bounds__(kTpb) pcg_finish_halo_kernel(PcgArgs<T> a, int
     for (int k = 0; k < 3; ++k) {
         const size_t j = 3 * (size_t)v + k;
         const T s = a.sn[j];
-        T nx = (T)0.9 * (xv[k] - s) + s;
-        nx = (nx - a.prev[j]) * a.omega + a.prev[j];
+        const T nx = cheb_update(xv[k], s, a.prev[j], a.omega);
         a.prev[j] = s;
         a.sn[j] = nx;
     }
@@ -1422,6 +1442,9 @@ template <class T> PdEngine<T>::~PdEngine() {
 }
 
 template <class T> void PdEngine<T>::reset() {
+    // the next update prepares again and reallocates the step's buffers, so a
+    // captured step (which holds their addresses) must not be replayed
+    drop_graph();
     ready_ = false;
     for (auto& e : perf_data_) e.ms = 0.0;
 }
@@ -1700,6 +1723,7 @@ template <class T> void PdEngine<T>::assemble(const PdParams& params) {
 }
 
 template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& params) {
+    drop_graph();  // the buffers a captured step refers to are reallocated below
     data_ = data;
     SBS_REQUIRE(params.dt > 0, "PdSolver: dt must be positive");
     const size_t n3 = 3 * (size_t)nv_;
@@ -2099,19 +2123,20 @@ template <class T> void PdEngine<T>::enqueue_step(PdData<T>& data, const PdParam
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (perf_) for (auto& e : ev) SBS_HIP(hipEventCreate(&e));
 
-    hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl);
-    hipLaunchKernelGGL(begin_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, data.X, data.V, data.mass,
-                       data.DBC, data.moreDBC, dt, (T)params.gravity, c_.get(), sn_.get(), sn_old_.get(), prev_.get(),
-                       (int)cheb);
-    SBS_HIP(hipGetLastError());
-    T omega = 1;
-    const T rho = (T)params.rho;
-    const int* skip = cheb ? nullptr : &ctl->outer_done;
     // PCG + Chebyshev on the halo kernels: the finish of solve i (its stop test
     // and the Chebyshev recurrence) runs inside the local step of iteration
     // i+1, which forms q from solve i's iterate; sn/prev then alternate
     // between two buffers.  The last solve keeps its finish launch.
     const bool fold = pcg && cheb && params.pcg_halo && halo_groups_ > 0 && chunk_rows_.get() != nullptr;
+    hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl);
+    hipLaunchKernelGGL(begin_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, data.X, data.V, data.mass,
+                       data.DBC, data.moreDBC, (const T*)data.DBCX, dt, (T)params.gravity, c_.get(), sn_.get(),
+                       sn_old_.get(), prev_.get(),
+                       (int)cheb, fold ? sn_alt_.get() : (T*)nullptr, fold ? prev_alt_.get() : (T*)nullptr);
+    SBS_HIP(hipGetLastError());
+    T omega = 1;
+    const T rho = (T)params.rho;
+    const int* skip = cheb ? nullptr : &ctl->outer_done;
     T* sn_cur = sn_.get();
     T* prev_cur = prev_.get();
     T* sn_nxt = sn_alt_.get();

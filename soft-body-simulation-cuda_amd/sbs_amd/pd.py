@@ -101,6 +101,26 @@ class SolverData:
         self.c.tIs = put("tIs", np.ones(self.nv, dtype=dt))
         self.c.Normals = put("Normals", np.zeros(3 * self.nv, dtype=np.float32))
 
+    def set_more_dbc(self, weights, targets=None) -> None:
+        """SolverData::moreDBC (def.h:32), the mouse-drag weights, with the
+        dragged vertices' targets written into DBCX (computeSn, pdUtil.cu:80-87,
+        writes the target there); None clears the weights.  The first update
+        sets DBCX from X0, so targets are written after it."""
+        if weights is None:
+            self.c.moreDBC = None
+            return
+        w = np.ascontiguousarray(weights, dtype=self.dtype).reshape(-1)
+        assert w.size == self.nv
+        if "moreDBC" in self._bufs:
+            self._bufs.pop("moreDBC").free()
+        self._bufs["moreDBC"] = capi.DeviceArray.from_numpy(w)
+        self.c.moreDBC = self._bufs["moreDBC"].ptr
+        if targets is not None:
+            d = self.get("DBCX")
+            sel = w > 0
+            d[sel] = np.asarray(targets, dtype=np.float64).reshape(-1, 3)[sel]
+            self._bufs["DBCX"].upload(np.ascontiguousarray(d, dtype=self.dtype).reshape(-1))
+
     def set_fixed_bodies(self, fixed) -> None:
         """SolverData::pFixedBodies (def.h:43): a FixedBodyData, or None."""
         self.fixed_bodies = fixed
