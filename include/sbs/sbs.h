@@ -351,6 +351,10 @@ int sbs_ipc_destroy(SbsIpc* h);
 /* One SolverStep; *converged = 0 when the step failed (then, like the
  * reference's latched `failed`, later updates do nothing until a reset). */
 int sbs_ipc_update(SbsIpc* h, SbsIpcData* data, const SbsIpcParams* params, int* converged);
+/* The elastic term of the IPC energy: 0 CorotatedEnergy (what IPEnergy's
+ * constructor builds, ip.cu:22), 1 NeoHookean08Energy (neohookean08.cu:99-206,
+ * the one-line swap there).  Default 0. */
+int sbs_ipc_set_elastic(SbsIpc* h, int model);
 int sbs_ipc_reset(SbsIpc* h);
 /* Linear solver of the Newton system.  Every type that converges to the
  * solution (0, 1, 2) runs Jacobi-PCG to lin_tol; 3 (plain Jacobi iteration)

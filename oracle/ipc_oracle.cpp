@@ -10,11 +10,15 @@
 //     DOFElimination                          src/simulation/solver/IPC/ipc.cu:142-258
 //   * IPEnergy Val / GradientHessian / UpdateKappa / InitStepSize
 //                                             src/simulation/energy/ip.cu:49-128
-//       (the elastic term is CorotatedEnergy, ip.cu:22)
+//       (the elastic term is CorotatedEnergy, ip.cu:22, or, selected, its
+//       one-line swap NeoHookean08Energy)
 //   * InertiaEnergy                           src/simulation/energy/inertia.cu:55-112
 //   * GravityEnergy (g = 9.8 along -y)        src/simulation/energy/gravity.cu:1-47, gravity.h:14
 //   * CorotatedEnergy Val / GradHessianKern   src/simulation/energy/corotated.cu:100-209
 //       (the Newton Hessian of the elastic term is not made positive definite there)
+//   * NeoHookean08Energy Val / GradHessianKern src/simulation/energy/neohookean08.cu:99-206
+//       (the Hessian it assembles omits the dC/dF term it forms, H3, and is
+//       not made positive definite)
 //   * BarrierEnergy Val / gradHessianKern     src/simulation/energy/barrier.cu:10-120,180-200
 //   * makePD (cyclic Jacobi, clamp at 1e-6)   src/simulation/matrix.h:441-518
 //   * distance types                          src/simulation/distance/distance_type.cuh:50-273
@@ -346,6 +350,22 @@ double barrier_d2(double d, double dhat, double kappa) {
 }
 
 // ------------------------------------------------------------ the solver --
+// det F and the cofactor matrix C = dJ/dF: column c of C is the cross
+// product of columns c+1 and c+2 of F (glm::cross(F[1], F[2]) and its
+// rotations in neohookean08.cu:21-24).
+double det3(const double F[3][3]) {
+    return F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+           F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+}
+void cofactor(const double F[3][3], double C[3][3]) {
+    for (int c = 0; c < 3; ++c) {
+        const int a = (c + 1) % 3, b = (c + 2) % 3;
+        C[0][c] = F[1][a] * F[2][b] - F[2][a] * F[1][b];
+        C[1][c] = F[2][a] * F[0][b] - F[0][a] * F[2][b];
+        C[2][c] = F[0][a] * F[1][b] - F[1][a] * F[0][b];
+    }
+}
+
 struct Query {
     uint32_t type, v[4];
     int dt;
@@ -360,6 +380,7 @@ struct Ipc {
     double dt = 0.01, damp = 0.999, dhat = 1e-3, tol = 5e-2, lin_tol = 1e-13, kappa = 100;
     int max_iter = 100, lin_max_iter = 20000;
     int ignore_self = 0;  // the detector's flag: UpdateQueries uses the last value set
+    int elastic = 0;      // 0 CorotatedEnergy (ip.cu:22), 1 NeoHookean08Energy
     std::vector<Query> qs;
     // fixed bodies: type (0 sphere, 1 plane, 2 cylinder) and 8 floats each
     // (p[3], u[3], r, 0), and each vertex's contact area
@@ -513,6 +534,15 @@ struct Ipc {
         for (int t = 0; t < nt; ++t) {
             double F[3][3], U[3][3], s[3], Vm[3][3];
             deformation(x, t, F);
+            if (this->elastic == 1) {
+                // NeoHookean08Energy::Val (neohookean08.cu:193-206)
+                double f2 = 0.0;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) f2 += F[r][c] * F[r][c];
+                const double lj = std::log(det3(F));
+                elastic += V0[t] * (mu[t] * 0.5 * (f2 - 3) - mu[t] * lj + 0.5 * lam[t] * lj * lj);
+                continue;
+            }
             svd(F, U, s, Vm);
             const double a = s[0] - 1, b = s[1] - 1, c = s[2] - 1, tr = a + b + c;
             elastic += V0[t] * (mu[t] * (a * a + b * b + c * c) + 0.5 * lam[t] * tr * tr);
@@ -528,6 +558,10 @@ struct Ipc {
     // gradient of the tet's elastic energy (scaled by coef) and, optionally, its
     // 12x12 Newton Hessian (row-major, vertex-major coordinates)
     void elastic_local(const std::vector<double>& x, int t, double coef, double* g12, double* H) const {
+        if (elastic == 1) {
+            nh_local(x, t, coef, g12, H);
+            return;
+        }
         double F[3][3], U[3][3], s[3], Vm[3][3];
         deformation(x, t, F);
         svd(F, U, s, Vm);
@@ -581,6 +615,56 @@ struct Ipc {
                 for (int k = 0; k < 3; ++k) v += cc * lk[k] * tw[k][a] * tw[k][b];
                 D[a][b] = v * w;
             }
+        for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < 12; ++j) {
+                double acc = 0.0;
+                for (int a = 0; a < 9; ++a) {
+                    if (G[a][i] == 0.0) continue;
+                    double row = 0.0;
+                    for (int b = 0; b < 9; ++b) row += D[a][b] * G[b][j];
+                    acc += G[a][i] * row;
+                }
+                H[12 * i + j] = acc;
+            }
+    }
+
+    // NeoHookean08Energy::GradHessianKern (neohookean08.cu:99-183): with
+    // J = det F and C = dJ/dF (cofactor matrix), P = mu (F - C/J) +
+    // lambda ln J C/J, and the Hessian that kernel assembles,
+    // mu I + (lambda (1 - ln J) + mu)/J^2 vec C vec C^T, mapped through
+    // dvec(F)/dx (the H3 term it forms is never added; no projection).
+    void nh_local(const std::vector<double>& x, int t, double coef, double* g12, double* H) const {
+        double F[3][3];
+        deformation(x, t, F);
+        double C[3][3];
+        cofactor(F, C);
+        const double J = det3(F), lj = std::log(J);
+        const double m_ = mu[t], l_ = lam[t], w = coef * V0[t];
+        double P[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) P[r][c] = m_ * (F[r][c] - C[r][c] / J) + l_ * lj * C[r][c] / J;
+        const double* dm = &Dm[9 * (size_t)t];
+        double G[9][12] = {};  // d vec(F) / d x, vec index 3c + r
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double s0 = 0.0;
+                for (int j = 0; j < 3; ++j) {
+                    G[3 * c + r][3 * (j + 1) + r] = dm[3 * j + c];
+                    s0 += dm[3 * j + c];
+                }
+                G[3 * c + r][r] = -s0;
+            }
+        for (int i = 0; i < 12; ++i) {
+            double acc = 0.0;
+            for (int e = 0; e < 9; ++e) acc += P[e % 3][e / 3] * G[e][i];
+            g12[i] = w * acc;
+        }
+        if (!H) return;
+        const double alpha = (l_ * (1 - lj) + m_) / (J * J);
+        double D[9][9];
+        for (int a = 0; a < 9; ++a)
+            for (int b = 0; b < 9; ++b)
+                D[a][b] = w * ((a == b ? m_ : 0.0) + alpha * C[a % 3][a / 3] * C[b % 3][b / 3]);
         for (int i = 0; i < 12; ++i)
             for (int j = 0; j < 12; ++j) {
                 double acc = 0.0;
@@ -923,6 +1007,29 @@ void* orc_ipc_create(int nv, int nt, int ntri, const double* X, const double* V,
     return s;
 }
 void orc_ipc_destroy(void* h) { delete (Ipc*)h; }
+// The elastic term: 0 CorotatedEnergy (ip.cu:22), 1 NeoHookean08Energy.
+void orc_ipc_set_elastic(void* h, int model) { ((Ipc*)h)->elastic = model; }
+// Neo-Hookean energy density, first Piola stress (9, row-major) and the
+// 9x9 d2Psi/dF2 of GradHessianKern (vec index 3c + r) of one F (row-major).
+double orc_nh_density(const double* F9, double mu, double lam, double* P9, double* D81) {
+    double F[3][3], C[3][3];
+    std::memcpy(F, F9, sizeof F);
+    cofactor(F, C);
+    const double J = det3(F), lj = std::log(J);
+    double f2 = 0.0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            f2 += F[r][c] * F[r][c];
+            if (P9) P9[3 * r + c] = mu * (F[r][c] - C[r][c] / J) + lam * lj * C[r][c] / J;
+        }
+    if (D81) {
+        const double alpha = (lam * (1 - lj) + mu) / (J * J);
+        for (int a = 0; a < 9; ++a)
+            for (int b = 0; b < 9; ++b)
+                D81[9 * a + b] = (a == b ? mu : 0.0) + alpha * C[a % 3][a / 3] * C[b % 3][b / 3];
+    }
+    return mu * 0.5 * (f2 - 3) - mu * lj + 0.5 * lam * lj * lj;
+}
 // One SolverStep; returns 1 on success, 0 when it failed (the reference's `failed`).
 int orc_ipc_step(void* h) { return ((Ipc*)h)->step() ? 1 : 0; }
 void orc_ipc_get(void* h, double* X, double* V) {

@@ -109,6 +109,9 @@ def lib(bench: bool = False) -> ctypes.CDLL:
         L.orc_ipc_create.argtypes = [ctypes.c_int] * 3 + [dp, dp, up, up, up, dp, dp, dp, ucp] + \
             [ctypes.c_double] * 5 + [ctypes.c_int]
         L.orc_ipc_destroy.argtypes = [vp]
+        L.orc_ipc_set_elastic.argtypes = [vp, ctypes.c_int]
+        L.orc_nh_density.restype = ctypes.c_double
+        L.orc_nh_density.argtypes = [dp, ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_ipc_step.restype = ctypes.c_int
         L.orc_ipc_step.argtypes = [vp]
         L.orc_ipc_get.argtypes = [vp, dp, dp]
@@ -355,11 +358,22 @@ def ipc_make_pd(M):
     return A
 
 
+def nh_density(F, mu, lam):
+    """NeoHookean08 energy density, first Piola stress P (3x3) and the 9x9
+    d2Psi/dF2 that GradHessianKern assembles (vec index 3c + r)."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    P = np.zeros((3, 3))
+    D = np.zeros((9, 9))
+    e = lib().orc_nh_density(_p(F, ctypes.c_double), float(mu), float(lam), _p(P, ctypes.c_double),
+                             _p(D, ctypes.c_double))
+    return e, P, D
+
+
 class IpcOracle:
     """The reference's IPCSolver step (ipc.cu:142-258) on the CPU, FP64."""
 
     def __init__(self, X, V, tet, tri, fathers, mass, mu, lam, fixed, dt=0.01, damp=0.999, dhat=1e-3, tol=5e-2,
-                 lin_tol=1e-13, max_iter=100):
+                 lin_tol=1e-13, max_iter=100, elastic="corotated"):
         self.L = lib()
         X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
         V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
@@ -375,6 +389,7 @@ class IpcOracle:
                                        _p(tri, cu), _p(fathers, cu), _p(self._keep[0], cd), _p(self._keep[1], cd),
                                        _p(self._keep[2], cd), _p(fx, ctypes.c_ubyte), dt, damp, dhat, tol, lin_tol,
                                        int(max_iter))
+        self.L.orc_ipc_set_elastic(self.h, {"corotated": 0, "neohookean": 1}[elastic])
 
     def step(self, n=1):
         ok = True

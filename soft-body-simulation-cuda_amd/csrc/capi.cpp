@@ -586,6 +586,15 @@ int sbs_ipc_update(SbsIpc* h, SbsIpcData* data, const SbsIpcParams* params, int*
     });
 }
 
+int sbs_ipc_set_elastic(SbsIpc* h, int model) {
+    return guarded([&] {
+        SBS_REQUIRE(h, "sbs_ipc_set_elastic: null handle");
+        SBS_REQUIRE(model == sbs::kElasticCorotated || model == sbs::kElasticNeoHookean,
+                    "sbs_ipc_set_elastic: model must be 0 (corotated) or 1 (Neo-Hookean)");
+        h->e->set_elastic(model);
+    });
+}
+
 int sbs_ipc_reset(SbsIpc* h) {
     return guarded([&] {
         SBS_REQUIRE(h, "sbs_ipc_reset: null handle");

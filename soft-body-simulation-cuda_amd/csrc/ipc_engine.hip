@@ -376,7 +376,7 @@ Bodies bodies_of(const FixedBodies* f) {
 __global__ void ipc_energy_kernel(int nv, int nt, long long nq, const double* X, const double* XT,
                                   const double* mass, const uint32_t* tet, const double* dm, const double* v0,
                                   const double* mu, const double* lam, const double* qd, double dhat, double kappa,
-                                  double h2, Bodies bodies, const double* area, double* parts) {
+                                  double h2, Bodies bodies, const double* area, int neo_hookean, double* parts) {
     const long long i = (long long)blockIdx.x * kT + threadIdx.x;
     double e[1] = {0.0};
     if (i < nv) {
@@ -394,11 +394,21 @@ __global__ void ipc_energy_kernel(int nv, int nt, long long nq, const double* X,
         }
     }
     if (i < nt) {
-        double F[3][3], U[3][3], sg[3], Vm[3][3];
+        double F[3][3];
         deformation(X, tet + 4 * i, dm + 9 * i, F);
-        Svd3<double>::run(F, U, sg, Vm);
-        const double a = sg[0] - 1, b = sg[1] - 1, c = sg[2] - 1, tr = a + b + c;
-        e[0] += h2 * v0[i] * (mu[i] * (a * a + b * b + c * c) + 0.5 * lam[i] * tr * tr);
+        if (neo_hookean) {
+            // NeoHookean08Energy::Val (neohookean08.cu:193-206)
+            double f2 = 0.0;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) f2 += F[r][c] * F[r][c];
+            const double lj = log(det3(F));
+            e[0] += h2 * v0[i] * (mu[i] * 0.5 * (f2 - 3) - mu[i] * lj + 0.5 * lam[i] * lj * lj);
+        } else {
+            double U[3][3], sg[3], Vm[3][3];
+            Svd3<double>::run(F, U, sg, Vm);
+            const double a = sg[0] - 1, b = sg[1] - 1, c = sg[2] - 1, tr = a + b + c;
+            e[0] += h2 * v0[i] * (mu[i] * (a * a + b * b + c * c) + 0.5 * lam[i] * tr * tr);
+        }
     }
     if (i < nq) {
         const double d = qd[i];
@@ -489,6 +499,67 @@ __global__ void __launch_bounds__(kT) ipc_elastic_kernel(int nt, const uint32_t*
                         }
                     H[12 * (3 * a + k) + 3 * b + l] = w * acc;
                 }
+    if (diag)
+        for (int a = 0; a < 4; ++a)
+            if (!fixed[tv[a]])
+                for (int k = 0; k < 3; ++k) atomicAdd(&diag[3 * (size_t)tv[a] + k], H[13 * (3 * a + k)]);
+}
+
+// Neo-Hookean elasticity of one tet (NeoHookean08Energy::GradHessianKern,
+// neohookean08.cu:99-183), scaled by coef: with J = det F and C = dJ/dF (the
+// cofactor matrix, columns F1 x F2, F2 x F0, F0 x F1), the gradient V0 P Dm^-T
+// with P = mu (F - C/J) + lambda ln J C/J into g (atomics), and with ht the
+// 12x12 Hessian G^T D G of the kernel's
+//   D = mu I + (lambda (1 - ln J) + mu) / J^2 vec C vec C^T,
+// which is what that kernel assembles: the term (lambda ln J - mu)/J dC/dF of
+// the exact second derivative is formed there (H3) but never added, and the
+// Hessian is not projected to positive definite.
+__global__ void __launch_bounds__(kT) ipc_nh_kernel(int nt, const uint32_t* tet, const double* X, const double* dm,
+                                                    const double* v0, const double* mu, const double* lam,
+                                                    double coef, const unsigned char* fixed, double* g, double* ht,
+                                                    double* diag) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t* tv = tet + 4 * (size_t)t;
+    const double* m = dm + 9 * (size_t)t;
+    double F[3][3];
+    deformation(X, tv, m, F);
+    double C[3][3];
+    for (int c = 0; c < 3; ++c) {
+        const int a = (c + 1) % 3, b = (c + 2) % 3;
+        C[0][c] = F[1][a] * F[2][b] - F[2][a] * F[1][b];
+        C[1][c] = F[2][a] * F[0][b] - F[0][a] * F[2][b];
+        C[2][c] = F[0][a] * F[1][b] - F[1][a] * F[0][b];
+    }
+    const double J = det3(F), invJ = 1.0 / J, lj = log(J);
+    const double mu_ = mu[t], la = lam[t], w = coef * v0[t];
+    double Gv[4][3];
+    for (int c = 0; c < 3; ++c) {
+        Gv[1][c] = m[c];
+        Gv[2][c] = m[3 + c];
+        Gv[3][c] = m[6 + c];
+        Gv[0][c] = -(m[c] + m[3 + c] + m[6 + c]);
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int c = 0; c < 3; ++c)
+                acc += (mu_ * (F[r][c] - invJ * C[r][c]) + la * lj * invJ * C[r][c]) * Gv[a][c];
+            atomicAdd(&g[3 * (size_t)tv[a] + r], w * acc);
+        }
+    if (!ht) return;
+    const double alpha = (la * (1.0 - lj) + mu_) * invJ * invJ;
+    double q[4][3], gg[4][4];  // q[a][k] = sum_c Gv[a][c] C[k][c]; gg[a][b] = sum_c Gv[a][c] Gv[b][c]
+    for (int a = 0; a < 4; ++a) {
+        for (int k = 0; k < 3; ++k) q[a][k] = Gv[a][0] * C[k][0] + Gv[a][1] * C[k][1] + Gv[a][2] * C[k][2];
+        for (int b = 0; b < 4; ++b) gg[a][b] = Gv[a][0] * Gv[b][0] + Gv[a][1] * Gv[b][1] + Gv[a][2] * Gv[b][2];
+    }
+    double* H = ht + 144 * (size_t)t;
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k)
+            for (int b = 0; b < 4; ++b)
+                for (int l = 0; l < 3; ++l)
+                    H[12 * (3 * a + k) + 3 * b + l] = w * ((k == l ? mu_ * gg[a][b] : 0.0) + alpha * q[a][k] * q[b][l]);
     if (diag)
         for (int a = 0; a < 4; ++a)
             if (!fixed[tv[a]])
@@ -906,7 +977,7 @@ double IpcEngine::energy(const double* X) {
     hipLaunchKernelGGL(ipc_energy_kernel, dim3(nb), dim3(kT), 0, stream_, nv_, nt_, nq_, X, (const double*)d_.XTilde,
                        d_.mass, d_.Tet, (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda,
                        (const double*)qd_.get(), p_.dhat, kappa_, p_.dt * p_.dt, bodies_of(d_.fixed_bodies),
-                       (const double*)area_.get(), parts_.get());
+                       (const double*)area_.get(), (int)(elastic_ == kElasticNeoHookean), parts_.get());
     SBS_HIP(hipGetLastError());
     return reduce_sum(parts_.get(), (int)nb);
 }
@@ -918,10 +989,7 @@ void IpcEngine::gradient(const double* X, bool hessian, double kap) {
     const double h2 = p_.dt * p_.dt;
     hipLaunchKernelGGL(ipc_grad_vertex_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, X,
                        (const double*)d_.XTilde, d_.mass, h2, g_.get(), hessian ? diag_.get() : nullptr);
-    hipLaunchKernelGGL(ipc_elastic_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet, X,
-                       (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda, h2,
-                       (const unsigned char*)fixed_.get(), g_.get(), hessian ? ht_.get() : nullptr,
-                       hessian ? diag_.get() : nullptr);
+    elastic(X, h2, hessian);
     body_gradient(X, h2, hessian);
     if (nq_ > 0)
         hipLaunchKernelGGL(ipc_barrier_kernel, dim3((unsigned)((nq_ + kQB - 1) / kQB)), dim3(kQB), 0, stream_, nq_,
@@ -932,6 +1000,16 @@ void IpcEngine::gradient(const double* X, bool hessian, double kap) {
         hipLaunchKernelGGL(ipc_fix_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_,
                            (const unsigned char*)fixed_.get(), g_.get(), diag_.get());
     SBS_HIP(hipGetLastError());
+}
+
+// The elastic term of IPEnergy (ip.cu:22 builds CorotatedEnergy; the
+// Neo-Hookean energy is the one-line swap there): coef * gradient into g and,
+// with `hessian`, the element Hessians and their Jacobi diagonal.
+void IpcEngine::elastic(const double* X, double coef, bool hessian) {
+    auto kern = elastic_ == kElasticNeoHookean ? ipc_nh_kernel : ipc_elastic_kernel;
+    hipLaunchKernelGGL(kern, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet, X, (const double*)dm_.get(),
+                       (const double*)v0_.get(), d_.mu, d_.lambda, coef, (const unsigned char*)fixed_.get(), g_.get(),
+                       hessian ? ht_.get() : nullptr, hessian ? diag_.get() : nullptr);
 }
 
 // ImplicitBarrierEnergy::GradientHessian (implicitBarrier.cu:150-240): adds
@@ -968,9 +1046,7 @@ void IpcEngine::update_kappa(const double* X) {
     const double h2 = p_.dt * p_.dt;
     hipLaunchKernelGGL(ipc_grad_vertex_kernel, dim3(nblk(nv_)), dim3(kT), 0, stream_, nv_, X,
                        (const double*)d_.XTilde, d_.mass, h2, g_.get(), (double*)nullptr);
-    hipLaunchKernelGGL(ipc_elastic_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, d_.Tet, X,
-                       (const double*)dm_.get(), (const double*)v0_.get(), d_.mu, d_.lambda, h2,
-                       (const unsigned char*)fixed_.get(), g_.get(), (double*)nullptr, (double*)nullptr);
+    elastic(X, h2, false);
     SBS_HIP(hipGetLastError());
     body_gradient(X, h2, false);
     const double ge = max_abs(g_.get(), n3);

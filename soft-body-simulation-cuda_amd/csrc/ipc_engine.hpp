@@ -43,6 +43,8 @@ struct IpcData {
     const FixedBodies* fixed_bodies = nullptr;  // SolverData::pFixedBodies: the implicit barrier's bodies
 };
 
+enum : int { kElasticCorotated = 0, kElasticNeoHookean = 1 };
+
 struct IpcParams {
     double dt = 0.01;
     double damp = 0.999;
@@ -68,6 +70,10 @@ public:
     // (the latched `failed`); returns false when this step failed.
     bool update(IpcData& data, const IpcParams& params);
     void reset() { failed_ = false; }
+    // The elastic term of IPEnergy: CorotatedEnergy (what ip.cu:22 builds) or
+    // NeoHookean08Energy (neohookean08.cu, the one-line swap of ip.cu:22).
+    void set_elastic(int model) { elastic_ = model; }
+    int elastic_model() const { return elastic_; }
     const IpcStats& stats() const { return stats_; }
     void set_stream(hipStream_t s);
     // Energy and gradient at X with the current x~ and kappa, the queries
@@ -87,10 +93,12 @@ private:
     double max_abs(const double* v, size_t n);
     double body_step(const double* X, const double* p);
     void body_gradient(const double* X, double coef, bool hessian);
+    void elastic(const double* X, double coef, bool hessian);
 
     int nv_ = 0, nt_ = 0, ntri_ = 0;
     IpcData d_{};
     IpcParams p_{};
+    int elastic_ = kElasticCorotated;
     hipStream_t stream_ = nullptr;
     bool failed_ = false;
     bool ignore_self_ = false;  // the detector's flag: UpdateQueries uses the last value set (bvh.h:82)

@@ -137,6 +137,7 @@ SIGNATURES = {
     "sbs_ipc_destroy": (_I, [_VP]),
     "sbs_ipc_update": (_I, [_VP, ctypes.POINTER(SbsIpcData), ctypes.POINTER(SbsIpcParams), ctypes.POINTER(_I)]),
     "sbs_ipc_reset": (_I, [_VP]),
+    "sbs_ipc_set_elastic": (_I, [_VP, _I]),
     "sbs_ipc_set_linear_solver": (_I, [_VP, _I]),
     "sbs_ipc_set_stream": (_I, [_VP, _VP]),
     "sbs_ipc_stats": (_I, [_VP, ctypes.POINTER(SbsIpcStats)]),

@@ -78,13 +78,18 @@ class IpcData:
 class IpcSolver:
     """IPCSolver(threadsPerBlock, SolverData<double>) over the C ABI."""
 
-    def __init__(self, data: IpcData):
+    ELASTIC = {"corotated": 0, "neohookean": 1}
+
+    def __init__(self, data: IpcData, elastic: str = "corotated"):
+        """elastic: the IPEnergy elastic term, "corotated" (what ip.cu:22
+        builds) or "neohookean" (NeoHookean08Energy, the swap there)."""
         self._L = capi.lib()
         self._data = data
         h = ctypes.c_void_p()
         cd = data.c()
         capi.check(self._L.sbs_ipc_create(ctypes.byref(cd), ctypes.byref(h)))
         self.h = h.value
+        capi.check(self._L.sbs_ipc_set_elastic(self.h, self.ELASTIC[elastic]))
 
     def Update(self, data: IpcData, params: IpcParams) -> bool:
         cd, cp, ok = data.c(), params.to_c(), ctypes.c_int()

@@ -1,0 +1,140 @@
+"""GPU parity of the IPC step with Neo-Hookean elasticity (NeoHookean08Energy,
+neohookean08.cu:99-206, the one-line swap of ip.cu:22), and of the contact
+scene of two 16^3-hex Neo-Hookean cubes (E = 1e5 Pa, nu = 0.4, the upper one
+5 mm above the lower with +-0.5 mm jitter, h = 1/100, dhat = 1e-3).
+
+Bars as for the corotated IPC step: energies 1e-12 relative, gradients 1e-10
+of their largest entry, positions 1e-10 of the displacement scale with equal
+Newton and line-search counts; query lists, times of impact and hit sets
+bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from sbs_amd import mesh
+from sbs_amd.ipc import IpcData, IpcParams, IpcSolver
+from sbs_amd.pd import CollisionDetection
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_for(s, p: IpcParams):
+    return O.IpcOracle(s.X, np.zeros_like(s.X), s.tet, s.tri, s.tri_fathers, s.mass, s.mu, s.lam, s.dbc > 0,
+                       dt=p.dt, damp=p.damp, dhat=p.dhat, tol=p.tol, lin_tol=p.lin_tol, max_iter=p.maxIterations,
+                       elastic="neohookean")
+
+
+def config4_scene():
+    return mesh.stack_scene(16, spacing=0.1, gap=5e-3, jitter=5e-4, seed=0, E=1e5, nu=0.4)
+
+
+def test_energy_and_gradient_match_oracle(gpu):
+    s = mesh.stack_scene(2, gap=4e-4, nu=0.4)
+    p = IpcParams(dt=0.01, dhat=1e-3)
+    data = IpcData(s)
+    solver = IpcSolver(data, elastic="neohookean")
+    o = oracle_for(s, p)
+    rng = np.random.default_rng(3)
+    X = s.X + rng.normal(scale=2e-3, size=s.X.shape)
+    X[s.body_vert_start[1]:, 1] -= 1e-4
+    XT = s.X + 1e-3
+    solver.set_state(XT, 350.0)
+    o.set_state(XT, 350.0)
+    e_gpu, g_gpu = solver.energy(X, grad=True)
+    e_orc, g_orc = o.energy(X, grad=True)
+    assert abs(e_gpu - e_orc) <= 1e-12 * abs(e_orc)
+    assert np.abs(g_gpu - g_orc).max() <= 1e-10 * np.abs(g_orc).max()
+    # and the corotated solver gives a different energy for the same state
+    c = IpcSolver(data)
+    c.set_state(XT, 350.0)
+    assert abs(c.energy(X) - e_gpu) > 1e-9 * abs(e_gpu)
+    c.close()
+    solver.close()
+    data.free()
+
+
+def run_pair(s, p, nsteps):
+    data = IpcData(s)
+    solver = IpcSolver(data, elastic="neohookean")
+    o = oracle_for(s, p)
+    out = []
+    try:
+        for k in range(nsteps):
+            ok_g = solver.Update(data, p)
+            ok_o = o.step()
+            assert ok_g and ok_o, (k, solver.stats(), o.stats())
+            out.append((data.get("X"), o.get()[0], solver.stats(), o.stats()))
+    finally:
+        solver.close()
+        data.free()
+    return out
+
+
+def check_steps(s, out):
+    for k, (Xg, Xo, sg, so) in enumerate(out):
+        scale = max(np.abs(Xo - s.X).max(), 1e-6)
+        assert np.abs(Xg - Xo).max() <= 1e-10 * scale, (k, np.abs(Xg - Xo).max(), scale, sg, so)
+        assert sg["newton_iterations"] == so["newton_iterations"], (k, sg, so)
+        assert sg["line_search_halvings"] == so["line_search_halvings"], (k, sg, so)
+        assert np.isclose(sg["kappa"], so["kappa"], rtol=1e-8)
+
+
+def test_contact_steps_match_oracle(gpu):
+    s = mesh.stack_scene(2, gap=1.5e-3, nu=0.4)
+    out = run_pair(s, IpcParams(dt=0.01, dhat=1e-3, tol=1e-3), 10)
+    check_steps(s, out)
+    assert any(sg["kappa"] > 100.0 for _, _, sg, _ in out)     # the barrier was engaged
+
+
+def test_config4_first_steps_match_oracle(gpu):
+    """The 16^3 pair itself: the steps up to and into contact."""
+    s = config4_scene()
+    out = run_pair(s, IpcParams(dt=0.01, damp=0.999, dhat=1e-3, tol=5e-2, lin_tol=1e-10, lin_max_iter=5000), 4)
+    check_steps(s, out)
+    assert out[-1][2]["queries"] > 0     # the faces came within the query distance
+
+
+def _config4_motion(drop):
+    s = config4_scene()
+    X = s.X.copy()
+    XT = X.copy()
+    upper = np.arange(s.nv) >= s.body_vert_start[1]
+    XT[upper, 1] -= drop
+    XT += np.random.default_rng(5).uniform(-2e-4, 2e-4, size=XT.shape)
+    return s, X, XT
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_config4_query_lists_and_hits_bit_exact(gpu, precision):
+    """DetectCollision over a step that carries the upper cube through the
+    lower one's top: queries, times of impact, normals, tI and the hit set
+    equal to the all-pairs oracle."""
+    s, X, XT = _config4_motion(8e-3)
+    cd = CollisionDetection(s.nv, s.tri, s.tri_fathers, precision)
+    try:
+        tI, nrm = cd.DetectCollision(X, XT, True)
+        q, toi, qn = cd.GetQueries()
+        pairs, hits = cd.stats()
+    finally:
+        cd.close()
+    r = O.collision_detect(X, XT, s.tri, s.tri_fathers, ignore_self=True, fp64=precision == 64)
+    np.testing.assert_array_equal(q, r["queries"])
+    np.testing.assert_array_equal(toi, r["toi"])
+    hit = toi < 1
+    np.testing.assert_array_equal(q[hit], r["queries"][r["toi"] < 1])
+    np.testing.assert_array_equal(qn[hit], r["normals"][hit])
+    np.testing.assert_array_equal(tI, r["tI"])
+    assert hits == r["hits"] and hits > 100
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_config4_min_step_bit_exact(gpu, precision):
+    s, X, XT = _config4_motion(8e-3)
+    cd = CollisionDetection(s.nv, s.tri, s.tri_fathers, precision)
+    try:
+        t = cd.ComputeMinStepSize(X, XT, False)
+    finally:
+        cd.close()
+    r = O.collision_detect(X, XT, s.tri, s.tri_fathers, ignore_self=False, fp64=precision == 64, resolve=False)
+    assert t == min(1.0, r["toi"].min()) and t < 1
