@@ -213,10 +213,48 @@ int sbs_collision_min_step(SbsCollision* c, const void* d_X, const void* d_XTild
 int sbs_collision_num_queries(SbsCollision* c, long long* n);
 int sbs_collision_get_queries(SbsCollision* c, uint32_t* queries, double* toi, double* normals);
 int sbs_collision_stats(SbsCollision* c, long long* candidate_pairs, int* hits);
+/* SetBuildType / GetBuildType (bvh.h:67-68): BVH<Scalar>::BuildType Serial (0),
+ * Atomic (1, the default), Cooperative (2) -- how the internal boxes are
+ * fitted (bvh.cu:118-135).  Serial fits one tree level per launch until the
+ * root is done; Atomic and Cooperative run the one-launch bottom-up climb.
+ * Every type gives the same boxes and so the same queries. */
+int sbs_collision_set_build_type(SbsCollision* c, int build_type);
+int sbs_collision_get_build_type(SbsCollision* c, int* build_type);
+/* UpdateQueries(numVerts, numTris, Tri, X, TriFathers, dhat) (bvh.h:52,
+ * broadphase.cu:574-587), precision 64 (the FP64 IPC solver is its caller):
+ * the queries of the triangle pairs whose boxes at X, grown by 2 dhat, overlap,
+ * with the ignore-self flag of the last detect / min_step call (false before
+ * any, as the reference's member), kept while the squared distance is
+ * <= dhat, sorted by (squared distance, distance type), ties by vertices.
+ * get_proximity copies them (5 u32 each), their distance types (the order
+ * of distance_type.cuh's DistanceType) and squared distances. */
+int sbs_collision_update_queries(SbsCollision* c, const void* d_X, double dhat, long long* num_queries);
+int sbs_collision_get_proximity(SbsCollision* c, uint32_t* queries, int* distance_types, double* distances);
 /* CCDKernel (src/simulation/collisionUtil.cu:50-70): vertices with tI < 1 get
  * V = -((XTilde - X).n) n and keep X; the others take X = XTilde. */
 int sbs_ccd_response(int precision, int num_verts, void* d_X, const void* d_XTilde, void* d_V, const void* d_tI,
                      const float* d_normals);
+
+/* LinearSolver<T>::Solve(int N, T* d_b, T* d_x, T* A, int nz, int* rowIdx,
+ * int* colIdx, T* d_guess) (src/simulation/solver/linear/linear.h:62) with
+ * the PCGJacobiSolver algorithm (pcgJacobi.h:7-10, pcgJacobi.cu:89-173), the
+ * linear solver IPCSolver uses (ipc.h:16) and calls with its Hessian's COO
+ * triplets (ipc.cu:239):
+ *   PCGJacobiSolver(N, max_iter = 2000, tolerance = 1e-5) -> sbs_linear_create
+ *   Solve(N, d_b, d_x, A, nz, rowIdx, colIdx, d_guess)    -> sbs_linear_solve
+ * The triplets (device arrays) may be unsorted and repeat an entry: as
+ * sort_coo (linear.cu:7-40) the repeats are summed.  x starts at d_guess (or
+ * 0); each iteration stops first when ||r||_2 < tolerance (absolute), then
+ * when |r.z| < 1e-15, else takes the Jacobi-preconditioned CG step; at most
+ * max_iter iterations, whose count is returned.  Indices outside [0, N) are
+ * rejected.  sbs_linear_stats reports the entries left after summing. */
+typedef struct SbsLinear SbsLinear;
+int sbs_linear_create(int precision, int max_iter, double tolerance, SbsLinear** out);
+int sbs_linear_destroy(SbsLinear* h);
+int sbs_linear_set_stream(SbsLinear* h, void* hip_stream);
+int sbs_linear_solve(SbsLinear* h, int n, void* d_b, void* d_x, const void* d_A, int nz, const int* d_rowIdx,
+                     const int* d_colIdx, const void* d_guess, int* iterations);
+int sbs_linear_stats(SbsLinear* h, long long* nnz);
 
 /* Fixed (kinematic) bodies: FixedBody (src/collision/rigid/rigid.h:14-28) as
  * Sphere (sphere.h), Plane (plane.h) or Cylinder (cylinder.h), held as

@@ -1041,6 +1041,15 @@ void orc_ipc_set_x(void* h, const double* X) {
     auto* s = (Ipc*)h;
     std::memcpy(s->X.data(), X, s->X.size() * sizeof(double));
 }
+// Positions, velocities and the detector's ignore-self flag of a state taken
+// from another run (the flag is set by the line search's ComputeMinStepSize,
+// barrier.cu:227, so it is 1 after any step that searched a line).
+void orc_ipc_set_xv(void* h, const double* X, const double* V, int ignore_self) {
+    auto* s = (Ipc*)h;
+    std::memcpy(s->X.data(), X, s->X.size() * sizeof(double));
+    std::memcpy(s->V.data(), V, s->V.size() * sizeof(double));
+    s->ignore_self = ignore_self;
+}
 // newton iterations, line-search halvings, last linear iterations, queries, kappa, first alpha0
 void orc_ipc_stats(void* h, int* ints, double* reals) {
     auto* s = (Ipc*)h;

@@ -116,6 +116,7 @@ def lib(bench: bool = False) -> ctypes.CDLL:
         L.orc_ipc_step.argtypes = [vp]
         L.orc_ipc_get.argtypes = [vp, dp, dp]
         L.orc_ipc_set_x.argtypes = [vp, dp]
+        L.orc_ipc_set_xv.argtypes = [vp, dp, dp, ctypes.c_int]
         L.orc_ipc_stats.argtypes = [vp, ip, dp]
         L.orc_ipc_energy.restype = ctypes.c_double
         L.orc_ipc_energy.argtypes = [vp, dp, dp]
@@ -331,6 +332,34 @@ DISTANCE_TYPES = ["P_T0", "P_T1", "P_T2", "P_E0", "P_E1", "P_E2", "P_T", "EA0_EB
                   "EA1_EB1", "EA_EB0", "EA_EB1", "EA0_EB", "EA1_EB", "EA_EB"]
 
 
+def update_queries(X, tri, fathers, dhat, ignore_self=False):
+    """UpdateQueries (broadphase.cu:574-587) on the CPU: all-pairs candidates
+    whose boxes grown by 2 dhat overlap, their distance types and squared
+    distances, kept while d <= dhat; (queries (n, 5), types (n,), d (n,))
+    sorted by (d, type, vertices)."""
+    L = lib()
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+    tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1)
+    fathers = np.ascontiguousarray(fathers, dtype=np.uint32).reshape(-1)
+    n = L.orc_col_proximity(tri.size // 3, _p(tri, ctypes.c_uint32), _p(fathers, ctypes.c_uint32),
+                            _p(X, ctypes.c_double), 2 * dhat, int(ignore_self), None)
+    raw = np.zeros((n, 5), np.uint32)
+    if n:
+        L.orc_col_proximity(tri.size // 3, _p(tri, ctypes.c_uint32), _p(fathers, ctypes.c_uint32),
+                            _p(X, ctypes.c_double), 2 * dhat, int(ignore_self), _p(raw, ctypes.c_uint32))
+    P = X.reshape(-1, 3)
+    rows = []
+    for q in raw:
+        x12 = P[q[1:5]].reshape(12)
+        t = ipc_distance_type(int(q[0]), x12)
+        d = ipc_distance(int(q[0]), t, x12)[0]
+        if d <= dhat:
+            rows.append((d, t, *[int(v) for v in q]))
+    rows.sort()
+    q = np.array([r[2:] for r in rows], dtype=np.uint32).reshape(-1, 5)
+    return q, np.array([r[1] for r in rows], dtype=np.int32), np.array([r[0] for r in rows])
+
+
 def ipc_distance_type(qtype, x12):
     x = np.ascontiguousarray(x12, dtype=np.float64).reshape(12)
     return lib().orc_ipc_distance_type(int(qtype), _p(x, ctypes.c_double))
@@ -409,6 +438,13 @@ class IpcOracle:
         self.L.orc_ipc_stats(self.h, _p(ints, ctypes.c_int), _p(reals, ctypes.c_double))
         return dict(newton_iterations=int(ints[0]), line_search_halvings=int(ints[1]), linear_iterations=int(ints[2]),
                     queries=int(ints[3]), kappa=float(reals[0]), alpha0=float(reals[1]))
+
+    def set_xv(self, X, V, ignore_self=True):
+        """Continue from another run's positions and velocities (the detector's
+        ignore-self flag as a line search leaves it)."""
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
+        self.L.orc_ipc_set_xv(self.h, _p(X, ctypes.c_double), _p(V, ctypes.c_double), int(ignore_self))
 
     def set_state(self, XT, kappa):
         XT = np.ascontiguousarray(XT, dtype=np.float64).reshape(-1)

@@ -11,6 +11,7 @@
 #include "../../include/sbs/sbs.h"
 #include "collision.hpp"
 #include "ipc_engine.hpp"
+#include "linear_solver.hpp"
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
 #include "scene_loader.hpp"
@@ -48,6 +49,19 @@ struct SbsCollision {
     int precision = 32;
     std::unique_ptr<sbs::CollisionDetector<float>> f32;
     std::unique_ptr<sbs::CollisionDetector<double>> f64;
+    // the ignoreSelfCollision member: false until DetectCollision or
+    // ComputeMinStepSize sets it, and UpdateQueries uses its last value (bvh.h:82)
+    bool ignore_self = false;
+    std::vector<uint32_t> prox_q;  // the last UpdateQueries result
+    std::vector<int> prox_type;
+    std::vector<double> prox_d;
+};
+
+struct SbsLinear {
+    int precision = 64;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<sbs::CooPcgSolver<float>> f32;
+    std::unique_ptr<sbs::CooPcgSolver<double>> f64;
 };
 
 namespace {
@@ -375,6 +389,7 @@ int sbs_collision_set_stream(SbsCollision* c, void* hip_stream) {
 int sbs_collision_detect(SbsCollision* c, const void* d_X, const void* d_XTilde, int ignore_self, void* d_tI,
                          float* d_normals) {
     return guarded([&] {
+        if (c) c->ignore_self = ignore_self != 0;
         with_detector(c, [&](auto& d) {
             using T = std::remove_reference_t<decltype(d)>;
             using S = std::conditional_t<std::is_same_v<T, sbs::CollisionDetector<double>>, double, float>;
@@ -387,6 +402,7 @@ int sbs_collision_detect(SbsCollision* c, const void* d_X, const void* d_XTilde,
 int sbs_collision_min_step(SbsCollision* c, const void* d_X, const void* d_XTilde, int ignore_self, double* toi) {
     return guarded([&] {
         SBS_REQUIRE(toi != nullptr, "sbs_collision_min_step: null output");
+        if (c) c->ignore_self = ignore_self != 0;
         with_detector(c, [&](auto& d) {
             using T = std::remove_reference_t<decltype(d)>;
             using S = std::conditional_t<std::is_same_v<T, sbs::CollisionDetector<double>>, double, float>;
@@ -404,6 +420,88 @@ int sbs_collision_num_queries(SbsCollision* c, long long* n) {
 
 int sbs_collision_get_queries(SbsCollision* c, uint32_t* queries, double* toi, double* normals) {
     return guarded([&] { with_detector(c, [&](auto& d) { d.get_queries(queries, toi, normals); }); });
+}
+
+int sbs_collision_set_build_type(SbsCollision* c, int build_type) {
+    return guarded([&] { with_detector(c, [&](auto& d) { d.set_build_type(build_type); }); });
+}
+
+int sbs_collision_get_build_type(SbsCollision* c, int* build_type) {
+    return guarded([&] {
+        SBS_REQUIRE(build_type != nullptr, "sbs_collision_get_build_type: null output");
+        with_detector(c, [&](auto& d) { *build_type = d.build_type(); });
+    });
+}
+
+int sbs_collision_update_queries(SbsCollision* c, const void* d_X, double dhat, long long* num_queries) {
+    return guarded([&] {
+        SBS_REQUIRE(c != nullptr, "null SbsCollision handle");
+        SBS_REQUIRE(c->precision == 64, "sbs_collision_update_queries: the reference calls UpdateQueries from the "
+                                        "FP64 IPC solver only; create the detector with precision 64");
+        SBS_REQUIRE(d_X != nullptr, "sbs_collision_update_queries: null X");
+        SBS_REQUIRE(dhat > 0, "sbs_collision_update_queries: dhat must be positive");
+        const long long n = sbs::update_queries_on(*c->f64, static_cast<const double*>(d_X), dhat, c->ignore_self,
+                                                   c->prox_q, c->prox_type, c->prox_d);
+        if (num_queries) *num_queries = n;
+    });
+}
+
+int sbs_collision_get_proximity(SbsCollision* c, uint32_t* queries, int* distance_types, double* distances) {
+    return guarded([&] {
+        SBS_REQUIRE(c != nullptr, "null SbsCollision handle");
+        const size_t n = c->prox_d.size();
+        if (queries) std::copy(c->prox_q.begin(), c->prox_q.end(), queries);
+        if (distance_types) std::copy(c->prox_type.begin(), c->prox_type.begin() + n, distance_types);
+        if (distances) std::copy(c->prox_d.begin(), c->prox_d.begin() + n, distances);
+    });
+}
+
+int sbs_linear_create(int precision, int max_iter, double tolerance, SbsLinear** out) {
+    return guarded([&] {
+        SBS_REQUIRE(out != nullptr, "sbs_linear_create: null output");
+        SBS_REQUIRE(precision == 32 || precision == 64, "sbs_linear_create: precision must be 32 or 64");
+        SBS_REQUIRE(max_iter >= 0 && tolerance >= 0, "sbs_linear_create: negative limit");
+        auto h = std::make_unique<SbsLinear>();
+        h->precision = precision;
+        if (precision == 64) h->f64 = std::make_unique<sbs::CooPcgSolver<double>>(max_iter, tolerance);
+        else h->f32 = std::make_unique<sbs::CooPcgSolver<float>>(max_iter, tolerance);
+        *out = h.release();
+    });
+}
+
+int sbs_linear_destroy(SbsLinear* h) {
+    return guarded([&] { delete h; });
+}
+
+int sbs_linear_set_stream(SbsLinear* h, void* hip_stream) {
+    return guarded([&] {
+        SBS_REQUIRE(h != nullptr, "null SbsLinear handle");
+        h->stream = static_cast<hipStream_t>(hip_stream);
+    });
+}
+
+int sbs_linear_solve(SbsLinear* h, int n, void* d_b, void* d_x, const void* d_A, int nz, const int* d_rowIdx,
+                     const int* d_colIdx, const void* d_guess, int* iterations) {
+    return guarded([&] {
+        SBS_REQUIRE(h != nullptr, "null SbsLinear handle");
+        int it = 0;
+        if (h->precision == 64)
+            it = h->f64->solve(n, static_cast<const double*>(d_b), static_cast<double*>(d_x),
+                               static_cast<const double*>(d_A), nz, d_rowIdx, d_colIdx,
+                               static_cast<const double*>(d_guess), h->stream);
+        else
+            it = h->f32->solve(n, static_cast<const float*>(d_b), static_cast<float*>(d_x),
+                               static_cast<const float*>(d_A), nz, d_rowIdx, d_colIdx,
+                               static_cast<const float*>(d_guess), h->stream);
+        if (iterations) *iterations = it;
+    });
+}
+
+int sbs_linear_stats(SbsLinear* h, long long* nnz) {
+    return guarded([&] {
+        SBS_REQUIRE(h != nullptr, "null SbsLinear handle");
+        if (nnz) *nnz = h->precision == 64 ? h->f64->last_nnz() : h->f32->last_nnz();
+    });
 }
 
 int sbs_collision_stats(SbsCollision* c, long long* candidate_pairs, int* hits) {

@@ -320,6 +320,38 @@ __global__ void refit_kernel(int n, const uint32_t* order, const T* leaf_box, co
     }
 }
 
+// The Serial build type: leaves' boxes first, then one launch per level.  In
+// launch k an internal node whose children were both fitted in earlier
+// launches (ready < k; leaves count as fitted before launch 1) writes their
+// union and records ready = k.  A child fitted in the same launch has
+// ready = k or is not yet visible, so it is never read half written.
+template <class T>
+__global__ void leaf_boxes_kernel(int n, const uint32_t* order, const T* leaf_box, T* node_box) {
+    const int k = blockIdx.x * kTpbC + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t tri = order[k];
+    for (int c = 0; c < 6; ++c) node_box[6 * ((size_t)k + n - 1) + c] = leaf_box[6 * (size_t)tri + c];
+}
+
+template <class T>
+__global__ void refit_level_kernel(int n, int launch, const int* left, const int* right, int* ready, T* node_box) {
+    const int i = blockIdx.x * kTpbC + threadIdx.x;
+    if (i >= n - 1 || ready[i] < launch) return;
+    const int l = left[i], r = right[i];
+    const int rl = l >= n - 1 ? 0 : __hip_atomic_load(&ready[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int rr = r >= n - 1 ? 0 : __hip_atomic_load(&ready[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (rl >= launch || rr >= launch) return;
+    T u[6];
+    union_box(node_box + 6 * (size_t)l, node_box + 6 * (size_t)r, u);
+    for (int c = 0; c < 6; ++c) node_box[6 * (size_t)i + c] = u[c];
+    __hip_atomic_store(&ready[i], launch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void ready_init_kernel(int n, int* ready) {
+    const int i = blockIdx.x * kTpbC + threadIdx.x;
+    if (i < n) ready[i] = 0x7fffffff;
+}
+
 template <class T> __device__ __forceinline__ bool overlap(const T* a, const T* b) {
     for (int k = 0; k < 3; ++k)
         if (a[3 + k] < b[k] || a[k] > b[3 + k]) return false;
@@ -740,6 +772,11 @@ CollisionDetector<T>::CollisionDetector(int num_verts, int num_tris, const uint3
     SBS_HIP(hipHostMalloc((void**)&host_, 4 * sizeof(unsigned long long), hipHostMallocDefault));
 }
 
+template <class T> void CollisionDetector<T>::set_build_type(int t) {
+    SBS_REQUIRE(t >= 0 && t <= 2, "CollisionDetection::SetBuildType: 0 Serial, 1 Atomic or 2 Cooperative");
+    build_type_ = t;
+}
+
 template <class T> CollisionDetector<T>::~CollisionDetector() {
     if (host_) (void)hipHostFree(host_);
 }
@@ -761,11 +798,28 @@ template <class T> void CollisionDetector<T>::broad_phase(const T* X, const T* X
                                                order_in_.get(), order_.get(), n, 0, 64, stream_));
     hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n,
                        (const unsigned long long*)keys_sorted_.get(), left_.get(), right_.get(), parent_.get());
-    SBS_HIP(hipMemsetAsync(flags_.get(), 0, flags_.bytes(), stream_));
-    hipLaunchKernelGGL(refit_kernel<T>, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n, (const uint32_t*)order_.get(),
-                       (const T*)leaf_box_.get(), (const int*)left_.get(), (const int*)right_.get(),
-                       (const int*)parent_.get(), flags_.get(), node_box_.get());
-    SBS_HIP(hipGetLastError());
+    if (build_type_ == 0) {
+        hipLaunchKernelGGL(leaf_boxes_kernel<T>, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n,
+                           (const uint32_t*)order_.get(), (const T*)leaf_box_.get(), node_box_.get());
+        hipLaunchKernelGGL(ready_init_kernel, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n - 1, flags_.get());
+        SBS_HIP(hipGetLastError());
+        // a tree of n leaves has at most n - 1 levels above them
+        int root_ready = 0x7fffffff;
+        for (int launch = 1; launch <= n - 1 && root_ready == 0x7fffffff; ++launch) {
+            hipLaunchKernelGGL(refit_level_kernel<T>, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n, launch,
+                               (const int*)left_.get(), (const int*)right_.get(), flags_.get(), node_box_.get());
+            SBS_HIP(hipGetLastError());
+            SBS_HIP(hipMemcpyAsync(&root_ready, flags_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+            SBS_HIP(hipStreamSynchronize(stream_));
+        }
+        SBS_REQUIRE(root_ready != 0x7fffffff, "CollisionDetection: serial BVH fit did not reach the root");
+    } else {
+        SBS_HIP(hipMemsetAsync(flags_.get(), 0, flags_.bytes(), stream_));
+        hipLaunchKernelGGL(refit_kernel<T>, dim3(blocks_for(n)), dim3(kTpbC), 0, stream_, n,
+                           (const uint32_t*)order_.get(), (const T*)leaf_box_.get(), (const int*)left_.get(),
+                           (const int*)right_.get(), (const int*)parent_.get(), flags_.get(), node_box_.get());
+        SBS_HIP(hipGetLastError());
+    }
     // each leaf's walk is split over the 2^depth subtrees below the root, so
     // that about 2^18 threads walk (one per leaf would leave most CUs idle)
     int depth = 0;

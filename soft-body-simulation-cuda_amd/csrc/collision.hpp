@@ -51,6 +51,15 @@ public:
     // Host copy of the last query list (sorted, unique) with toi and normals.
     void get_queries(uint32_t* q5, double* toi, double* n3) const;
     void set_stream(hipStream_t s) { stream_ = s; }
+    // SetBuildType / GetBuildType (bvh.h:67-68): BVH<Scalar>::BuildType
+    // Serial (0), Atomic (1, the default), Cooperative (2) choose how the
+    // boxes of the internal nodes are fitted (bvh.cu:118-135).  Serial runs
+    // one launch per tree level until the root is fitted; Atomic and
+    // Cooperative run the one-launch bottom-up climb (a grid-wide barrier
+    // costs more than that climb on this GPU).  Internal boxes are exact
+    // unions either way, so every type gives the same queries.
+    void set_build_type(int t);
+    int build_type() const { return build_type_; }
 
 private:
     void broad_phase(const T* X, const T* XT, bool ignore_self, T bound = 0);
@@ -84,6 +93,7 @@ private:
     DevBuf<T> red_;                      // min-toi partials
     long long nq_ = 0, npairs_ = 0;
     int hits_ = 0;
+    int build_type_ = 1;
     unsigned long long* host_ = nullptr;  // pinned scratch for counts
 };
 

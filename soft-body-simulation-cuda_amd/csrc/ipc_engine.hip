@@ -5,9 +5,13 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <tuple>
 
 #include "ipc_distance.hpp"
 #include "ipc_engine.hpp"
+
+#include <algorithm>
+#include <tuple>
 #include "svd3.hpp"
 
 namespace sbs {
@@ -969,6 +973,46 @@ void IpcEngine::update_queries(const double* X) {
     SBS_HIP(hipMemcpyAsync(&nk, kept, sizeof(nk), hipMemcpyDeviceToHost, stream_));
     SBS_HIP(hipStreamSynchronize(stream_));
     stats_.queries = (long long)nk;
+}
+
+// UpdateQueries (broadphase.cu:574-587) as CollisionDetection exposes it: the
+// candidates of the triangle pairs whose boxes at X, grown by 2 dhat, overlap
+// (the detector's last ignore-self flag), their distance types and squared
+// distances, and the queries with d <= dhat, sorted by (d, distance type) as
+// the reference's final sort orders them, ties by the canonical vertex list.
+long long update_queries_on(CollisionDetector<double>& cd, const double* X, double dhat, bool ignore_self,
+                            std::vector<uint32_t>& q5, std::vector<int>& dtype, std::vector<double>& dist) {
+    q5.clear();
+    dtype.clear();
+    dist.clear();
+    const long long n = cd.proximity(X, 2 * dhat, ignore_self);
+    if (n == 0) return 0;
+    DevBuf<int> t(n);
+    DevBuf<double> d(n);
+    DevBuf<unsigned long long> kept(1);
+    SBS_HIP(hipMemset(kept.get(), 0, sizeof(unsigned long long)));
+    hipLaunchKernelGGL(ipc_query_kernel, dim3(nblk(n)), dim3(kT), 0, nullptr, n, cd.queries(), X, dhat, t.get(),
+                       d.get(), kept.get());
+    SBS_HIP(hipGetLastError());
+    std::vector<QueryKey> qh(n);
+    std::vector<int> th(n);
+    std::vector<double> dh(n);
+    SBS_HIP(hipMemcpy(qh.data(), cd.queries(), n * sizeof(QueryKey), hipMemcpyDeviceToHost));
+    SBS_HIP(hipMemcpy(th.data(), t.get(), n * sizeof(int), hipMemcpyDeviceToHost));
+    SBS_HIP(hipMemcpy(dh.data(), d.get(), n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<long long> idx;
+    for (long long i = 0; i < n; ++i)
+        if (dh[i] <= dhat) idx.push_back(i);
+    auto key = [&](long long i) {
+        return std::make_tuple(dh[i], th[i], qh[i].type, qh[i].v0, qh[i].v1, qh[i].v2, qh[i].v3);
+    };
+    std::sort(idx.begin(), idx.end(), [&](long long a, long long b) { return key(a) < key(b); });
+    for (long long i : idx) {
+        q5.insert(q5.end(), {qh[i].type, qh[i].v0, qh[i].v1, qh[i].v2, qh[i].v3});
+        dtype.push_back(th[i]);
+        dist.push_back(dh[i]);
+    }
+    return (long long)idx.size();
 }
 
 double IpcEngine::energy(const double* X) {

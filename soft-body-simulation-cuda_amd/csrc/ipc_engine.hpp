@@ -45,6 +45,12 @@ struct IpcData {
 
 enum : int { kElasticCorotated = 0, kElasticNeoHookean = 1 };
 
+// UpdateQueries (broadphase.cu:574-587) on a detector's triangles: the kept
+// queries (d <= dhat) on the host, 5 u32 each, with their distance types and
+// squared distances, sorted by (d, type).  Returns their number.
+long long update_queries_on(CollisionDetector<double>& cd, const double* X, double dhat, bool ignore_self,
+                            std::vector<uint32_t>& q5, std::vector<int>& dtype, std::vector<double>& dist);
+
 struct IpcParams {
     double dt = 0.01;
     double damp = 0.999;

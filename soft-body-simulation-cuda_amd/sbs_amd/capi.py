@@ -133,6 +133,15 @@ SIGNATURES = {
     "sbs_fixed_destroy": (_I, [_VP]),
     "sbs_fixed_handle_collisions": (_I, [_VP, _I, _VP, _VP, _I, _D, _D, _VP]),
     "sbs_fixed_params": (_I, [_VP, _VP]),
+    "sbs_collision_set_build_type": (_I, [_VP, _I]),
+    "sbs_collision_get_build_type": (_I, [_VP, ctypes.POINTER(_I)]),
+    "sbs_collision_update_queries": (_I, [_VP, _VP, _D, ctypes.POINTER(ctypes.c_longlong)]),
+    "sbs_collision_get_proximity": (_I, [_VP, _VP, _VP, _VP]),
+    "sbs_linear_create": (_I, [_I, _I, _D, ctypes.POINTER(_VP)]),
+    "sbs_linear_destroy": (_I, [_VP]),
+    "sbs_linear_set_stream": (_I, [_VP, _VP]),
+    "sbs_linear_solve": (_I, [_VP, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, ctypes.POINTER(_I)]),
+    "sbs_linear_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
     "sbs_ipc_create": (_I, [ctypes.POINTER(SbsIpcData), ctypes.POINTER(_VP)]),
     "sbs_ipc_destroy": (_I, [_VP]),
     "sbs_ipc_update": (_I, [_VP, ctypes.POINTER(SbsIpcData), ctypes.POINTER(SbsIpcParams), ctypes.POINTER(_I)]),

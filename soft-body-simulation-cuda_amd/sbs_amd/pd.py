@@ -299,6 +299,30 @@ class CollisionDetection:
                                                   ctypes.byref(t)))
         return t.value
 
+    def UpdateQueries(self, X, dhat: float):
+        """UpdateQueries(numVerts, numTris, Tri, X, TriFathers, dhat) (bvh.h:52,
+        broadphase.cu:574-587), precision 64: (queries (n, 5) u32, distance
+        types (n,), squared distances (n,)) of the kept queries, sorted by
+        (distance, type)."""
+        self._x.upload(np.ascontiguousarray(X, dtype=self.dtype).reshape(-1))
+        n = ctypes.c_longlong()
+        capi.check(self._L.sbs_collision_update_queries(self.h, self._x.ptr, float(dhat), ctypes.byref(n)))
+        q = np.zeros((n.value, 5), np.uint32)
+        t = np.zeros(n.value, np.int32)
+        d = np.zeros(n.value)
+        if n.value:
+            capi.check(self._L.sbs_collision_get_proximity(self.h, q.ctypes.data, t.ctypes.data, d.ctypes.data))
+        return q, t, d
+
+    def SetBuildType(self, build_type: int) -> None:
+        """BVH<Scalar>::BuildType: 0 Serial, 1 Atomic (default), 2 Cooperative."""
+        capi.check(self._L.sbs_collision_set_build_type(self.h, int(build_type)))
+
+    def GetBuildType(self) -> int:
+        t = ctypes.c_int()
+        capi.check(self._L.sbs_collision_get_build_type(self.h, ctypes.byref(t)))
+        return t.value
+
     def GetNumQueries(self) -> int:
         n = ctypes.c_longlong()
         capi.check(self._L.sbs_collision_num_queries(self.h, ctypes.byref(n)))

@@ -87,12 +87,33 @@ def test_contact_steps_match_oracle(gpu):
     assert any(sg["kappa"] > 100.0 for _, _, sg, _ in out)     # the barrier was engaged
 
 
-def test_config4_first_steps_match_oracle(gpu):
-    """The 16^3 pair itself: the steps up to and into contact."""
+def test_config4_steps_match_oracle(gpu):
+    """The 16^3 pair itself: the first three steps from rest, then, from the
+    GPU run's state once the barrier is engaged (the lower cube sags with the
+    upper one, so the 5 mm gap closes only after some 20 steps), three contact
+    steps of both."""
     s = config4_scene()
-    out = run_pair(s, IpcParams(dt=0.01, damp=0.999, dhat=1e-3, tol=5e-2, lin_tol=1e-10, lin_max_iter=5000), 4)
+    p = IpcParams(dt=0.01, damp=0.999, dhat=1e-3, tol=5e-2, lin_tol=1e-10, lin_max_iter=5000)
+    check_steps(s, run_pair(s, p, 3))
+    data = IpcData(s)
+    solver = IpcSolver(data, elastic="neohookean")
+    try:
+        for k in range(100):
+            assert solver.Update(data, p), solver.stats()
+            if solver.stats()["kappa"] > 100.0:
+                break
+        assert solver.stats()["kappa"] > 100.0, "the barrier never engaged"
+        o = oracle_for(s, p)
+        o.set_xv(data.get("X"), data.get("V"))
+        out = []
+        for k in range(3):
+            assert solver.Update(data, p) and o.step()
+            out.append((data.get("X"), o.get()[0], solver.stats(), o.stats()))
+    finally:
+        solver.close()
+        data.free()
     check_steps(s, out)
-    assert out[-1][2]["queries"] > 0     # the faces came within the query distance
+    assert all(sg["queries"] > 0 for _, _, sg, _ in out)
 
 
 def _config4_motion(drop):
