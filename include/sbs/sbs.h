@@ -57,6 +57,15 @@ extern "C" {
  * CholeskyDnLinearSolver path (src/simulation/solver/linear/cholesky.cu:26-70,
  * 161-166) for scenes of small bodies (contiguous vertex ranges). */
 #define SBS_SOLVER_DENSE_CHOLESKY 4
+/* Batched bodies: every connected body of the scene (the bodies the
+ * reference's DataLoader concatenates, dataLoader.cu:292-378) is solved on its
+ * own -- its own Jacobi-PCG (alpha, beta and stop test per body,
+ * pcgJacobi.cu:89-173) + Chebyshev -- with one workgroup per body running the
+ * whole step in one persistent launch per body-size class; the classes'
+ * workgroup sizes come from an autotuner run at the first batched update
+ * (SbsSolverParams.batch_autotune).  Results equal one PdSolver per body with
+ * SBS_SOLVER_PCG and pcg_chebyshev.  moreDBC is not taken. */
+#define SBS_SOLVER_BATCHED_PCG 5
 
 /* Mirrors SolverData<Scalar> (src/def.h:22-56); pointers are device memory
  * holding float (precision 32) or double (precision 64). */
@@ -112,6 +121,9 @@ typedef struct SbsSolverParams {
                              caps; no collision handling, perf or kernel timing).  On MI355X
                              a replay costs what the eager launches cost (64^3 cube: 248 vs
                              256 steps/s; 8^3: 660 vs 654), so 0 is the usual choice */
+    int batch_autotune;   /* SBS_SOLVER_BATCHED_PCG: 1 times the workgroup sizes of each
+                             body-size class at the first batched update (state unchanged);
+                             0 keeps one thread per 16 tets of the class's largest body */
 } SbsSolverParams;
 
 typedef struct SbsPd SbsPd;
@@ -200,6 +212,15 @@ int sbs_pd_export_system(SbsPd* pd, void* c, void* diagA, void* diagL, void* V0,
  * keep the query first in (v1, v2, v3) order, and a vertex touched by several
  * hits takes the normal of the last one in the sorted query list. */
 typedef struct SbsCollision SbsCollision;
+/* Batched solver of the handle (after an update with SBS_SOLVER_BATCHED_PCG):
+ * bodies and size classes; per class info = {min tets, max tets, bodies,
+ * threads per workgroup} and the tuner's (threads, ms per step) pairs; per
+ * body 6 ints {vertices, tets, PCG iterations, solves, most iterations of one
+ * solve, solves stopped by pcg_max_iter} accumulated over the updates. */
+int sbs_pd_batch_info(SbsPd* pd, int* num_bodies, int* num_classes);
+int sbs_pd_batch_class(SbsPd* pd, int cls, int* info, double* tuned_ms, int max_tried, int* num_tried);
+int sbs_pd_batch_bodies(SbsPd* pd, int* stats);
+
 int sbs_collision_create(int precision, int num_verts, int num_tris, const uint32_t* d_tri,
                          const uint32_t* d_tri_fathers, SbsCollision** out);
 int sbs_collision_destroy(SbsCollision* c);

@@ -14,6 +14,7 @@
 #include "linear_solver.hpp"
 #include "direct_solver.hpp"
 #include "pd_engine.hpp"
+#include "batch_engine.hpp"
 #include "scene_loader.hpp"
 
 namespace sbs {
@@ -128,6 +129,7 @@ sbs::PdParams to_params(const SbsSolverParams* p) {
     o.mu_n = p->mu_n;
     o.mu_t = p->mu_t;
     o.use_graph = p->use_graph;
+    o.batch_autotune = p->batch_autotune;
     return o;
 }
 
@@ -171,7 +173,7 @@ int sbs_pd_destroy(SbsPd* pd) {
 
 int sbs_pd_set_global_solver(SbsPd* pd, int solver_type) {
     return guarded([&] {
-        SBS_REQUIRE(solver_type >= 0 && solver_type <= 4, "sbs_pd_set_global_solver: unknown solver type");
+        SBS_REQUIRE(solver_type >= 0 && solver_type <= 5, "sbs_pd_set_global_solver: unknown solver type");
         with_engine(pd, [&](auto& e) { e.set_global_solver(static_cast<sbs::GlobalSolver>(solver_type)); });
     });
 }
@@ -360,6 +362,56 @@ int sbs_pd_dense_info(SbsPd* pd, int* bodies, int* max_body, long long* inverse_
                       double* last_ratio) {
     return guarded(
         [&] { with_engine(pd, [&](auto& e) { e.dense_info(bodies, max_body, inverse_bytes, last_iterations, last_ratio); }); });
+}
+
+int sbs_pd_batch_info(SbsPd* pd, int* num_bodies, int* num_classes) {
+    return guarded([&] {
+        with_engine(pd, [&](auto& e) {
+            auto* b = e.batch();
+            SBS_REQUIRE(b != nullptr, "sbs_pd_batch_info: no batched update has run on this handle");
+            if (num_bodies) *num_bodies = b->num_bodies();
+            if (num_classes) *num_classes = (int)b->classes().size();
+        });
+    });
+}
+
+int sbs_pd_batch_class(SbsPd* pd, int cls, int* info, double* tuned_ms, int max_tried, int* num_tried) {
+    return guarded([&] {
+        with_engine(pd, [&](auto& e) {
+            auto* b = e.batch();
+            SBS_REQUIRE(b != nullptr, "sbs_pd_batch_class: no batched update has run on this handle");
+            SBS_REQUIRE(cls >= 0 && cls < (int)b->classes().size(), "sbs_pd_batch_class: class out of range");
+            const auto& c = b->classes()[cls];
+            if (info) {
+                info[0] = c.min_tets;
+                info[1] = c.max_tets;
+                info[2] = c.bodies;
+                info[3] = c.threads;
+            }
+            const int n = (int)c.tried.size();
+            if (num_tried) *num_tried = n;
+            for (int k = 0; tuned_ms && k < std::min(n, max_tried); ++k) {
+                tuned_ms[2 * k] = c.tried[k].first;
+                tuned_ms[2 * k + 1] = c.tried[k].second;
+            }
+        });
+    });
+}
+
+int sbs_pd_batch_bodies(SbsPd* pd, int* stats) {
+    return guarded([&] {
+        SBS_REQUIRE(stats != nullptr, "sbs_pd_batch_bodies: null output");
+        with_engine(pd, [&](auto& e) {
+            auto* b = e.batch();
+            SBS_REQUIRE(b != nullptr, "sbs_pd_batch_bodies: no batched update has run on this handle");
+            const auto v = b->body_stats();
+            for (size_t i = 0; i < v.size(); ++i) {
+                int* o = stats + 6 * i;
+                o[0] = v[i].verts; o[1] = v[i].tets; o[2] = v[i].iterations;
+                o[3] = v[i].solves; o[4] = v[i].max_per_solve; o[5] = v[i].unconverged;
+            }
+        });
+    });
 }
 
 int sbs_collision_create(int precision, int num_verts, int num_tris, const uint32_t* d_tri,

@@ -11,6 +11,7 @@
 #include "direct_solver.hpp"
 #include "halo_tiles.hpp"
 #include "pd_engine.hpp"
+#include "batch_engine.hpp"
 #include "svd3.hpp"
 
 namespace sbs {
@@ -1743,6 +1744,7 @@ template <class T> void PdEngine<T>::prepare(PdData<T>& data, const PdParams& pa
     SBS_HIP(hipMemcpyAsync(data.DBCX, data.X0, n3 * sizeof(T), hipMemcpyDeviceToDevice, stream_));
     direct_.reset();
     dense_.reset();
+    batch_.reset();
     if (is_direct()) direct_factor(params);
     SBS_HIP(hipStreamSynchronize(stream_));
     ready_ = true;
@@ -1774,6 +1776,41 @@ template <class T> void PdEngine<T>::direct_factor(const PdParams& params) {
     } else {
         direct_ = std::make_unique<BandCholesky<T>>();
         direct_->factor(nv_, ex.rowptr, ex.col, ex.val, stream_);
+    }
+}
+
+// BatchedPCG: the bodies (connected components), their systems from the
+// assembled matrix, and the tuned workgroup size of each size class.
+template <class T> void PdEngine<T>::build_batch(const PdParams& params) {
+    SBS_REQUIRE(system_built_, "batched solver requested before the system was assembled");
+    drop_graph();
+    Export ex = export_system();
+    BatchSystem<T> sys;
+    sys.nv = nv_;
+    sys.nt = nt_;
+    sys.tets = host_tets_;
+    sys.dminv = std::move(ex.dminv);
+    sys.w = std::move(ex.w);
+    sys.c0 = std::move(ex.c);
+    sys.diagA = std::move(ex.diagA);
+    sys.rowptr = std::move(ex.rowptr);
+    sys.col = std::move(ex.col);
+    sys.val = std::move(ex.val);
+    sys.mass.resize(nv_);
+    sys.dbc.resize(nv_);
+    SBS_HIP(hipMemcpy(sys.mass.data(), data_.mass, nv_ * sizeof(T), hipMemcpyDeviceToHost));
+    SBS_HIP(hipMemcpy(sys.dbc.data(), data_.DBC, nv_ * sizeof(T), hipMemcpyDeviceToHost));
+    batch_ = std::make_unique<BatchEngine<T>>(sys);
+    if (params.batch_autotune) {
+        BatchParams bp;
+        bp.iterations = params.num_iterations;
+        bp.dt = params.dt;
+        bp.gravity = params.gravity;
+        bp.rho = params.rho;
+        bp.pcg_tol = params.pcg_tol;
+        bp.pcg_floor = params.pcg_floor;
+        bp.pcg_max_iter = params.pcg_max_iter;
+        batch_->autotune(data_.X, data_.V, data_.DBCX, bp, stream_);
     }
 }
 
@@ -1997,6 +2034,10 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
         dense_.reset();
         direct_factor(params);
     }
+    if (solver_ == GlobalSolver::BatchedPCG) {
+        SBS_REQUIRE(data.moreDBC == nullptr, "PdSolver: the batched solver does not take moreDBC weights");
+        if (!batch_) build_batch(params);
+    }
     // this step's pinned counter slot was last written two steps ago: fold it in
     const int slot = (int)(steps_ & 1);
     fold_slot(slot);
@@ -2011,7 +2052,8 @@ template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& par
     if (pcg)
         for (int i = 0; i < params.num_iterations; ++i) caps_known = caps_known && cap_[i] > 0;
     const bool graph = params.use_graph && stream_ != nullptr && !perf_ && timing_mask_ == 0 && !params.handle_collision &&
-                       (solver_ == GlobalSolver::Jacobi || (pcg && params.pcg_adaptive && caps_known));
+                       (solver_ == GlobalSolver::Jacobi || solver_ == GlobalSolver::BatchedPCG ||
+                        (pcg && params.pcg_adaptive && caps_known));
     if (graph) {
         const std::vector<uint64_t> key = step_key(data, params);
         if (!graph_exec_ || key != graph_key_ || graph_caps_stale(pcg ? params.num_iterations : 0)) {
@@ -2129,80 +2171,102 @@ template <class T> void PdEngine<T>::enqueue_step(PdData<T>& data, const PdParam
     // between two buffers.  The last solve keeps its finish launch.
     const bool fold = pcg && cheb && params.pcg_halo && halo_groups_ > 0 && chunk_rows_.get() != nullptr;
     hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, stream_, ctl);
-    hipLaunchKernelGGL(begin_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, data.X, data.V, data.mass,
-                       data.DBC, data.moreDBC, (const T*)data.DBCX, dt, (T)params.gravity, c_.get(), sn_.get(),
-                       sn_old_.get(), prev_.get(),
-                       (int)cheb, fold ? sn_alt_.get() : (T*)nullptr, fold ? prev_alt_.get() : (T*)nullptr);
-    SBS_HIP(hipGetLastError());
-    T omega = 1;
-    const T rho = (T)params.rho;
-    const int* skip = cheb ? nullptr : &ctl->outer_done;
-    T* sn_cur = sn_.get();
-    T* prev_cur = prev_.get();
-    T* sn_nxt = sn_alt_.get();
-    T* prev_nxt = prev_alt_.get();
-    int open_L = 0;  // launches of the solve whose finish is pending (0: none)
-    for (int i = 0; i < params.num_iterations; ++i) {
+    if (solver_ == GlobalSolver::BatchedPCG) {
+        // every body's whole step in one persistent launch per size class
+        BatchParams bp;
+        bp.iterations = params.num_iterations;
+        bp.dt = params.dt;
+        bp.gravity = params.gravity;
+        bp.rho = params.rho;
+        bp.pcg_tol = params.pcg_tol;
+        bp.pcg_floor = params.pcg_floor;
+        bp.pcg_max_iter = params.pcg_max_iter;
+        bp.write_x = params.handle_collision == 0;
         if (perf_) SBS_HIP(hipEventRecord(ev[0], stream_));
-        if (open_L > 0) {
-            launch_local_fold(cg_x_.get(), sn_cur, prev_cur, sn_nxt, prev_nxt, omega, open_L, params);
-            std::swap(sn_cur, sn_nxt);
-            std::swap(prev_cur, prev_nxt);
-            open_L = 0;
-        } else {
-            launch_local(sn_cur, jacobi, skip);
-        }
-        if (perf_) SBS_HIP(hipEventRecord(ev[1], stream_));
-        if (cheb) {
-            if (i <= 10) omega = 1;
-            else if (i == 11) omega = (T)2 / ((T)2 - rho * rho);
-            else omega = (T)4 / ((T)4 - rho * rho * omega);
-        }
-        if (jacobi) {
-            tick(kKJacobi, true);
-            hipLaunchKernelGGL(jacobi_chebyshev_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, diagL_.get(),
-                               sn_.get(), prev_.get(), omega);
-            tick(kKJacobi, false);
-        } else if (pcg) {
-            // PCG warm-started from the current iterate, then the Chebyshev
-            // recurrence, or computeError and the early exit
-            const bool defer = fold && i + 1 < params.num_iterations;
-            const int L = launch_pcg(1, nullptr, sn_cur, next_.get(), cheb, omega, params, skip, i,
-                                     !params.pcg_adaptive, sn_cur, prev_cur, defer);
-            if (defer) open_L = L;
-            if (!cheb) {
-                hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_,
-                                   (const T*)next_.get(), sn_.get(), prev_.get(), slots_.get(), skip);
-                hipLaunchKernelGGL(outer_decide_kernel, dim3(1), dim3(64), 0, stream_, (const double*)slots_.get(),
-                                   nv_, params.tol, ctl, skip);
-            }
-        } else {
-            tick(kKRhs, true);
-            hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, rhs_.get(), skip);
-            tick(kKRhs, false);
-            tick(kKDirect, true);
-            direct_solve(rhs_.get(), sn_.get(), params);
-            tick(kKDirect, false);
-            hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(),
-                               (T*)nullptr, prev_.get(), slots_.get(), skip);
-            hipLaunchKernelGGL(outer_decide_kernel, dim3(1), dim3(64), 0, stream_, (const double*)slots_.get(), nv_,
-                               params.tol, ctl, skip);
-        }
-        SBS_HIP(hipGetLastError());
+        batch_->step(data.X, data.V, data.XTilde, data.DBCX, bp, stream_);
         if (perf_) {
-            SBS_HIP(hipEventRecord(ev[2], stream_));
-            SBS_HIP(hipEventSynchronize(ev[2]));
-            float a = 0, gm = 0;
-            SBS_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-            SBS_HIP(hipEventElapsedTime(&gm, ev[1], ev[2]));
-            perf_data_[0].ms += a;
-            perf_data_[1].ms += gm;
+            SBS_HIP(hipEventRecord(ev[1], stream_));
+            SBS_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            SBS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            perf_data_[1].ms += ms;  // local and global steps run fused, counted as the global step
         }
+    } else {
+        hipLaunchKernelGGL(begin_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, data.X, data.V, data.mass,
+                           data.DBC, data.moreDBC, (const T*)data.DBCX, dt, (T)params.gravity, c_.get(), sn_.get(),
+                           sn_old_.get(), prev_.get(),
+                           (int)cheb, fold ? sn_alt_.get() : (T*)nullptr, fold ? prev_alt_.get() : (T*)nullptr);
+        SBS_HIP(hipGetLastError());
+        T omega = 1;
+        const T rho = (T)params.rho;
+        const int* skip = cheb ? nullptr : &ctl->outer_done;
+        T* sn_cur = sn_.get();
+        T* prev_cur = prev_.get();
+        T* sn_nxt = sn_alt_.get();
+        T* prev_nxt = prev_alt_.get();
+        int open_L = 0;  // launches of the solve whose finish is pending (0: none)
+        for (int i = 0; i < params.num_iterations; ++i) {
+            if (perf_) SBS_HIP(hipEventRecord(ev[0], stream_));
+            if (open_L > 0) {
+                launch_local_fold(cg_x_.get(), sn_cur, prev_cur, sn_nxt, prev_nxt, omega, open_L, params);
+                std::swap(sn_cur, sn_nxt);
+                std::swap(prev_cur, prev_nxt);
+                open_L = 0;
+            } else {
+                launch_local(sn_cur, jacobi, skip);
+            }
+            if (perf_) SBS_HIP(hipEventRecord(ev[1], stream_));
+            if (cheb) {
+                if (i <= 10) omega = 1;
+                else if (i == 11) omega = (T)2 / ((T)2 - rho * rho);
+                else omega = (T)4 / ((T)4 - rho * rho * omega);
+            }
+            if (jacobi) {
+                tick(kKJacobi, true);
+                hipLaunchKernelGGL(jacobi_chebyshev_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, diagL_.get(),
+                                   sn_.get(), prev_.get(), omega);
+                tick(kKJacobi, false);
+            } else if (pcg) {
+                // PCG warm-started from the current iterate, then the Chebyshev
+                // recurrence, or computeError and the early exit
+                const bool defer = fold && i + 1 < params.num_iterations;
+                const int L = launch_pcg(1, nullptr, sn_cur, next_.get(), cheb, omega, params, skip, i,
+                                         !params.pcg_adaptive, sn_cur, prev_cur, defer);
+                if (defer) open_L = L;
+                if (!cheb) {
+                    hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_,
+                                       (const T*)next_.get(), sn_.get(), prev_.get(), slots_.get(), skip);
+                    hipLaunchKernelGGL(outer_decide_kernel, dim3(1), dim3(64), 0, stream_, (const double*)slots_.get(),
+                                       nv_, params.tol, ctl, skip);
+                }
+            } else {
+                tick(kKRhs, true);
+                hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, rhs_.get(), skip);
+                tick(kKRhs, false);
+                tick(kKDirect, true);
+                direct_solve(rhs_.get(), sn_.get(), params);
+                tick(kKDirect, false);
+                hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(),
+                                   (T*)nullptr, prev_.get(), slots_.get(), skip);
+                hipLaunchKernelGGL(outer_decide_kernel, dim3(1), dim3(64), 0, stream_, (const double*)slots_.get(), nv_,
+                                   params.tol, ctl, skip);
+            }
+            SBS_HIP(hipGetLastError());
+            if (perf_) {
+                SBS_HIP(hipEventRecord(ev[2], stream_));
+                SBS_HIP(hipEventSynchronize(ev[2]));
+                float a = 0, gm = 0;
+                SBS_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+                SBS_HIP(hipEventElapsedTime(&gm, ev[1], ev[2]));
+                perf_data_[0].ms += a;
+                perf_data_[1].ms += gm;
+            }
+        }
+        hipLaunchKernelGGL(end_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_cur, (T)1 / dt,
+                           data.XTilde, data.V, params.handle_collision ? (T*)nullptr : data.X, data.moreDBC);
+        SBS_HIP(hipGetLastError());
     }
     const bool collide = params.handle_collision != 0;
-    hipLaunchKernelGGL(end_step_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_cur, (T)1 / dt,
-                       data.XTilde, data.V, collide ? (T*)nullptr : data.X, data.moreDBC);
-    SBS_HIP(hipGetLastError());
     if (collide) {
         // DetectCollision between bodies on the step's motion X -> XTilde, then
         // CCDKernel (pdSolver.cu:218-224)

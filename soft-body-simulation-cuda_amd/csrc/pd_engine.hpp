@@ -36,8 +36,12 @@ namespace sbs {
 template <class T> class BandCholesky;
 template <class T> class DenseRefinedSolver;
 template <class T> class CollisionDetector;
+template <class T> class BatchEngine;
 
-enum class GlobalSolver : int { Jacobi = 0, CuSolverCholesky = 1, EigenCholesky = 2, PCG = 3, DenseCholesky = 4 };
+enum class GlobalSolver : int {
+    Jacobi = 0, CuSolverCholesky = 1, EigenCholesky = 2, PCG = 3, DenseCholesky = 4,
+    BatchedPCG = 5  // every connected body with its own Jacobi-PCG, one workgroup per body
+};
 
 // Mirrors the fields of SolverParams<Scalar> (src/def.h:83-100) used by the
 // PD step, plus the settings of the PCG global step.
@@ -58,6 +62,7 @@ struct PdParams {
     int dense_max_iter = 8;     // DenseCholesky: refinement corrections per solve
     int pcg_halo = 1;           // 1: halo-staged PCG kernels (when the tiles could be built); 0: row gathers
     int use_graph = 0;          // replay a captured HIP graph of the step when its launch sequence is fixed
+    int batch_autotune = 1;     // BatchedPCG: time the workgroup sizes of each body-size class at prepare
     double mu_n = 0.5;          // fixed-body response: normal restitution (SolverParams::muN)
     double mu_t = 0.5;          // fixed-body response: friction (SolverParams::muT)
 };
@@ -172,6 +177,8 @@ public:
     // DenseCholesky: bodies, largest body, bytes of the inverse fragments, and
     // the last solve's corrections and residual ratio (waits for the stream).
     void dense_info(int* bodies, int* max_body, long long* bytes, int* iterations, double* ratio);
+    // BatchedPCG: the engine (null before the first batched update)
+    BatchEngine<T>* batch() const { return batch_.get(); }
 
 private:
     void build_chunks(const std::vector<double>& X0h);
@@ -271,6 +278,8 @@ private:
     // direct solver (band Cholesky factor, GPU triangular solves) for small bodies
     std::unique_ptr<BandCholesky<T>> direct_;
     std::unique_ptr<DenseRefinedSolver<T>> dense_;   // per-body dense inverses (DenseCholesky)
+    std::unique_ptr<BatchEngine<T>> batch_;          // per-body persistent solve (BatchedPCG)
+    void build_batch(const PdParams& params);
     int dense_prec_ = -1;
     // collision handling after the step (built on first use)
     std::unique_ptr<CollisionDetector<T>> collision_;

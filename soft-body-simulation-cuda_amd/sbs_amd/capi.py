@@ -20,6 +20,7 @@ SOLVER_CUSOLVER_CHOLESKY = 1
 SOLVER_EIGEN_CHOLESKY = 2
 SOLVER_PCG = 3
 SOLVER_DENSE_CHOLESKY = 4
+SOLVER_BATCHED_PCG = 5
 
 KERNELS = ["local_step", "pcg_solve", "jacobi_chebyshev", "direct_solve", "rhs_gather", "pcg_iteration"]
 
@@ -50,7 +51,8 @@ class SbsSolverParams(ctypes.Structure):
                 ("pcg_tol", ctypes.c_double), ("pcg_floor", ctypes.c_double), ("pcg_max_iter", ctypes.c_int),
                 ("pcg_chebyshev", ctypes.c_int), ("pcg_adaptive", ctypes.c_int), ("dense_precision", ctypes.c_int),
                 ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int), ("pcg_halo", ctypes.c_int),
-                ("mu_n", ctypes.c_double), ("mu_t", ctypes.c_double), ("use_graph", ctypes.c_int)]
+                ("mu_n", ctypes.c_double), ("mu_t", ctypes.c_double), ("use_graph", ctypes.c_int),
+                ("batch_autotune", ctypes.c_int)]
 
 
 class SbsPdStats(ctypes.Structure):
@@ -116,6 +118,9 @@ SIGNATURES = {
     "sbs_pd_export_system": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "sbs_pd_dense_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong),
                                ctypes.POINTER(_I), ctypes.POINTER(_D)]),
+    "sbs_pd_batch_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "sbs_pd_batch_class": (_I, [_VP, _I, _VP, _VP, _I, ctypes.POINTER(_I)]),
+    "sbs_pd_batch_bodies": (_I, [_VP, _VP]),
     "sbs_collision_create": (_I, [_I, _I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
     "sbs_collision_destroy": (_I, [_VP]),
     "sbs_collision_set_stream": (_I, [_VP, _VP]),

@@ -25,6 +25,7 @@ class SolverType(enum.IntEnum):
     EigenCholesky = capi.SOLVER_EIGEN_CHOLESKY
     PCG = capi.SOLVER_PCG
     DenseCholesky = capi.SOLVER_DENSE_CHOLESKY
+    BatchedPCG = capi.SOLVER_BATCHED_PCG
 
 
 @dataclasses.dataclass
@@ -48,6 +49,7 @@ class SolverParams:
     muN: float = 0.5              # fixed-body response: normal restitution
     muT: float = 0.5              # fixed-body response: friction
     use_graph: bool = False       # replay the step as a captured HIP graph (needs a non-null stream)
+    batch_autotune: bool = True   # BatchedPCG: time each size class's workgroup sizes at the first update
 
     def to_c(self) -> capi.SbsSolverParams:
         return capi.SbsSolverParams(int(self.numIterations), float(self.dt), float(self.gravity), float(self.tol),
@@ -55,7 +57,7 @@ class SolverParams:
                                     float(self.pcg_floor), int(self.pcg_max_iter), int(self.pcg_chebyshev),
                                     int(self.pcg_adaptive), int(self.dense_precision), float(self.dense_tol),
                                     int(self.dense_max_iter), int(self.pcg_halo), float(self.muN),
-                                    float(self.muT), int(self.use_graph))
+                                    float(self.muT), int(self.use_graph), int(self.batch_autotune))
 
 
 class SolverData:
@@ -215,6 +217,24 @@ class PdSolver:
         capi.check(self._L.sbs_pd_dense_info(self.h, ctypes.byref(b), ctypes.byref(m), ctypes.byref(nb),
                                              ctypes.byref(it), ctypes.byref(r)))
         return dict(bodies=b.value, max_body=m.value, inverse_bytes=nb.value, iterations=it.value, ratio=r.value)
+
+    def batch_info(self) -> dict:
+        """BatchedPCG: bodies, size classes (tets range, bodies, tuned threads
+        per workgroup, the tuner's (threads, ms) trials) and per-body PCG
+        statistics (accumulated over the updates)."""
+        nb, nc = ctypes.c_int(), ctypes.c_int()
+        capi.check(self._L.sbs_pd_batch_info(self.h, ctypes.byref(nb), ctypes.byref(nc)))
+        classes = []
+        for c in range(nc.value):
+            info = (ctypes.c_int * 4)()
+            tried = (ctypes.c_double * 16)()
+            nt = ctypes.c_int()
+            capi.check(self._L.sbs_pd_batch_class(self.h, c, info, tried, 8, ctypes.byref(nt)))
+            classes.append(dict(min_tets=info[0], max_tets=info[1], bodies=info[2], threads=info[3],
+                                tried=[(int(tried[2 * k]), tried[2 * k + 1]) for k in range(min(nt.value, 8))]))
+        st = np.zeros((nb.value, 6), np.int32)
+        capi.check(self._L.sbs_pd_batch_bodies(self.h, st.ctypes.data))
+        return dict(bodies=nb.value, classes=classes, body_stats=st)
 
     def export_system(self) -> dict:
         dt = self.data.dtype

@@ -1,0 +1,156 @@
+"""GPU parity of the batched solver (SBS_SOLVER_BATCHED_PCG): every body of a
+concatenated scene advanced with its own Jacobi-PCG + Chebyshev in one
+persistent launch per size class, against the oracle run on each body alone
+(the reference's step of that body, pdSolver.cu:141-208 with the PCG global
+solve and its per-body stop test, pcgJacobi.cu:134-135) and against one
+PdSolver per body.
+
+Bars: FP64 positions within 1e-10 relative of the FP64 oracle per body, FP32
+within 1e-5 of the FP32 oracle per body (L2 and max norm), captured-graph
+replays equal to eager steps bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import max_rel, rel_err
+from sbs_amd import mesh
+from sbs_amd.pd import PdSolver, SolverData, SolverParams, SolverType
+
+pytestmark = pytest.mark.gpu
+
+BODIES = [dict(nx=3, ny=3, nz=3, seed=1, E=1e5), dict(nx=5, ny=4, nz=3, seed=2, E=3e5),
+          dict(nx=8, ny=6, nz=5, seed=3, E=6e4), dict(nx=2, ny=2, nz=2, seed=4, E=2e5),
+          dict(nx=10, ny=3, nz=4, seed=5, E=4.5e5), dict(nx=4, ny=9, nz=4, seed=6, E=8e4)]
+
+
+def scene_bodies():
+    out = []
+    for i, b in enumerate(BODIES):
+        out.append(dict(b, spacing=0.1, origin=(1.5 * i, 0.0, 0.0), jitter=1e-3, nu=0.3, density=1000.0,
+                        pin_top=True))
+    return out
+
+
+def params(precision, **kw):
+    p = dict(numIterations=10, dt=1.0 / 60.0, gravity=9.8, tol=1e-6, rho=0.9992, pcg_chebyshev=True,
+             pcg_max_iter=400, pcg_tol=1e-6 if precision == 32 else 1e-13,
+             pcg_floor=1e-8 if precision == 32 else 0.0)
+    p.update(kw)
+    return SolverParams(**p)
+
+
+def run_batched(scene, precision, p, nsteps, stream=None):
+    data = SolverData(scene, precision)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(SolverType.BatchedPCG)
+    if stream is not None:
+        pd.set_stream(stream)
+    try:
+        for _ in range(nsteps):
+            pd.Update(data, p)
+        pd.synchronize()
+        return data.get("X"), pd.batch_info(), pd.stats()
+    finally:
+        pd.close()
+        data.free()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_batched_matches_oracle_per_body(gpu, precision):
+    bodies = scene_bodies()
+    s = mesh.make_scene(bodies)
+    p = params(precision)
+    nsteps = 5
+    X, info, _ = run_batched(s, precision, p, nsteps)
+    assert info["bodies"] == len(bodies)
+    st = info["body_stats"]
+    assert (st[:, 3] == 10 * nsteps).all() and (st[:, 2] > 0).all() and (st[:, 5] == 0).all()
+    tol = 1e-10 if precision == 64 else 1e-5
+    for i, b in enumerate(bodies):
+        sb = mesh.make_scene([b])
+        o = O.PdOracle(sb, O.make_params(solver=O.SOLVER_PCG, num_iterations=10, rho=0.9992, pcg_chebyshev=1,
+                                         pcg_tol=p.pcg_tol, pcg_floor=p.pcg_floor, pcg_max_iter=400),
+                       fp64=(precision == 64))
+        o.step(nsteps)
+        xo = o.get("X")
+        o.close()
+        xb = X[s.body_vert_start[i]:s.body_vert_start[i + 1]]
+        assert rel_err(xb, xo) < tol, (i, rel_err(xb, xo))
+        assert max_rel(xb, xo) < tol, (i, max_rel(xb, xo))
+        assert np.abs(xo - sb.X).max() > 1e-3      # the body moved
+
+
+def test_batched_matches_per_body_solvers(gpu):
+    bodies = scene_bodies()
+    s = mesh.make_scene(bodies)
+    p = params(32)
+    X, _, _ = run_batched(s, 32, p, 5)
+    pp = params(32, pcg_adaptive=False)
+    for i, b in enumerate(bodies):
+        sb = mesh.make_scene([b])
+        data = SolverData(sb, 32)
+        pd = PdSolver(128, data)
+        pd.SetGlobalSolver(SolverType.PCG)
+        for _ in range(5):
+            pd.Update(data, pp)
+        pd.synchronize()
+        xe = data.get("X")
+        pd.close()
+        data.free()
+        xb = X[s.body_vert_start[i]:s.body_vert_start[i + 1]]
+        assert rel_err(xb, xe) < 1e-5, (i, rel_err(xb, xe))
+
+
+def test_autotuner_and_classes(gpu):
+    bodies = scene_bodies() + [dict(nx=20, ny=12, nz=12, seed=9, E=1e5, spacing=0.1, origin=(9.5, 0.0, 0.0),
+                                    jitter=1e-3, nu=0.3, density=1000.0, pin_top=True)]
+    s = mesh.make_scene(bodies)
+    Xa, info, _ = run_batched(s, 64, params(64), 2)
+    Xb, info_b, _ = run_batched(s, 64, params(64, batch_autotune=False), 2)
+    assert len(info["classes"]) >= 2
+    assert sum(c["bodies"] for c in info["classes"]) == len(bodies)
+    for c in info["classes"]:
+        assert c["tried"] and c["threads"] in [t for t, _ in c["tried"]]
+        assert c["threads"] == min(c["tried"], key=lambda x: x[1])[0]
+    # other workgroup sizes only change the reductions' rounding
+    assert rel_err(Xa, Xb) < 1e-12
+
+
+def test_graph_replay_matches_eager(gpu):
+    from sbs_amd.capi import Stream
+    s = mesh.make_scene(scene_bodies())
+    runs = []
+    for use_graph in (True, False):
+        data = SolverData(s, 32)
+        pd = PdSolver(128, data)
+        pd.SetGlobalSolver(SolverType.BatchedPCG)
+        st = Stream()
+        pd.set_stream(st)
+        runs.append((data, pd, st, params(32, use_graph=use_graph, batch_autotune=False)))
+    try:
+        for k in range(6):
+            for data, pd, _, pp in runs:
+                pd.Update(data, pp)
+                pd.synchronize()
+            np.testing.assert_array_equal(runs[0][0].get("X"), runs[1][0].get("X"), err_msg=f"step {k}")
+        sg = runs[0][1].stats()
+        assert sg.graph_captures == 1 and sg.graph_replays == 6
+    finally:
+        for data, pd, st, _ in runs:
+            pd.close()
+            data.free()
+            st.close()
+
+
+def test_lone_vertex_is_its_own_body(gpu):
+    base = mesh.make_scene(scene_bodies()[:2])
+    X = np.vstack([base.X, [[5.0, 0.4, 0.3]]])
+    s = mesh.Scene(X=X, tet=base.tet, tri=base.tri, tri_fathers=base.tri_fathers,
+                   mass=np.append(base.mass, base.mass[0]), mu=base.mu, lam=base.lam, dbc=np.append(base.dbc, 0.0),
+                   body_vert_start=np.array([0, base.body_vert_start[1], base.nv, base.nv + 1]),
+                   body_tet_start=np.append(base.body_tet_start, base.nt))
+    p = params(64)
+    Xg, info, _ = run_batched(s, 64, p, 4)
+    assert info["bodies"] == 3
+    drop = 9.8 * p.dt ** 2 * 4 * 5 / 2
+    assert abs(Xg[-1, 1] - (0.4 - drop)) < 1e-12
