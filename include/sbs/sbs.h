@@ -276,6 +276,16 @@ int sbs_linear_set_stream(SbsLinear* h, void* hip_stream);
 int sbs_linear_solve(SbsLinear* h, int n, void* d_b, void* d_x, const void* d_A, int nz, const int* d_rowIdx,
                      const int* d_colIdx, const void* d_guess, int* iterations);
 int sbs_linear_stats(SbsLinear* h, long long* nnz);
+/* The same solve without any device allocation: no counterpart in the
+ * reference; for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Its
+ * workspace is a fixed 32 MiB array per GPU, held until the process ends
+ * (about N = 100,000 unknowns with 400,000 triplets in FP64); larger systems
+ * are rejected with SBS_E_INVALID.  Every call on one GPU uses that GPU's
+ * array, so calls that overlap in time on one GPU race; each call
+ * reinitialises what it reads.  Results equal sbs_linear_solve's. */
+int sbs_linear_solve_noalloc(SbsLinear* h, int n, void* d_b, void* d_x, const void* d_A, int nz,
+                             const int* d_rowIdx, const int* d_colIdx, const void* d_guess, int* iterations);
 
 /* Fixed (kinematic) bodies: FixedBody (src/collision/rigid/rigid.h:14-28) as
  * Sphere (sphere.h), Plane (plane.h) or Cylinder (cylinder.h), held as

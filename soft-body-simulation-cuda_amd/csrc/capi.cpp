@@ -549,6 +549,23 @@ int sbs_linear_solve(SbsLinear* h, int n, void* d_b, void* d_x, const void* d_A,
     });
 }
 
+int sbs_linear_solve_noalloc(SbsLinear* h, int n, void* d_b, void* d_x, const void* d_A, int nz,
+                             const int* d_rowIdx, const int* d_colIdx, const void* d_guess, int* iterations) {
+    return guarded([&] {
+        SBS_REQUIRE(h != nullptr, "null SbsLinear handle");
+        int it = 0;
+        if (h->precision == 64)
+            it = h->f64->solve_no_alloc(n, static_cast<const double*>(d_b), static_cast<double*>(d_x),
+                                        static_cast<const double*>(d_A), nz, d_rowIdx, d_colIdx,
+                                        static_cast<const double*>(d_guess), h->stream);
+        else
+            it = h->f32->solve_no_alloc(n, static_cast<const float*>(d_b), static_cast<float*>(d_x),
+                                        static_cast<const float*>(d_A), nz, d_rowIdx, d_colIdx,
+                                        static_cast<const float*>(d_guess), h->stream);
+        if (iterations) *iterations = it;
+    });
+}
+
 int sbs_linear_stats(SbsLinear* h, long long* nnz) {
     return guarded([&] {
         SBS_REQUIRE(h != nullptr, "null SbsLinear handle");

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 namespace sbs {
 namespace {
@@ -258,6 +260,46 @@ template <class T> struct Layout {
 };
 
 }  // namespace
+
+// Workspace of the allocation-free solve (solve_no_alloc): one array per GPU,
+// held until the process ends.  Host code never passes the array's name to a
+// kernel or a copy (on the host it is only a stand-in for the symbol): it
+// uses the device address hipGetSymbolAddress returns for the current GPU.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current GPU's address of `scratch`, looked up once per device.
+void* scratch_ptr() {
+    int dev = 0;
+    SBS_HIP(hipGetDevice(&dev));
+    SBS_REQUIRE(dev >= 0 && dev < kMaxDevices, "allocation-free solve: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SBS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SBS_REQUIRE(p != nullptr, "allocation-free solve: no device address for the workspace");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+template <class T>
+int CooPcgSolver<T>::solve_no_alloc(int n, const T* b, T* x, const T* A, long long nz, const int* row,
+                                    const int* col, const T* guess, hipStream_t stream) {
+    SBS_REQUIRE(n >= 0 && nz >= 0, "LinearSolver::Solve: negative size");
+    const size_t need = CooWorkspace::bytes(n, nz, sizeof(T), sort_tmp_bytes(nz, n));
+    if (need > kNoAllocScratchBytes)
+        throw Error(kInvalidArgument, "allocation-free solve: N = " + std::to_string(n) + ", nz = " +
+                                          std::to_string(nz) + " needs " + std::to_string(need) +
+                                          " bytes of workspace; its fixed workspace holds " +
+                                          std::to_string(kNoAllocScratchBytes));
+    return solve_in(n, b, x, A, nz, row, col, guess, stream, scratch_ptr(), kNoAllocScratchBytes);
+}
 
 template <class T> size_t CooPcgSolver<T>::sort_tmp_bytes(long long nz, int n) {
     const int items = (int)std::max<long long>(nz, 1);

@@ -19,6 +19,11 @@
 
 namespace sbs {
 
+// Size of the fixed device workspace of the allocation-free solve (one per
+// GPU, for the life of the process): 32 MiB, about N = 100,000 unknowns with
+// 400,000 triplets in FP64.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+
 // Device workspace of one solve, carved from one buffer: the sorted triplets,
 // the CSR arrays, the five PCG vectors, reduction partials and the control
 // block.  `bytes` is what solve() needs for (N, nz).
@@ -36,6 +41,11 @@ public:
     // throws when it is too small.
     int solve_in(int n, const T* b, T* x, const T* A, long long nz, const int* row, const int* col, const T* guess,
                  hipStream_t stream, void* ws, size_t ws_bytes);
+    // Solve in the library's fixed per-GPU workspace (kNoAllocScratchBytes):
+    // no hipMalloc or hipFree; throws when (n, nz) needs more.  Calls on one
+    // GPU share that workspace, so calls that overlap in time on one GPU race.
+    int solve_no_alloc(int n, const T* b, T* x, const T* A, long long nz, const int* row, const int* col,
+                       const T* guess, hipStream_t stream);
     // hipCUB temporary bytes for sorting and reducing nz triplets.
     static size_t sort_tmp_bytes(long long nz, int n);
     long long last_nnz() const { return nnz_; }

@@ -147,6 +147,7 @@ SIGNATURES = {
     "sbs_linear_set_stream": (_I, [_VP, _VP]),
     "sbs_linear_solve": (_I, [_VP, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, ctypes.POINTER(_I)]),
     "sbs_linear_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "sbs_linear_solve_noalloc": (_I, [_VP, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, ctypes.POINTER(_I)]),
     "sbs_ipc_create": (_I, [ctypes.POINTER(SbsIpcData), ctypes.POINTER(_VP)]),
     "sbs_ipc_destroy": (_I, [_VP]),
     "sbs_ipc_update": (_I, [_VP, ctypes.POINTER(SbsIpcData), ctypes.POINTER(SbsIpcParams), ctypes.POINTER(_I)]),

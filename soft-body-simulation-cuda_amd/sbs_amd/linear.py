@@ -38,6 +38,14 @@ class PCGJacobiSolver:
         self.iterations = it.value
         return it.value
 
+    def SolveNoAlloc(self, N: int, d_b, d_x, A=None, nz: int = 0, rowIdx=None, colIdx=None, d_guess=None) -> int:
+        """Solve without device allocation (a fixed 32 MiB workspace per GPU)."""
+        it = ctypes.c_int()
+        capi.check(self._L.sbs_linear_solve_noalloc(self.h, int(N), _ptr(d_b), _ptr(d_x), _ptr(A), int(nz),
+                                                    _ptr(rowIdx), _ptr(colIdx), _ptr(d_guess), ctypes.byref(it)))
+        self.iterations = it.value
+        return it.value
+
     def set_stream(self, stream) -> None:
         capi.check(self._L.sbs_linear_set_stream(self.h, _ptr(stream)))
 
