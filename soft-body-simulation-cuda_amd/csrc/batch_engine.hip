@@ -40,6 +40,7 @@ template <class T> struct BatchArgs {
     const T* c0;
     const T* mass;
     const T* dbc;
+    const T* origin;  // 3 per body: the body's local origin
     T* X;
     T* V;
     T* XT;
@@ -100,6 +101,11 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
     const size_t ebase = (size_t)a.eptr[b];
     const int B = blockDim.x, tid = threadIdx.x;
     const T dt = a.dt;
+    // Every vector of the body is held relative to its origin o (the step is
+    // translation invariant: L o = 0, so A (y + o) = b becomes A y = b - c o),
+    // so FP32 rounds offsets of the body's size, not of its distance from the
+    // world origin.
+    const T o[3] = {a.origin[3 * b], a.origin[3 * b + 1], a.origin[3 * b + 2]};
 
     // prologue (pdSolver.cu:154-162): s_n = x + h v + f_ext / c, sn_old, prev
     for (int i = v0 + tid; i < v1; i += B) {
@@ -107,9 +113,9 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
         const T cv = a.c0[i];
         const T fy = -a.gravity * a.mass[i];
         const T inv = (T)1 / cv;
-        const T s0 = a.X[3 * (size_t)g] + dt * a.V[3 * (size_t)g];
-        const T s1 = a.X[3 * (size_t)g + 1] + dt * a.V[3 * (size_t)g + 1] + inv * fy;
-        const T s2 = a.X[3 * (size_t)g + 2] + dt * a.V[3 * (size_t)g + 2];
+        const T s0 = (a.X[3 * (size_t)g] - o[0]) + dt * a.V[3 * (size_t)g];
+        const T s1 = (a.X[3 * (size_t)g + 1] - o[1]) + dt * a.V[3 * (size_t)g + 1] + inv * fy;
+        const T s2 = (a.X[3 * (size_t)g + 2] - o[2]) + dt * a.V[3 * (size_t)g + 2];
         T* sn = a.sn + 3 * (size_t)i;
         T* so = a.sn_old + 3 * (size_t)i;
         T* pv = a.prev + 3 * (size_t)i;
@@ -170,9 +176,13 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
                 for (int k = 0; k < 3; ++k) bv[k] += f[k];
             }
             if (a.dbc[i] > (T)0) {
+                // DBCX wdbc - c o (translation of b = DBCX wdbc by A o = c o):
+                // (DBCX - o) wdbc - (c - wdbc) o, where both differences are
+                // exact (nearby values), so nothing cancels
                 const uint32_t g = a.perm[i];
+                const T cw = cv - a.wdbc;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) bv[k] = a.DBCX[3 * (size_t)g + k] * a.wdbc;
+                for (int k = 0; k < 3; ++k) bv[k] = (a.DBCX[3 * (size_t)g + k] - o[k]) * a.wdbc - cw * o[k];
             }
             const T dA = a.diagA[i], di = (T)1 / dA;
             T Ax[3];
@@ -300,9 +310,10 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
         for (int k = 0; k < 3; ++k) {
             const T n = a.sn[3 * (size_t)i + k];
             const size_t j = 3 * (size_t)g + k;
-            a.V[j] = (n - a.XT[j]) * dt_inv;
-            a.XT[j] = n;
-            if (a.write_x) a.X[j] = n;
+            a.V[j] = (n - (a.XT[j] - o[k])) * dt_inv;
+            const T xa = n + o[k];
+            a.XT[j] = xa;
+            if (a.write_x) a.X[j] = xa;
         }
     }
     if (tid == 0) {
@@ -423,6 +434,11 @@ template <class T> BatchEngine<T>::BatchEngine(const BatchSystem<T>& s) {
         }
     }
     SBS_REQUIRE(eptr[nb_] >= 0 && (size_t)nt_ * 4 < (1ull << 32), "batched PD: scene too large");
+    // each body's origin: the rest position of its first vertex
+    std::vector<T> origin(3 * (size_t)nb_);
+    for (int b = 0; b < nb_; ++b)
+        for (int k = 0; k < 3; ++k) origin[3 * (size_t)b + k] = s.x0[3 * (size_t)perm[vptr_h_[b]] + k];
+    upload(origin_, origin);
     upload(vptr_, vptr_h_);
     upload(tptr_, tptr_h_);
     upload(eptr_, eptr);
@@ -534,6 +550,7 @@ void BatchEngine<T>::launch_class(int c, T* X, T* V, T* XT, const T* DBCX, const
     a.c0 = c0_.get();
     a.mass = mass_.get();
     a.dbc = dbc_.get();
+    a.origin = origin_.get();
     a.X = X;
     a.V = V;
     a.XT = XT;

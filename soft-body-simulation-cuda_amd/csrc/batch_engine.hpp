@@ -13,6 +13,11 @@
 // renumbered body by body; caller arrays are read and written through that
 // permutation at the step's ends.
 //
+// Precision: each body's vectors are held relative to a local origin (the rest
+// position of its first vertex), so FP32 rounds offsets of the body's size
+// rather than of its distance from the world origin; dot products accumulate
+// in FP64.
+//
 // Tile choice: the workgroup size of each class (64..1024 threads) comes from
 // an autotuner that times one step of the class with every candidate on a
 // copy of the state at prepare.
@@ -41,6 +46,7 @@ template <class T> struct BatchSystem {
     std::vector<uint32_t> tets;       // 4 per tet
     std::vector<T> dminv, w;          // 9 (row-major) and 1 per tet
     std::vector<T> c0, diagA, mass, dbc;  // per vertex
+    std::vector<T> x0;                // 3 per vertex: rest positions (the bodies' local origins)
     std::vector<int> rowptr, col;     // CSR of the scalar matrix, diagonal included
     std::vector<T> val;
 };
@@ -78,7 +84,7 @@ private:
     DevBuf<uint32_t> perm_, ecol_, inc_;
     DevBuf<int> inc_ptr_;
     DevBuf<uint4> tets_;
-    DevBuf<T> dminv_, w_, eval_, diagA_, c0_, mass_, dbc_;
+    DevBuf<T> dminv_, w_, eval_, diagA_, c0_, mass_, dbc_, origin_;
     DevBuf<T> sn_, sn_old_, prev_, x_, r_, p_, q_, force_;
     DevBuf<int> stats_;
     std::vector<BatchClass> cls_;

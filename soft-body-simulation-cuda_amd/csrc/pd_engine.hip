@@ -1800,6 +1800,8 @@ template <class T> void PdEngine<T>::build_batch(const PdParams& params) {
     sys.dbc.resize(nv_);
     SBS_HIP(hipMemcpy(sys.mass.data(), data_.mass, nv_ * sizeof(T), hipMemcpyDeviceToHost));
     SBS_HIP(hipMemcpy(sys.dbc.data(), data_.DBC, nv_ * sizeof(T), hipMemcpyDeviceToHost));
+    sys.x0.resize(3 * (size_t)nv_);
+    SBS_HIP(hipMemcpy(sys.x0.data(), data_.X0, sys.x0.size() * sizeof(T), hipMemcpyDeviceToHost));
     batch_ = std::make_unique<BatchEngine<T>>(sys);
     if (params.batch_autotune) {
         BatchParams bp;

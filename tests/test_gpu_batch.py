@@ -154,3 +154,30 @@ def test_lone_vertex_is_its_own_body(gpu):
     assert info["bodies"] == 3
     drop = 9.8 * p.dt ** 2 * 4 * 5 / 2
     assert abs(Xg[-1, 1] - (0.4 - drop)) < 1e-12
+
+
+def test_far_bodies_keep_their_accuracy(gpu):
+    """Bodies 75 m from the world origin: the batched FP32 step computes in
+    body-local coordinates, so what remains of its distance to the FP64 oracle
+    is mostly the rounding of the FP32 positions it hands back (1 ulp at 75 m is
+    7.6e-6 m); the FP32 oracle, which computes in world coordinates as the
+    reference does, is several times further off, and the batched positions
+    stay within 1e-6 relative of the FP64 oracle's."""
+    bodies = [dict(b, origin=(75.0 + 1.5 * i, 0.0, 75.0)) for i, b in enumerate(scene_bodies())]
+    s = mesh.make_scene(bodies)
+    X, _, _ = run_batched(s, 32, params(32), 5)
+    for i, b in enumerate(bodies):
+        sb = mesh.make_scene([b])
+        ref = {}
+        for fp64 in (True, False):
+            o = O.PdOracle(sb, O.make_params(solver=O.SOLVER_PCG, num_iterations=10, rho=0.9992, pcg_chebyshev=1,
+                                             pcg_tol=1e-10 if fp64 else 1e-6, pcg_floor=0.0 if fp64 else 1e-8,
+                                             pcg_max_iter=400), fp64=fp64)
+            o.step(5)
+            ref[fp64] = o.get("X")
+            o.close()
+        xb = X[s.body_vert_start[i]:s.body_vert_start[i + 1]]
+        e_batched = np.linalg.norm(xb - ref[True])
+        e_world32 = np.linalg.norm(ref[False] - ref[True])
+        assert e_batched <= 0.3 * e_world32, (i, e_batched, e_world32)
+        assert rel_err(xb, ref[True]) < 1e-6, (i, rel_err(xb, ref[True]))
