@@ -575,11 +575,12 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=None,
                     help="steps timed kernel by kernel after the timed region (default: as many as --steps)")
     ap.add_argument("--workload", default="cube64",
-                    choices=["cube64", "dense256", "batched", "ipc", "jacobi64", "direct14", "collide"],
+                    choices=["cube64", "dense256", "batched", "ipc", "jacobi64", "direct14", "direct27", "collide"],
                     help="cube64: the headline 64^3 cube; dense256: 256 small bodies on the dense solver; "
                          "batched: 1024 uneven bodies in one engine vs per-body engines; "
                          "ipc: the FP64 IPC step on two 16^3 cubes in contact; jacobi64: the 64^3 cube on the "
-                         "Jacobi + Chebyshev global step; direct14: a 14^3 cube on the band Cholesky; collide: "
+                         "Jacobi + Chebyshev global step; direct14, direct27: a 14^3 or 27^3 cube on the band Cholesky; "
+                         "collide: "
                          "two 24^3 cubes colliding, with fixed bodies")
     ap.add_argument("--per-body-sample", type=int, default=16, help="bodies timed as separate engines (batched)")
     ap.add_argument("--ipc-steps", type=int, default=100, help="steps of the IPC contact scene")
@@ -606,9 +607,10 @@ def main():
     if args.workload == "jacobi64":
         run_solver_workload(args, SolverType.Jacobi, 64, "jacobi_chebyshev", "Jacobi + Chebyshev global step")
         return
-    if args.workload == "direct14":
-        # the band factor of one system is held whole (at most 3,818 vertices in FP32)
-        run_solver_workload(args, SolverType.CuSolverCholesky, 14, "direct_solve", "band Cholesky global step")
+    if args.workload in ("direct14", "direct27"):
+        # band factor (host) + ring-buffer band triangular solves (GPU)
+        run_solver_workload(args, SolverType.CuSolverCholesky, int(args.workload[6:]), "direct_solve",
+                            "band Cholesky global step")
         return
     if args.workload == "collide":
         run_collide_workload(args)

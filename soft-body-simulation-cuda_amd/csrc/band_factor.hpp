@@ -77,6 +77,34 @@ inline int bandwidth_of(int n, const std::vector<int>& rowptr, const std::vector
     return bw;
 }
 
+// Half bandwidth after the ordering band_factor would choose (the narrower of
+// natural and RCM order), without factoring.
+inline int band_width_estimate(int n, const std::vector<int>& rowptr, const std::vector<int>& col) {
+    std::vector<int> natural(n);
+    for (int i = 0; i < n; ++i) natural[i] = i;
+    std::vector<int> rcm = rcm_order(n, rowptr, col), pos_rcm(n);
+    for (int i = 0; i < n; ++i) pos_rcm[rcm[i]] = i;
+    return std::min(bandwidth_of(n, rowptr, col, natural), bandwidth_of(n, rowptr, col, pos_rcm));
+}
+
+// Independent systems of the reordered matrix: position p starts a new one
+// when no row before p couples to a row at or after p (the factor is then
+// block diagonal there).  Returns the first position of each system and n.
+inline std::vector<int> band_systems(int n, const std::vector<int>& rowptr, const std::vector<int>& col,
+                                     const std::vector<int>& perm) {
+    std::vector<int> pos(n);
+    for (int i = 0; i < n; ++i) pos[perm[i]] = i;
+    std::vector<int> sys = {0};
+    int reach = -1;
+    for (int p = 0; p < n; ++p) {
+        if (p > 0 && reach < p) sys.push_back(p);
+        const int v = perm[p];
+        for (int k = rowptr[v]; k < rowptr[v + 1]; ++k) reach = std::max(reach, pos[col[k]]);
+    }
+    sys.push_back(n);
+    return sys;
+}
+
 // A = L L^T for a symmetric positive definite CSR matrix (both triangles,
 // any column order), in double, after the narrower of natural / RCM order.
 // Throws std::invalid_argument for malformed input and std::domain_error

@@ -8,151 +8,171 @@ namespace {
 
 constexpr int kBlk = 64;       // rows per block = one wavefront
 constexpr int kTileLd = 65;    // padded tile stride: conflict-free row and column reads
-constexpr int kThreads = 256;  // 4 wavefronts
-constexpr size_t kLdsBudget = 64 * 1024;
+constexpr int kWaves = 8;      // the band update of a block is split over 8 wavefronts
+constexpr int kThreads = 64 * kWaves;
+// host factor work N bw^2 / 2 beyond which the direct modes decline (the
+// band factor is computed on the host; about a minute of one core)
+constexpr double kMaxFactorWork = 6e10;
 
-template <class T> size_t lds_bytes(int n) {
-    return (3 * (size_t)n + (size_t)kBlk * kTileLd + 4 * (size_t)kBlk * 3) * sizeof(T);
+template <class T> size_t lds_bytes(int ring) {
+    return ((size_t)ring + (size_t)kBlk * kTileLd + (size_t)kWaves * kBlk) * sizeof(T);
 }
 
+// smallest power of two holding the bw rows a block reads plus its own 64
+int ring_rows(int bw) {
+    int r = kBlk;
+    while (r < bw + kBlk) r *= 2;
+    return r;
+}
+
+// value of `v` in lane `l` of the wavefront (l uniform): v_readlane, no LDS trip
+__device__ inline float lane_value(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ inline double lane_value(double v, int l) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// One workgroup per (independent system, right-hand side component): the
+// rows of system s are [sys[s], sys[s+1]) of the reordered matrix, and no
+// row of one system couples to another (the factor is block diagonal), so
+// the systems and the x, y, z components are solved side by side.  Rows go
+// in blocks of 64:
+//   * band update: the block's rows times the y (forward) or x (backward)
+//     values of the bw rows before (after) it, kept in a ring of R rows in
+//     LDS (R a power of two >= bw + 64); kWaves wavefronts each take every
+//     kWaves-th diagonal, 4 loads in flight per lane;
+//   * the 64x64 triangular tile is solved by one wavefront, the solved value
+//     of each row broadcast by v_readlane and scaled by the reciprocal of the
+//     diagonal the tile holds.
+// y is also kept in `ybuf` for the backward sweep.  LDS grows with the
+// bandwidth, not with the number of rows.
 template <class T>
-__global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, const T* __restrict__ Ld,
-                                                              const int* __restrict__ perm, const T* __restrict__ b,
-                                                              T* __restrict__ x, const int* __restrict__ skip) {
+__global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int R, const int* __restrict__ sys,
+                                                              const T* __restrict__ Ld, const int* __restrict__ perm,
+                                                              const T* __restrict__ b, T* __restrict__ x,
+                                                              T* __restrict__ ybuf, const int* __restrict__ skip) {
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
-    T* ys = reinterpret_cast<T*>(smem_raw);
-    T* tile = ys + 3 * (size_t)n;
+    T* ring = reinterpret_cast<T*>(smem_raw);
+    T* tile = ring + R;
     T* part = tile + kBlk * kTileLd;
+    const int mask = R - 1;
+    const int comp = blockIdx.x % 3, sy = blockIdx.x / 3;
+    const int s0 = sys[sy], s1 = sys[sy + 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    for (int i = tid; i < n; i += kThreads) {
-        const int o = perm[i];
-        ys[i] = b[3 * (size_t)o];
-        ys[n + i] = b[3 * (size_t)o + 1];
-        ys[2 * n + i] = b[3 * (size_t)o + 2];
-    }
-    __syncthreads();
-    const int nblk = (n + kBlk - 1) / kBlk;
-    // ---- forward: L y = b ----
-    for (int J = 0; J < nblk; ++J) {
-        const int i0 = J * kBlk, r = i0 + lane;
-        T a0 = 0, a1 = 0, a2 = 0;
-        if (r < n) {
-            const int dlo = r - i0 + 1, dhi = min(bw, r);
-            for (int d = dlo + wid; d <= dhi; d += 4) {
-                const T l = Ld[(size_t)d * n + r];
-                const int j = r - d;
-                a0 += l * ys[j];
-                a1 += l * ys[n + j];
-                a2 += l * ys[2 * n + j];
-            }
-        }
-        part[(wid * kBlk + lane) * 3 + 0] = a0;
-        part[(wid * kBlk + lane) * 3 + 1] = a1;
-        part[(wid * kBlk + lane) * 3 + 2] = a2;
+    const size_t step = (size_t)kWaves * n;
+    auto fill_tile = [&](int i0) {
         for (int e = tid; e < kBlk * kBlk; e += kThreads) {
             const int l = e / kBlk, t = e % kBlk, row = i0 + l, d = l - t;
             T v = 0;
-            if (row < n && d >= 0 && d <= bw) v = Ld[(size_t)d * n + row];
-            tile[l * kTileLd + t] = v;
+            if (row < s1 && d >= 0 && d <= bw) v = Ld[(size_t)d * n + row];
+            tile[l * kTileLd + t] = (d == 0 && row < s1) ? T(1) / v : v;
         }
+    };
+    // ---- forward: L y = b ----
+    for (int i0 = s0; i0 < s1; i0 += kBlk) {
+        const int r = i0 + lane;
+        T a = 0;
+        if (r < s1) {
+            const int dlo = r - i0 + 1 + wid, dhi = min(bw, r - s0);
+            const T* L = Ld + (size_t)dlo * n + r;
+#pragma unroll 4
+            for (int d = dlo; d <= dhi; d += kWaves, L += step) a += *L * ring[(r - d) & mask];
+        }
+        part[wid * kBlk + lane] = a;
+        fill_tile(i0);
         __syncthreads();
         if (wid == 0) {
-            T s0 = 0, s1 = 0, s2 = 0;
-            if (r < n) {
-                s0 = ys[r]; s1 = ys[n + r]; s2 = ys[2 * n + r];
-                for (int w = 0; w < 4; ++w) {
-                    s0 -= part[(w * kBlk + lane) * 3 + 0];
-                    s1 -= part[(w * kBlk + lane) * 3 + 1];
-                    s2 -= part[(w * kBlk + lane) * 3 + 2];
-                }
+            T sv = 0;
+            if (r < s1) {
+                sv = b[3 * (size_t)perm[r] + comp];
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
             }
-            const int tmax = min(kBlk, n - i0);
+            const int tmax = min(kBlk, s1 - i0);
             for (int t = 0; t < tmax; ++t) {
-                const T inv = (T)1 / tile[t * kTileLd + t];
-                const T y0 = __shfl(s0, t, 64) * inv, y1 = __shfl(s1, t, 64) * inv, y2 = __shfl(s2, t, 64) * inv;
-                if (lane > t) {
-                    const T l = tile[lane * kTileLd + t];
-                    s0 -= l * y0; s1 -= l * y1; s2 -= l * y2;
-                } else if (lane == t) {
-                    s0 = y0; s1 = y1; s2 = y2;
-                }
+                const T y = lane_value(sv, t) * tile[t * kTileLd + t];
+                if (lane > t) sv -= tile[lane * kTileLd + t] * y;
+                else if (lane == t) sv = y;
             }
-            if (r < n) { ys[r] = s0; ys[n + r] = s1; ys[2 * n + r] = s2; }
+            if (r < s1) {
+                ring[r & mask] = sv;
+                ybuf[3 * (size_t)r + comp] = sv;
+            }
         }
         __syncthreads();
     }
     // ---- backward: L^T x = y ----
-    for (int J = nblk - 1; J >= 0; --J) {
-        const int i0 = J * kBlk, r = i0 + lane;
-        T a0 = 0, a1 = 0, a2 = 0;
-        if (r < n) {
-            const int dlo = i0 + kBlk - r, dhi = min(bw, n - 1 - r);
-            for (int d = dlo + wid; d <= dhi; d += 4) {
-                const int j = r + d;
-                const T l = Ld[(size_t)d * n + j];
-                a0 += l * ys[j];
-                a1 += l * ys[n + j];
-                a2 += l * ys[2 * n + j];
-            }
+    const int last = s0 + ((s1 - s0 - 1) / kBlk) * kBlk;
+    for (int i0 = last; i0 >= s0; i0 -= kBlk) {
+        const int r = i0 + lane;
+        T a = 0;
+        if (r < s1) {
+            const int dlo = i0 + kBlk - r + wid, dhi = min(bw, s1 - 1 - r);
+            const T* L = Ld + (size_t)dlo * n + r + dlo;
+#pragma unroll 4
+            for (int d = dlo; d <= dhi; d += kWaves, L += step + kWaves) a += *L * ring[(r + d) & mask];
         }
-        part[(wid * kBlk + lane) * 3 + 0] = a0;
-        part[(wid * kBlk + lane) * 3 + 1] = a1;
-        part[(wid * kBlk + lane) * 3 + 2] = a2;
-        for (int e = tid; e < kBlk * kBlk; e += kThreads) {
-            const int l = e / kBlk, t = e % kBlk, row = i0 + l, d = l - t;
-            T v = 0;
-            if (row < n && d >= 0 && d <= bw) v = Ld[(size_t)d * n + row];
-            tile[l * kTileLd + t] = v;
-        }
+        part[wid * kBlk + lane] = a;
+        fill_tile(i0);
         __syncthreads();
         if (wid == 0) {
-            T s0 = 0, s1 = 0, s2 = 0;
-            if (r < n) {
-                s0 = ys[r]; s1 = ys[n + r]; s2 = ys[2 * n + r];
-                for (int w = 0; w < 4; ++w) {
-                    s0 -= part[(w * kBlk + lane) * 3 + 0];
-                    s1 -= part[(w * kBlk + lane) * 3 + 1];
-                    s2 -= part[(w * kBlk + lane) * 3 + 2];
-                }
+            T sv = 0;
+            if (r < s1) {
+                sv = ybuf[3 * (size_t)r + comp];
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
             }
-            const int tmax = min(kBlk, n - i0);
+            const int tmax = min(kBlk, s1 - i0);
             for (int t = tmax - 1; t >= 0; --t) {
-                const T inv = (T)1 / tile[t * kTileLd + t];
-                const T x0 = __shfl(s0, t, 64) * inv, x1 = __shfl(s1, t, 64) * inv, x2 = __shfl(s2, t, 64) * inv;
-                if (lane < t) {
-                    const T l = tile[t * kTileLd + lane];
-                    s0 -= l * x0; s1 -= l * x1; s2 -= l * x2;
-                } else if (lane == t) {
-                    s0 = x0; s1 = x1; s2 = x2;
-                }
+                const T xv = lane_value(sv, t) * tile[t * kTileLd + t];
+                if (lane < t) sv -= tile[t * kTileLd + lane] * xv;
+                else if (lane == t) sv = xv;
             }
-            if (r < n) { ys[r] = s0; ys[n + r] = s1; ys[2 * n + r] = s2; }
+            if (r < s1) {
+                ring[r & mask] = sv;
+                x[3 * (size_t)perm[r] + comp] = sv;
+            }
         }
         __syncthreads();
     }
-    for (int i = tid; i < n; i += kThreads) {
-        const int o = perm[i];
-        x[3 * (size_t)o] = ys[i];
-        x[3 * (size_t)o + 1] = ys[n + i];
-        x[3 * (size_t)o + 2] = ys[2 * n + i];
-    }
+}
+
+// LDS a workgroup may use on the current device (64 KB unless it reports more).
+size_t lds_limit() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        return 64 * 1024;
+    return (size_t)v;
 }
 
 }  // namespace
 
-template <class T> int BandCholesky<T>::max_n() {
-    const size_t fixed = ((size_t)kBlk * kTileLd + 4 * (size_t)kBlk * 3) * sizeof(T);
-    return (int)((kLdsBudget - fixed) / (3 * sizeof(T)));
+template <class T> int BandCholesky<T>::max_bandwidth() {
+    const size_t fixed = ((size_t)kBlk * kTileLd + (size_t)kWaves * kBlk) * sizeof(T);
+    size_t ring = kBlk;  // largest power-of-two ring that fits
+    while ((2 * ring) * sizeof(T) + fixed <= lds_limit()) ring *= 2;
+    return (int)ring - kBlk;
 }
 
 template <class T>
 void BandCholesky<T>::factor(int n, const std::vector<int>& rowptr, const std::vector<int>& col,
                              const std::vector<T>& val, hipStream_t stream) {
     SBS_REQUIRE(n > 0, "BandCholesky: empty matrix");
-    if (n > max_n())
-        throw Error(kInvalidArgument, "direct (Cholesky) global solve supports at most " + std::to_string(max_n()) +
-                                          " vertices per system in this precision; use the PCG global solver");
+    const int bw_est = band_width_estimate(n, rowptr, col);
+    if (bw_est > max_bandwidth())
+        throw Error(kInvalidArgument, "direct (Cholesky) global solve: the reordered matrix's half bandwidth " +
+                                          std::to_string(bw_est) + " exceeds the " + std::to_string(max_bandwidth()) +
+                                          " rows the solve keeps in LDS in this precision; use the PCG global solver");
+    if ((double)n * bw_est * bw_est / 2 > kMaxFactorWork)
+        throw Error(kInvalidArgument, "direct (Cholesky) global solve: the band factor of " + std::to_string(n) +
+                                          " rows with half bandwidth " + std::to_string(bw_est) +
+                                          " is too much host work; use the PCG global solver");
     BandFactor f;
     try {
         f = band_factor(n, rowptr, col, val);
@@ -165,19 +185,29 @@ void BandCholesky<T>::factor(int n, const std::vector<int>& rowptr, const std::v
     std::vector<T> ld((size_t)(bw + 1) * n);
     for (int i = 0; i < n; ++i)
         for (int d = 0; d <= bw; ++d) ld[(size_t)d * n + i] = (T)f.Lb[(size_t)i * (bw + 1) + d];
+    const std::vector<int> sys = band_systems(n, rowptr, col, f.perm);
     n_ = n;
     bw_ = bw;
+    nsys_ = (int)sys.size() - 1;
+    ring_ = ring_rows(bw);
     ld_.alloc(ld.size());
     perm_.alloc(n);
+    sys_.alloc(sys.size());
+    ybuf_.alloc(3 * (size_t)n);
     SBS_HIP(hipMemcpyAsync(ld_.get(), ld.data(), ld.size() * sizeof(T), hipMemcpyHostToDevice, stream));
     SBS_HIP(hipMemcpyAsync(perm_.get(), f.perm.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+    SBS_HIP(hipMemcpyAsync(sys_.get(), sys.data(), sys.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    const size_t lds = lds_bytes<T>(ring_);
+    if (lds > 64 * 1024)
+        SBS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&band_solve_kernel<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     SBS_HIP(hipStreamSynchronize(stream));
 }
 
 template <class T> void BandCholesky<T>::solve(const T* b, T* x, hipStream_t stream, const int* skip) const {
     SBS_REQUIRE(n_ > 0, "BandCholesky: solve before factor");
-    hipLaunchKernelGGL(band_solve_kernel<T>, dim3(1), dim3(kThreads), lds_bytes<T>(n_), stream, n_, bw_, ld_.get(),
-                       perm_.get(), b, x, skip);
+    hipLaunchKernelGGL(band_solve_kernel<T>, dim3(3 * nsys_), dim3(kThreads), lds_bytes<T>(ring_), stream, n_, bw_,
+                       ring_, (const int*)sys_.get(), ld_.get(), perm_.get(), b, x, ybuf_.get(), skip);
     SBS_HIP(hipGetLastError());
 }
 

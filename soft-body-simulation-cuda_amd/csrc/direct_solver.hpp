@@ -11,10 +11,13 @@
 //     L is uploaded in the solve precision, stored by diagonals
 //     (Ld[d*N + i] = L[i][i-d]) so a wavefront reading one diagonal for 64
 //     consecutive rows issues one coalesced load;
-//   * the solve runs in one workgroup per system: the right-hand sides live in
-//     LDS, rows are processed in 64-row blocks -- a band update spread over
-//     4 wavefronts, then the 64x64 triangular tile solved by one wavefront
-//     with register broadcasts -- forward for L, backward for L^T.
+//   * the solve runs one workgroup per (independent system, right-hand side
+//     component): rows are processed in 64-row blocks -- a band update
+//     spread over 4 wavefronts, then the 64x64 triangular tile solved by one
+//     wavefront with register broadcasts -- forward for L, backward for L^T;
+//     the band update reads the previous (next) bw rows' values from a ring
+//     of bw + 64 rows in LDS, so the limit is the bandwidth (up to the LDS a
+//     workgroup may use), not the number of rows.
 #pragma once
 
 #include <vector>
@@ -33,12 +36,16 @@ public:
     void solve(const T* b, T* x, hipStream_t stream, const int* skip) const;
     int n() const { return n_; }
     int bandwidth() const { return bw_; }
-    static int max_n();
+    int systems() const { return nsys_; }
+    // largest half bandwidth the solve can hold (the LDS of one workgroup)
+    static int max_bandwidth();
 
 private:
-    int n_ = 0, bw_ = 0;
+    int n_ = 0, bw_ = 0, nsys_ = 0, ring_ = 0;
     DevBuf<T> ld_;          // (bw+1) diagonals of N
     DevBuf<int> perm_;      // position i holds original row perm[i]
+    DevBuf<int> sys_;       // first row of each independent system (nsys+1)
+    DevBuf<T> ybuf_;        // forward-sweep result, 3 per row
 };
 
 }  // namespace sbs

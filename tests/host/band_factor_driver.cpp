@@ -1,5 +1,6 @@
 // Test driver: reads a CSR matrix from stdin, factors it with the library's
-// host band Cholesky (csrc/band_factor.hpp), prints bw, perm and L by rows.
+// host band Cholesky (csrc/band_factor.hpp), prints bw, perm, L by rows, the
+// first row of each independent system and the bandwidth estimate.
 // Exit code 2: invalid input, 3: not positive definite.
 #include <cstdio>
 #include <iostream>
@@ -22,6 +23,8 @@ int main() {
         std::printf("\n");
         for (size_t i = 0; i < f.Lb.size(); ++i) std::printf("%.17g ", f.Lb[i]);
         std::printf("\n");
+        for (int p : sbs::band_systems(n, rowptr, col, f.perm)) std::printf("%d ", p);
+        std::printf("\n%d\n", sbs::band_width_estimate(n, rowptr, col));
     } catch (const std::invalid_argument&) {
         return 2;
     } catch (const std::domain_error&) {

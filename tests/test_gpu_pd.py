@@ -10,6 +10,7 @@ import pytest
 import oracle as O
 from conftest import max_rel, rel_err
 from sbs_amd import mesh
+from sbs_amd.capi import SbsError
 from sbs_amd.pd import PdSolver, SolverData, SolverParams, SolverType
 
 pytestmark = pytest.mark.gpu
@@ -137,6 +138,69 @@ def test_eigen_cholesky_mode_matches_cusolver_mode(gpu, precision):
     a, _ = run_gpu(s, SolverType.CuSolverCholesky, p, 20, precision)
     b, _ = run_gpu(s, SolverType.EigenCholesky, p, 20, precision)
     assert rel_err(a["X"], b["X"]) < (1e-5 if precision == 32 else 1e-13)
+
+
+@pytest.mark.parametrize("precision,solver", [(32, SolverType.CuSolverCholesky), (64, SolverType.CuSolverCholesky),
+                                               (32, SolverType.EigenCholesky)])
+def test_direct_modes_at_21952_vertices(gpu, precision, solver):
+    """Direct (Cholesky) global solve on the 27^3-hex cube (21,952 vertices,
+    118,098 tets, reordered half bandwidth ~800): past what a solve holding all
+    right-hand sides in one workgroup's LDS can take.  3 steps of 10 iterations
+    against the oracle's envelope Cholesky in the same mode."""
+    s = mesh.cube_scene(27)
+    assert s.nv == 21952
+    p = params_for(solver, n_iter=10, tol=0.0)
+    g, _ = run_gpu(s, solver, p, 3, precision)
+    r = run_oracle(s, solver, p, 3, precision)
+    # FP64: the GPU eliminates in reverse Cuthill-McKee order over a band of
+    # ~800 (the oracle: natural-order envelope), kappa(A) ~ 1.1e6, so single
+    # coordinates may differ by up to ~1e-9 relative (measured 9.7e-10) while
+    # the l2 distance stays at 1e-10
+    max_tol = {32: TOL[32], 64: 1e-8}[precision]
+    for k in ("X", "XTilde"):
+        assert rel_err(g[k], r[k]) < TOL[precision], (k, rel_err(g[k], r[k]))
+        assert max_rel(g[k], r[k]) < max_tol, (k, max_rel(g[k], r[k]))
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_direct_mode_multi_body(gpu, precision):
+    """Bodies of different sizes, one pinned: each body is an independent system
+    of the reordered matrix and is solved by its own workgroups."""
+    s = mesh.make_scene([dict(nx=9, ny=5, nz=4, seed=1, pin_top=True, E=1e5),
+                         dict(nx=3, ny=3, nz=3, seed=2, origin=(2.0, 0.0, 0.0), E=3e5),
+                         dict(nx=12, ny=2, nz=2, seed=3, origin=(0.0, 1.5, 0.0), E=5e4)])
+    p = params_for(SolverType.CuSolverCholesky, n_iter=10, tol=0.0)
+    g, _ = run_gpu(s, SolverType.CuSolverCholesky, p, 30, precision)
+    r = run_oracle(s, SolverType.CuSolverCholesky, p, 30, precision)
+    if precision == 64:
+        for k in ("X", "XTilde"):
+            assert rel_err(g[k], r[k]) < TOL[64], (k, rel_err(g[k], r[k]))
+            assert max_rel(g[k], r[k]) < TOL[64], (k, max_rel(g[k], r[k]))
+        return
+    # FP32: kappa(A) = 2.4e6 (the small stiff cube); over 30 steps the FP32
+    # oracle itself drifts 3.0e-5 (l2) / 5.5e-5 (max) from the FP64 oracle, so
+    # the bar is against FP64: the GPU's FP32 run within 2x the FP32 oracle's
+    # own distance and within 1e-4
+    r64 = run_oracle(s, SolverType.CuSolverCholesky, p, 30, 64)
+    for k in ("X", "XTilde"):
+        e_gpu, e_orc = rel_err(g[k], r64[k]), rel_err(r[k], r64[k])
+        assert e_gpu < min(1e-4, 2 * e_orc), (k, e_gpu, e_orc)
+        m_gpu, m_orc = max_rel(g[k], r64[k]), max_rel(r[k], r64[k])
+        assert m_gpu < min(1e-4, 2 * m_orc), (k, m_gpu, m_orc)
+
+
+def test_direct_mode_declines_an_oversized_factor(gpu):
+    """60^3 hexes (226,981 vertices, half bandwidth ~3,700): the host band
+    factor would be ~1.6e12 multiply-adds, so the direct mode refuses with a
+    clear error instead of stalling; PCG handles this size."""
+    s = mesh.cube_scene(60)
+    data = SolverData(s, 32)
+    pd = PdSolver(128, data)
+    pd.SetGlobalSolver(SolverType.CuSolverCholesky)
+    with pytest.raises(SbsError, match="too much host work"):
+        pd.Update(data, params_for(SolverType.CuSolverCholesky))
+    pd.close()
+    data.free()
 
 
 @pytest.mark.parametrize("precision", [32, 64])

@@ -36,6 +36,8 @@ def run(exe, rowptr, col, val):
         for d in range(bw + 1):
             if i - d >= 0:
                 L[i, i - d] = Lb[i, d]
+    run.systems = list(map(int, lines[3].split()))
+    run.bw_estimate = int(lines[4])
     return 0, (bw, perm, L)
 
 
@@ -55,6 +57,7 @@ def test_factor_of_the_pd_matrix(driver):
     Ap = A[np.ix_(perm, perm)]
     np.testing.assert_allclose(L @ L.T, Ap, rtol=1e-12, atol=1e-12 * np.abs(A).max())
     assert sorted(perm) == list(range(s.nv))
+    assert run.systems == [0, s.nv] and run.bw_estimate == bw
     nat_bw = max(abs(r - c) for r in range(s.nv) for c in ex["col"][ex["rowptr"][r]:ex["rowptr"][r + 1]])
     assert bw <= nat_bw
 
@@ -82,6 +85,7 @@ def test_two_components_and_errors(driver):
     rc, (bw, perm, L) = run(driver, rowptr, cols, A[rows, cols])
     assert rc == 0
     np.testing.assert_allclose(L @ L.T, A[np.ix_(perm, perm)], rtol=1e-14)
+    assert run.systems == [0, 2, 4]
     # indefinite -> not positive definite
     A[0, 0] = -1.0
     assert run(driver, rowptr, cols, A[rows, cols])[0] == 3
@@ -89,3 +93,22 @@ def test_two_components_and_errors(driver):
     bad = cols.copy()
     bad[0] = 9
     assert run(driver, rowptr, bad, A[rows, cols])[0] == 2
+
+
+def test_independent_systems_of_a_multi_body_scene(driver):
+    """Three bodies: three systems, each a contiguous block of whole bodies, and
+    no coupling across a cut (the solve runs each system in its own workgroup)."""
+    s = mesh.make_scene([dict(nx=3, ny=2, nz=2, seed=1), dict(nx=2, ny=2, nz=2, seed=2, origin=(3.0, 0, 0)),
+                         dict(nx=4, ny=2, nz=1, seed=3, origin=(0, 3.0, 0))])
+    ex = O.PdOracle(s, O.make_params(), fp64=True).export()
+    rowptr, col = np.asarray(ex["rowptr"]), np.asarray(ex["col"])
+    rc, (bw, perm, L) = run(driver, rowptr, col, ex["val"])
+    assert rc == 0
+    sysv = run.systems
+    assert len(sysv) == 4 and sysv[0] == 0 and sysv[-1] == s.nv
+    pos = np.empty(s.nv, int)
+    pos[perm] = np.arange(s.nv)
+    which = np.searchsorted(sysv, pos, side="right") - 1
+    for r in range(s.nv):
+        assert np.all(which[col[rowptr[r]:rowptr[r + 1]]] == which[r])
+    assert run.bw_estimate == bw
