@@ -45,9 +45,12 @@ __device__ inline double lane_value(double v, int l) {
 //     values of the bw rows before (after) it, kept in a ring of R rows in
 //     LDS (R a power of two >= bw + 64); kWaves wavefronts each take every
 //     kWaves-th diagonal, 4 loads in flight per lane;
-//   * the 64x64 triangular tile is solved by one wavefront, the solved value
-//     of each row broadcast by v_readlane and scaled by the reciprocal of the
-//     diagonal the tile holds.
+//   * the block's 64x64 lower triangle is read diagonal by diagonal (64
+//     consecutive rows of one diagonal per load: coalesced) into registers
+//     before the band update, so its loads overlap the band's, as do the
+//     right-hand side and the next block's permutation (wavefront 0);
+//   * the tile is solved by wavefront 0, the solved value of each row
+//     broadcast by v_readlane and scaled by the reciprocal of the diagonal.
 // y is also kept in `ybuf` for the backward sweep.  LDS grows with the
 // bandwidth, not with the number of rows.
 template <class T>
@@ -56,6 +59,7 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
                                                               const T* __restrict__ b, T* __restrict__ x,
                                                               T* __restrict__ ybuf, const int* __restrict__ skip) {
     if (skip && *skip) return;
+    constexpr int kDiagPerWave = kBlk / kWaves;  // tile diagonals each wavefront loads
     extern __shared__ unsigned char smem_raw[];
     T* ring = reinterpret_cast<T*>(smem_raw);
     T* tile = ring + R;
@@ -65,17 +69,35 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
     const int s0 = sys[sy], s1 = sys[sy + 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const size_t step = (size_t)kWaves * n;
-    auto fill_tile = [&](int i0) {
-        for (int e = tid; e < kBlk * kBlk; e += kThreads) {
-            const int l = e / kBlk, t = e % kBlk, row = i0 + l, d = l - t;
-            T v = 0;
-            if (row < s1 && d >= 0 && d <= bw) v = Ld[(size_t)d * n + row];
-            tile[l * kTileLd + t] = (d == 0 && row < s1) ? T(1) / v : v;
+    const int dtile = min(bw, kBlk - 1);
+    // diagonals wid, wid + kWaves, ... of the tile rows i0 + lane
+    auto load_tile = [&](int i0, T (&tv)[kDiagPerWave]) {
+        const int row = i0 + lane;
+#pragma unroll
+        for (int j = 0; j < kDiagPerWave; ++j) {
+            const int d = wid + j * kWaves;
+            tv[j] = (row < s1 && d <= dtile && d <= lane) ? Ld[(size_t)d * n + row] : T(0);
+        }
+    };
+    auto store_tile = [&](const T (&tv)[kDiagPerWave]) {
+#pragma unroll
+        for (int j = 0; j < kDiagPerWave; ++j) {
+            const int d = wid + j * kWaves;
+            if (d <= lane) tile[lane * kTileLd + lane - d] = (d == 0) ? T(1) / tv[j] : tv[j];
         }
     };
     // ---- forward: L y = b ----
+    int pr = (wid == 0 && s0 + lane < s1) ? perm[s0 + lane] : 0;
     for (int i0 = s0; i0 < s1; i0 += kBlk) {
         const int r = i0 + lane;
+        T bv = 0;
+        int pn = 0;
+        if (wid == 0) {
+            if (r < s1) bv = b[3 * (size_t)pr + comp];
+            if (r + kBlk < s1) pn = perm[r + kBlk];
+        }
+        T tv[kDiagPerWave];
+        load_tile(i0, tv);
         T a = 0;
         if (r < s1) {
             const int dlo = r - i0 + 1 + wid, dhi = min(bw, r - s0);
@@ -84,15 +106,12 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
             for (int d = dlo; d <= dhi; d += kWaves, L += step) a += *L * ring[(r - d) & mask];
         }
         part[wid * kBlk + lane] = a;
-        fill_tile(i0);
+        store_tile(tv);
         __syncthreads();
         if (wid == 0) {
-            T sv = 0;
-            if (r < s1) {
-                sv = b[3 * (size_t)perm[r] + comp];
+            T sv = bv;
 #pragma unroll
-                for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
-            }
+            for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
             const int tmax = min(kBlk, s1 - i0);
             for (int t = 0; t < tmax; ++t) {
                 const T y = lane_value(sv, t) * tile[t * kTileLd + t];
@@ -103,13 +122,23 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
                 ring[r & mask] = sv;
                 ybuf[3 * (size_t)r + comp] = sv;
             }
+            pr = pn;
         }
         __syncthreads();
     }
     // ---- backward: L^T x = y ----
     const int last = s0 + ((s1 - s0 - 1) / kBlk) * kBlk;
+    pr = (wid == 0 && last + lane < s1) ? perm[last + lane] : 0;
     for (int i0 = last; i0 >= s0; i0 -= kBlk) {
         const int r = i0 + lane;
+        T yv = 0;
+        int pn = 0;
+        if (wid == 0) {
+            if (r < s1) yv = ybuf[3 * (size_t)r + comp];
+            if (i0 > s0) pn = perm[r - kBlk];
+        }
+        T tv[kDiagPerWave];
+        load_tile(i0, tv);
         T a = 0;
         if (r < s1) {
             const int dlo = i0 + kBlk - r + wid, dhi = min(bw, s1 - 1 - r);
@@ -118,15 +147,12 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
             for (int d = dlo; d <= dhi; d += kWaves, L += step + kWaves) a += *L * ring[(r + d) & mask];
         }
         part[wid * kBlk + lane] = a;
-        fill_tile(i0);
+        store_tile(tv);
         __syncthreads();
         if (wid == 0) {
-            T sv = 0;
-            if (r < s1) {
-                sv = ybuf[3 * (size_t)r + comp];
+            T sv = yv;
 #pragma unroll
-                for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
-            }
+            for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
             const int tmax = min(kBlk, s1 - i0);
             for (int t = tmax - 1; t >= 0; --t) {
                 const T xv = lane_value(sv, t) * tile[t * kTileLd + t];
@@ -135,8 +161,9 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
             }
             if (r < s1) {
                 ring[r & mask] = sv;
-                x[3 * (size_t)perm[r] + comp] = sv;
+                x[3 * (size_t)pr + comp] = sv;
             }
+            pr = pn;
         }
         __syncthreads();
     }
