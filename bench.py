@@ -172,15 +172,7 @@ def parity_vs_oracle(scene, steps: int):
 
 
 def dense_scene(nbodies: int = 256, n: int = 6, seed: int = 0):
-    """The mixed-precision small-body workload: nbodies n^3-hex cubes on a
-    grid, E per body uniform in [5e4, 5e5] Pa from the seed, nu = 0.3, top
-    faces pinned, +-1 mm jitter."""
-    E = mesh.uniform01(seed, np.arange(nbodies, dtype=np.uint64)) * 4.5e5 + 5e4
-    side = int(np.ceil(np.sqrt(nbodies)))
-    bodies = [dict(nx=n, ny=n, nz=n, spacing=0.1, origin=(1.0 * (i % side), 0.0, 1.0 * (i // side)),
-                   jitter=1e-3, seed=seed + 1 + i, E=float(E[i]), nu=0.3, density=1000.0, pin_top=True)
-              for i in range(nbodies)]
-    return mesh.make_scene(bodies)
+    return mesh.small_bodies_scene(nbodies, n, seed)
 
 
 def dense_measure(steps: int, warmup: int):

@@ -1139,6 +1139,28 @@ __device__ __forceinline__ void halo_row_diag(const HaloView<T>& h, int first, i
         apply(b);
     }
 }
+// halo_row_diag with the first two batches already loaded.
+template <class T>
+__device__ __forceinline__ void halo_row_diag2(const HaloView<T>& h, int first, int width, int t,
+                                               const RowBatch<T>& b0, const RowBatch<T>& b1, T (*lu)[kHaloStride],
+                                               unsigned own, T& y0, T& y1, T& y2, T& d) {
+    auto apply = [&](const RowBatch<T>& b) {
+#pragma unroll
+        for (int j = 0; j < kEllBatch; ++j) {
+            y0 += b.m[j] * lu[0][b.c[j]];
+            y1 += b.m[j] * lu[1][b.c[j]];
+            y2 += b.m[j] * lu[2][b.c[j]];
+            d += b.c[j] == own ? b.m[j] : (T)0;
+        }
+    };
+    apply(b0);
+    apply(b1);
+    for (int e0 = 2 * kEllBatch; e0 < width; e0 += kEllBatch) {
+        RowBatch<T> b;
+        row_batch(h, first, width, t, e0, b);
+        apply(b);
+    }
+}
 
 // init1: r0 = b - A x0 (x0 staged at the halo), R'0 = D^-1 r0, x = x0, (D^-1 b)^2.
 template <class T>
@@ -1229,20 +1251,20 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
     stripe_add<9>(vals, iter_slot(a.slots, -1));
 }
 
-// One PCG iteration K(i), halo-staged.  The halo's row indices are requested
-// with the reduction stripes of the previous launch; once the scalars are
-// formed the halo planes, the row's own planes, p and x are requested
-// together, so the launch costs about two dependent memory round trips.  The
-// iteration that finds the solve converged returns after the scalars, before
-// any of those loads (the decision is the same in every wavefront).
+// One PCG iteration K(i), halo-staged.  The group descriptor and the halo's
+// row indices are requested with the reduction stripes and the state of the
+// previous launch (a launch after the solve stopped carries that state
+// forward inside pcg_state); once the scalars are formed the halo planes, the
+// row's own planes, p, x and the row's first kEllBatch matrix entries are
+// requested together, the next kEllBatch as soon as the halo values are
+// staged, so the launch costs about two dependent memory round trips for rows
+// of up to 2 kEllBatch entries.  The iteration that finds the solve
+// converged returns after the scalars, before any of those loads (the
+// decision is the same in every wavefront).
 template <class T, bool kFirst>
 __global__ void __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
 pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     if (a.skip && *a.skip) return;
-    if (!kFirst && a.ctl->st[(i - 1) & 1].done) {  // stopped earlier: carry the state forward
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->st[i & 1] = a.ctl->st[(i - 1) & 1];
-        return;
-    }
     __shared__ T lu[3][kHaloStride];
     const unsigned g = xcd_block(blockIdx.x, gridDim.x);
     const HaloGroup G = halo_group(a.h, g);
@@ -1274,7 +1296,7 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     T x0, x1, x2, p0 = 0, p1 = 0, p2 = 0;
     if (!kFirst) ld3(a.p, ir, p0, p1, p2);
     ld3(a.h.x, ir, x0, x1, x2);
-    RowBatch<T> b0;
+    RowBatch<T> b0, b1;
     row_batch(a.h, G.first, G.width, t, 0, b0);
     const T al0 = (T)st.alpha[0], al1 = (T)st.alpha[1], al2 = (T)st.alpha[2];
     const T be0 = (T)st.beta[0], be1 = (T)st.beta[1], be2 = (T)st.beta[2];
@@ -1292,13 +1314,16 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
             lu[2][e] = fma(-al2, sq2, hq[k][2]);
         }
     }
+    // the second batch is requested once the staged halo values are out of
+    // registers, so its latency overlaps the barrier and the first batch
+    row_batch(a.h, G.first, G.width, t, kEllBatch, b1);
     if (blockIdx.x == 0) {  // slot of launch i+1: last read by launch i-1, which has finished
         double* z = iter_slot(a.slots, i + 1);
         for (int k = threadIdx.x; k < kSlotSize; k += kTpb) z[k] = 0.0;
     }
     __syncthreads();
     T y0 = 0, y1 = 0, y2 = 0, d = 0;  // w_{i+1} = A u_{i+1}
-    halo_row_diag(a.h, G.first, G.width, t, b0, lu, (unsigned)(G.own0 + t), y0, y1, y2, d);
+    halo_row_diag2(a.h, G.first, G.width, t, b0, b1, lu, (unsigned)(G.own0 + t), y0, y1, y2, d);
     double vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (row) {
         // p_i = u_i + beta p_{i-1} (u_i = R'_i), S'_i, x_{i+1}, R'_{i+1} = u_{i+1}

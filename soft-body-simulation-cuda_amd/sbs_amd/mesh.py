@@ -198,6 +198,18 @@ def cube_scene(n: int, spacing: float = 0.1, jitter: float = 1e-3, seed: int = 0
                             E=E, nu=nu, density=density, pin_top=pin_top)])
 
 
+def small_bodies_scene(nbodies: int = 256, n: int = 6, seed: int = 0) -> Scene:
+    """The mixed-precision small-body workload: nbodies n^3-hex cubes on a
+    grid 1 m apart, E per body uniform in [5e4, 5e5] Pa from the seed,
+    nu = 0.3, top faces pinned, +-1 mm jitter."""
+    E = uniform01(seed, np.arange(nbodies, dtype=np.uint64)) * 4.5e5 + 5e4
+    side = int(np.ceil(np.sqrt(nbodies)))
+    bodies = [dict(nx=n, ny=n, nz=n, spacing=0.1, origin=(1.0 * (i % side), 0.0, 1.0 * (i // side)),
+                   jitter=1e-3, seed=seed + 1 + i, E=float(E[i]), nu=0.3, density=1000.0, pin_top=True)
+              for i in range(nbodies)]
+    return make_scene(bodies)
+
+
 def scalar_laplacian_nnz(tet: np.ndarray, nv: int) -> int:
     """N_v + 2*(number of distinct mesh edges): the scalar system's nonzeros."""
     t = tet.astype(np.int64)

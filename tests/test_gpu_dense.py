@@ -107,3 +107,33 @@ def test_dense_linear_solve(gpu, precision):
     assert rel_err(x, xe) < (1e-6 if precision == 32 else 1e-10)
     pd.close()
     data.free()
+
+
+def test_dense_256_bodies_of_6cube_vs_fp64_oracle(gpu):
+    """The dense small-body workload at its size: 256 bodies of 6^3 hexes
+    (87,808 vertices, 331,776 tets, 343 vertices per body, coordinates up to
+    ~16 m), 10 steps of 10 iterations, no early exit, against the FP64 oracle.
+
+    The error is FP32's, not the MFMA precision's: positions at 16 m round at
+    ~1e-6 m, and the FP32 oracle (the reference's arithmetic) is itself
+    4.5e-6 (l2) / 2.7e-4 m (max) from FP64 here.  Bar: both MFMA modes within
+    1.5x the FP32 oracle's distance from FP64, and within 1e-5 (l2) / 5e-4 m
+    (max) absolutely; every solve refined to 1e-10."""
+    s = mesh.small_bodies_scene(256, 6)
+    assert (s.nv, s.nt) == (87808, 331776)
+    o64 = O.PdOracle(s, O.make_params(solver=O.SOLVER_CHOLESKY, num_iterations=10, dt=1.0 / 60.0, gravity=9.8,
+                                      tol=0.0, rho=0.9992), fp64=True)
+    o64.step(10)
+    r64 = o64.get("X")
+    o64.close()
+    r32 = run_oracle(s, params(tol=0.0), 10, False)
+    e_orc = rel_err(r32, r64)
+    m_orc = np.abs(r32 - r64).max()
+    for prec in (0, 1):
+        X, info, worst = run_dense(s, params(tol=0.0, dense_precision=prec, dense_tol=1e-10, dense_max_iter=16),
+                                   10, 32)
+        assert info["bodies"] == 256 and info["max_body"] == 343
+        assert info["ratio"] <= 1e-10
+        e, m = rel_err(X, r64), np.abs(X - r64).max()
+        assert e < min(1e-5, 1.5 * e_orc), (prec, e, e_orc)
+        assert m < min(5e-4, 1.5 * m_orc), (prec, m, m_orc)
