@@ -85,7 +85,7 @@ def measured_traffic(kernel: str):
 
 
 def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int, halo: bool = True,
-            graph: bool = False):
+            graph: bool = False, polar: bool = False):
     data = SolverData(scene, precision)
     pd = PdSolver(128, data)
     pd.SetGlobalSolver(SolverType.PCG)
@@ -93,6 +93,7 @@ def run_gpu(scene, steps: int, warmup: int, precision: int, timing_steps: int, h
     pd.set_stream(stream)
     p = workload_params(precision == 64, halo)
     p.use_graph = graph
+    p.polar_rotation = polar
     for _ in range(warmup):
         pd.Update(data, p)
     pd.synchronize()
@@ -154,12 +155,12 @@ def cpu_baseline(scene, budget_s: float, max_steps: int):
                        f"dot products and updates included, runs in parallel), {el:.1f} s")
 
 
-def parity_vs_oracle(scene, steps: int):
+def parity_vs_oracle(scene, steps: int, polar: bool):
     """The FP32 GPU step against the FP32 oracle (the reference's step in its
-    precision) after `steps` steps from rest."""
+    precision, the SVD rotation) after `steps` steps from rest."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
-    g = run_gpu(scene, steps, 0, 32, timing_steps=0)
+    g = run_gpu(scene, steps, 0, 32, timing_steps=0, polar=polar)
     o = O.PdOracle(scene, O.make_params(solver=O.SOLVER_PCG, num_iterations=20, dt=1.0 / 60.0, gravity=9.8,
                                         tol=1e-6, rho=0.9992, pcg_tol=1e-6, pcg_floor=1e-8, pcg_max_iter=200,
                                         pcg_chebyshev=1), fp64=False)
@@ -580,6 +581,9 @@ def main():
                     help="skip the other workloads' summaries in the default line (8^3 direct, 256 dense "
                          "bodies, 1024 batched bodies, the IPC contact scene)")
     ap.add_argument("--graph", action="store_true", help="replay each step as a captured HIP graph")
+    ap.add_argument("--rotation", default="polar", choices=["polar", "svd"],
+                    help="FP32 local step rotation: Newton's polar iteration (SVD fallback) or the TR1690 SVD "
+                         "throughout (the reference's arithmetic)")
     ap.add_argument("--pcg-kernels", default="halo", choices=["halo", "gather"],
                     help="PCG kernel family: halo-staged (LDS) or per-row neighbour gathers")
     args = ap.parse_args()
@@ -612,7 +616,8 @@ def main():
     print(f"# scene: {scene.nv} vertices, {scene.nt} tets", file=sys.stderr, flush=True)
     halo = args.pcg_kernels == "halo"
     timing_steps = args.steps if args.timing_steps is None else args.timing_steps
-    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=timing_steps, halo=halo,
+    polar = args.rotation == "polar"
+    g = run_gpu(scene, args.steps, args.warmup, 32, timing_steps=timing_steps, halo=halo, polar=polar,
                 graph=args.graph)
     print(f"# gpu fp32: {args.steps} steps in {g['seconds']:.3f} s", file=sys.stderr, flush=True)
     sps = args.steps / g["seconds"]
@@ -666,7 +671,7 @@ def main():
 
     parity = None
     if not args.no_cpu and args.n == 64:
-        parity = parity_vs_oracle(scene, 10)
+        parity = parity_vs_oracle(scene, 10, polar)
         print(f"# parity vs FP32 oracle after 10 steps: {parity['rel_l2']:.2e}", file=sys.stderr, flush=True)
 
     configs = None
@@ -693,6 +698,7 @@ def main():
                    "verts": scene.nv, "tets": scene.nt, "nnz_scalar": g["nnz_off"] + scene.nv,
                    "iterations_per_step": 20, "dt": 1.0 / 60.0, "pcg_tol": 1e-6, "pcg_floor": 1e-8,
                    "pcg_kernels": args.pcg_kernels, "halo_groups": int(g["stats"].halo_groups),
+                   "local_step_rotation": "Newton polar iteration, TR1690 SVD fallback" if polar else "TR1690 SVD",
                    "graph_replays": g["graph_replays"], "graph_captures": g["graph_captures"]},
         "roofline": roof,
         "cpu_baseline": cpu,

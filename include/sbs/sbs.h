@@ -124,6 +124,14 @@ typedef struct SbsSolverParams {
     int batch_autotune;   /* SBS_SOLVER_BATCHED_PCG: 1 times the workgroup sizes of each
                              body-size class at the first batched update (state unchanged);
                              0 keeps one thread per 16 tets of the class's largest body */
+    int polar_rotation;   /* FP32 local step: 0 takes each element's rotation from the TR1690
+                             SVD as the reference does (computeLocal, pdUtil.cu:112-125);
+                             1 from Newton's polar iteration R <- (R + R^-T)/2, 5 steps,
+                             falling back to the SVD for elements it does not settle
+                             (det F <= 0, singular values outside ~[0.27, 3.7]): about a
+                             third of the SVD's arithmetic, and the exact polar factor
+                             where the SVD's 4 Jacobi sweeps stop short of it (strongly
+                             sheared elements, up to ~1e-3).  FP64 always uses the SVD. */
 } SbsSolverParams;
 
 typedef struct SbsPd SbsPd;

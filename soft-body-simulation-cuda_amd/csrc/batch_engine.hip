@@ -57,6 +57,7 @@ template <class T> struct BatchArgs {
     int iters;
     int max_iter;
     int write_x;
+    int polar;  // rotation by Newton's polar iteration (FP32)
     T dt;
     T gravity;
     T rho;
@@ -147,7 +148,7 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
                 for (int k = 0; k < 3; ++k) F[r][k] = d0 * mm[k] + d1 * mm[3 + k] + d2 * mm[6 + k];
             }
             T P[3][3];
-            pd_projection(F, false, P);
+            pd_projection(F, false, P, a.polar != 0);
             const T ww = a.w[t];
             T* f = a.force + 12 * (size_t)t;
 #pragma unroll
@@ -567,6 +568,7 @@ void BatchEngine<T>::launch_class(int c, T* X, T* V, T* XT, const T* DBCX, const
     a.iters = p.iterations;
     a.max_iter = p.pcg_max_iter;
     a.write_x = p.write_x ? 1 : 0;
+    a.polar = p.polar ? 1 : 0;
     a.dt = (T)p.dt;
     a.gravity = (T)p.gravity;
     a.rho = (T)p.rho;

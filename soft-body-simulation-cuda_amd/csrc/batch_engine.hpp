@@ -57,6 +57,7 @@ struct BatchParams {
     double pcg_tol = 1e-6, pcg_floor = 1e-8;
     int pcg_max_iter = 200;
     bool write_x = true;  // false when the collision response decides X
+    bool polar = false;   // FP32 local step: rotation by Newton's polar iteration (SVD fallback)
 };
 
 template <class T> class BatchEngine {

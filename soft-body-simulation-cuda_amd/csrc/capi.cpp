@@ -130,6 +130,7 @@ sbs::PdParams to_params(const SbsSolverParams* p) {
     o.mu_t = p->mu_t;
     o.use_graph = p->use_graph;
     o.batch_autotune = p->batch_autotune;
+    o.polar_rotation = p->polar_rotation;
     return o;
 }
 

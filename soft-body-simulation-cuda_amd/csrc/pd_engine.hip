@@ -947,13 +947,17 @@ template <class T> struct FoldIn {
     T omega;
 };
 
+// flags of the local step
+constexpr int kLocalResidual = 1;  // P = R - F (Jacobi mode) instead of R
+constexpr int kLocalPolar = 2;     // FP32: rotation by Newton's polar iteration (SVD fallback)
+
 template <class T, bool kFold>
 __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
     int nt, const uint2* __restrict__ lidx, const int* __restrict__ chunk_vptr,
     const uint32_t* __restrict__ chunk_verts, const uint32_t* __restrict__ entry_pos,
     const uint16_t* __restrict__ inc_codes,
     const uint16_t* __restrict__ inc_end, const T* __restrict__ dminv, const T* __restrict__ w,
-    const T* __restrict__ q, T* __restrict__ partial, int residual_form, const int* __restrict__ skip,
+    const T* __restrict__ q, T* __restrict__ partial, int flags, const int* __restrict__ skip,
     double* __restrict__ zero_slots, int zero_n, FoldIn<T> fold, PcgArgs<T> close, int close_L) {
     if (skip && *skip) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1013,7 +1017,7 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
             for (int k = 0; k < 3; ++k) F[r][k] = d0 * m[k] + d1 * m[3 + k] + d2 * m[6 + k];
         }
         T P[3][3];
-        pd_projection(F, residual_form != 0, P);
+        pd_projection(F, (flags & kLocalResidual) != 0, P, (flags & kLocalPolar) != 0);
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             // column j of w P Dm^-T is w P (row j of Dm^-1); corner 0 gets minus their sum
@@ -1837,6 +1841,7 @@ template <class T> void PdEngine<T>::build_batch(const PdParams& params) {
         bp.pcg_tol = params.pcg_tol;
         bp.pcg_floor = params.pcg_floor;
         bp.pcg_max_iter = params.pcg_max_iter;
+        bp.polar = params.polar_rotation != 0;
         batch_->autotune(data_.X, data_.V, data_.DBCX, bp, stream_);
     }
 }
@@ -1875,7 +1880,8 @@ template <class T> void PdEngine<T>::launch_local(const T* q, bool residual_form
     // block 0 also clears the reduction slots of the global step that follows
     hipLaunchKernelGGL((local_chunk_kernel<T, false>), dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_,
                        tet_lidx_.get(), chunk_vptr_.get(), chunk_verts_.get(), entry_pos_.get(), inc_codes_.get(),
-                       inc_end_.get(), dminv_.get(), w_.get(), q, partial_f_.get(), (int)residual_form, skip,
+                       inc_end_.get(), dminv_.get(), w_.get(), q, partial_f_.get(),
+                       (residual_form ? kLocalResidual : 0) | (polar_ ? kLocalPolar : 0), skip,
                        slots_.get(), kNumSlots * kSlotSize, FoldIn<T>{}, PcgArgs<T>{}, 0);
     tick(kKLocal, false);
     SBS_HIP(hipGetLastError());
@@ -1898,8 +1904,9 @@ void PdEngine<T>::launch_local_fold(const T* x_int, const T* sn_in, const T* pre
     tick(kKLocal, true);
     hipLaunchKernelGGL((local_chunk_kernel<T, true>), dim3(nchunks_), dim3(kChunkTets), lds, stream_, nt_,
                        tet_lidx_.get(), chunk_vptr_.get(), chunk_verts_.get(), entry_pos_.get(), inc_codes_.get(),
-                       inc_end_.get(), dminv_.get(), w_.get(), (const T*)nullptr, partial_f_.get(), 0,
-                       (const int*)nullptr, slots_.get(), kNumSlots * kSlotSize, f, c, close_L);
+                       inc_end_.get(), dminv_.get(), w_.get(), (const T*)nullptr, partial_f_.get(),
+                       polar_ ? kLocalPolar : 0, (const int*)nullptr, slots_.get(), kNumSlots * kSlotSize, f, c,
+                       close_L);
     tick(kKLocal, false);
     SBS_HIP(hipGetLastError());
 }
@@ -2050,6 +2057,8 @@ template <class T> void PdEngine<T>::sync_counters() {
 }
 
 template <class T> void PdEngine<T>::update(PdData<T>& data, const PdParams& params) {
+    SBS_REQUIRE(params.polar_rotation == 0 || params.polar_rotation == 1, "PdSolver: polar_rotation must be 0 or 1");
+    polar_ = params.polar_rotation != 0;
     if (!ready_) prepare(data, params);
     data_ = data;
     SBS_REQUIRE(params.num_iterations >= 0, "PdSolver: numIterations must be >= 0");
@@ -2146,7 +2155,7 @@ template <class T> std::vector<uint64_t> PdEngine<T>::step_key(const PdData<T>& 
                                (uint64_t)(uintptr_t)stream_, (uint64_t)solver_, (uint64_t)p.num_iterations,
                                bits(p.dt), bits(p.gravity), bits(p.tol), bits(p.rho), (uint64_t)p.pcg_max_iter,
                                bits(p.pcg_tol), bits(p.pcg_floor), (uint64_t)p.pcg_chebyshev, (uint64_t)p.pcg_halo,
-                               bits(p.mu_n), bits(p.mu_t)};
+                               bits(p.mu_n), bits(p.mu_t), (uint64_t)p.polar_rotation};
     k.push_back((uint64_t)cap_epoch_);
     return k;
 }
@@ -2209,6 +2218,7 @@ template <class T> void PdEngine<T>::enqueue_step(PdData<T>& data, const PdParam
         bp.pcg_floor = params.pcg_floor;
         bp.pcg_max_iter = params.pcg_max_iter;
         bp.write_x = params.handle_collision == 0;
+        bp.polar = params.polar_rotation != 0;
         if (perf_) SBS_HIP(hipEventRecord(ev[0], stream_));
         batch_->step(data.X, data.V, data.XTilde, data.DBCX, bp, stream_);
         if (perf_) {

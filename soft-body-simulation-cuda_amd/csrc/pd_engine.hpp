@@ -63,6 +63,7 @@ struct PdParams {
     int pcg_halo = 1;           // 1: halo-staged PCG kernels (when the tiles could be built); 0: row gathers
     int use_graph = 0;          // replay a captured HIP graph of the step when its launch sequence is fixed
     int batch_autotune = 1;     // BatchedPCG: time the workgroup sizes of each body-size class at prepare
+    int polar_rotation = 0;     // FP32 local step: 0 TR1690 SVD (as the reference), 1 Newton polar iteration
     double mu_n = 0.5;          // fixed-body response: normal restitution (SolverParams::muN)
     double mu_t = 0.5;          // fixed-body response: friction (SolverParams::muT)
 };
@@ -213,6 +214,7 @@ private:
     int nv_ = 0, nt_ = 0;
     PdData<T> data_{};
     GlobalSolver solver_ = GlobalSolver::Jacobi;
+    bool polar_ = false;  // local step rotation: Newton polar iteration (FP32) instead of the SVD
     bool ready_ = false;
     bool system_built_ = false;
     bool perf_ = false;

@@ -163,10 +163,8 @@ template <class T> __device__ __forceinline__ T det3(const T (&m)[3][3]) {
            m[0][2] * (m[1][0] * m[2][1] - m[2][0] * m[1][1]);
 }
 
-// Projection of computeLocal (src/simulation/solver/projective/pdUtil.cu:112-125):
-// R = U V^T, negate column 2 if det R < 0, and R - F in Jacobi (residual) mode.
-template <class T>
-__device__ __forceinline__ void pd_projection(const T (&F)[3][3], bool residual_form, T (&P)[3][3]) {
+// R = U V^T from the TR1690 SVD, column 2 negated if det R < 0.
+template <class T> __device__ __forceinline__ void rotation_tr1690(const T (&F)[3][3], T (&P)[3][3]) {
     T U[3][3], sig[3], V[3][3];
     Svd3<T>::run(F, U, sig, V);
 #pragma unroll
@@ -176,6 +174,67 @@ __device__ __forceinline__ void pd_projection(const T (&F)[3][3], bool residual_
     const bool flip = det3(P) < (T)0;
 #pragma unroll
     for (int r = 0; r < 3; ++r) P[r][2] = flip ? -P[r][2] : P[r][2];
+}
+
+// Rotation factor of the polar decomposition F = R S by Newton's iteration
+// R <- (R + R^-T) / 2 from R = F (Higham 1986, unscaled): in the SVD basis
+// each singular value goes s -> (s + 1/s) / 2, which converges
+// quadratically to 1, so R -> U V^T, the same rotation TR1690 gives when
+// det F > 0.  R^-T is the cofactor matrix over the determinant.  Five
+// iterations bring every singular value in about [0.27, 3.7] to 1 within
+// FP32 rounding.  Returns false (R unusable) when det F <= 0 or an entry of
+// the last iteration still moved by more than 5e-4 (then the error after it
+// exceeds ~1.3e-7) or is not finite.
+__device__ __forceinline__ bool rotation_newton(const float (&F)[3][3], float (&R)[3][3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R[r][c] = F[r][c];
+    bool ok = true;
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        float C[3][3];
+        C[0][0] = R[1][1] * R[2][2] - R[1][2] * R[2][1];
+        C[0][1] = R[1][2] * R[2][0] - R[1][0] * R[2][2];
+        C[0][2] = R[1][0] * R[2][1] - R[1][1] * R[2][0];
+        C[1][0] = R[0][2] * R[2][1] - R[0][1] * R[2][2];
+        C[1][1] = R[0][0] * R[2][2] - R[0][2] * R[2][0];
+        C[1][2] = R[0][1] * R[2][0] - R[0][0] * R[2][1];
+        C[2][0] = R[0][1] * R[1][2] - R[0][2] * R[1][1];
+        C[2][1] = R[0][2] * R[1][0] - R[0][0] * R[1][2];
+        C[2][2] = R[0][0] * R[1][1] - R[0][1] * R[1][0];
+        const float det = R[0][0] * C[0][0] + R[0][1] * C[0][1] + R[0][2] * C[0][2];
+        if (it == 0) ok = det > 0.0f;
+        const float h = 0.5f * __builtin_amdgcn_rcpf(det);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float n = fmaf(0.5f, R[r][c], h * C[r][c]);
+                if (it == 4) ok = ok && fabsf(n - R[r][c]) <= 5e-4f;  // false for NaN
+                R[r][c] = n;
+            }
+    }
+    return ok;
+}
+
+// Projection of computeLocal (src/simulation/solver/projective/pdUtil.cu:112-125):
+// R = U V^T, negate column 2 if det R < 0, and R - F in Jacobi (residual) mode.
+// `polar` (FP32 only): take the rotation from Newton's polar iteration and
+// fall back to the TR1690 SVD for the elements it does not settle (inverted,
+// degenerate or strongly stretched).  Where TR1690's four sweeps have
+// converged the two agree to rounding; on strongly sheared elements the
+// sweeps can stop up to ~1e-3 short of the polar factor, which Newton's
+// rotation is to FP32 rounding -- so `polar` is exact where the reference's
+// SVD is not, and differs from it there.
+template <class T>
+__device__ __forceinline__ void pd_projection(const T (&F)[3][3], bool residual_form, T (&P)[3][3],
+                                              bool polar = false) {
+    if constexpr (sizeof(T) == 4) {
+        if (!polar || !rotation_newton(F, P)) rotation_tr1690(F, P);
+    } else {
+        rotation_tr1690(F, P);
+    }
     if (residual_form) {
 #pragma unroll
         for (int r = 0; r < 3; ++r)

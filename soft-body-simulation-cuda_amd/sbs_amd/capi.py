@@ -52,7 +52,7 @@ class SbsSolverParams(ctypes.Structure):
                 ("pcg_chebyshev", ctypes.c_int), ("pcg_adaptive", ctypes.c_int), ("dense_precision", ctypes.c_int),
                 ("dense_tol", ctypes.c_double), ("dense_max_iter", ctypes.c_int), ("pcg_halo", ctypes.c_int),
                 ("mu_n", ctypes.c_double), ("mu_t", ctypes.c_double), ("use_graph", ctypes.c_int),
-                ("batch_autotune", ctypes.c_int)]
+                ("batch_autotune", ctypes.c_int), ("polar_rotation", ctypes.c_int)]
 
 
 class SbsPdStats(ctypes.Structure):

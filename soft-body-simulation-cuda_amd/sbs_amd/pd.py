@@ -50,6 +50,7 @@ class SolverParams:
     muT: float = 0.5              # fixed-body response: friction
     use_graph: bool = False       # replay the step as a captured HIP graph (needs a non-null stream)
     batch_autotune: bool = True   # BatchedPCG: time each size class's workgroup sizes at the first update
+    polar_rotation: bool = False  # FP32 local step: Newton polar iteration instead of the TR1690 SVD
 
     def to_c(self) -> capi.SbsSolverParams:
         return capi.SbsSolverParams(int(self.numIterations), float(self.dt), float(self.gravity), float(self.tol),
@@ -57,7 +58,8 @@ class SolverParams:
                                     float(self.pcg_floor), int(self.pcg_max_iter), int(self.pcg_chebyshev),
                                     int(self.pcg_adaptive), int(self.dense_precision), float(self.dense_tol),
                                     int(self.dense_max_iter), int(self.pcg_halo), float(self.muN),
-                                    float(self.muT), int(self.use_graph), int(self.batch_autotune))
+                                    float(self.muT), int(self.use_graph), int(self.batch_autotune),
+                                    int(self.polar_rotation))
 
 
 class SolverData:

@@ -71,10 +71,10 @@ def run_pair(s, p, nsteps):
     return out
 
 
-def check_steps(s, out):
+def check_steps(s, out, rel=1e-10):
     for k, (Xg, Xo, sg, so) in enumerate(out):
         scale = max(np.abs(Xo - s.X).max(), 1e-6)
-        assert np.abs(Xg - Xo).max() <= 1e-10 * scale, (k, np.abs(Xg - Xo).max(), scale, sg, so)
+        assert np.abs(Xg - Xo).max() <= rel * scale, (k, np.abs(Xg - Xo).max(), scale, sg, so)
         assert sg["newton_iterations"] == so["newton_iterations"], (k, sg, so)
         assert sg["line_search_halvings"] == so["line_search_halvings"], (k, sg, so)
         assert np.isclose(sg["kappa"], so["kappa"], rtol=1e-8)
@@ -112,7 +112,12 @@ def test_config4_steps_match_oracle(gpu):
     finally:
         solver.close()
         data.free()
-    check_steps(s, out)
+    # in contact each Newton step is a linear solve stopped at a relative
+    # residual of 1e-10 (~450 PCG iterations at kappa ~2.5e3) and Newton stops
+    # at 5e-2, so the two solvers' iterates may differ by ~1e-10 of the step
+    # (the GPU's float atomics make the last digits vary run to run): 2e-11 of
+    # the 0.15 m displacement measured, bar 1e-9 of it
+    check_steps(s, out, rel=1e-9)
     assert all(sg["queries"] > 0 for _, _, sg, _ in out)
 
 
