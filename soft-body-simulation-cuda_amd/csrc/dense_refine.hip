@@ -33,6 +33,7 @@ constexpr int kWaves = 4;      // wavefronts per workgroup of the correction ker
 constexpr int kSliceD = 64;    // sliced-ELL slice height of the engine's matrix
 constexpr int kBSlot = 0;      // norms: slot 0 holds max |D^-1 b|; slot k+1 holds max |D^-1 r_k|
 constexpr int kTpbD = 256;
+constexpr size_t kFusedLdsMax = 64 * 1024;  // one-workgroup-per-body solve when a body's LDS fits
 
 inline uint16_t to_bf16(float f) {  // round to nearest even
     uint32_t u;
@@ -215,6 +216,185 @@ __global__ void refine_report_kernel(int max_iter, const unsigned long long* nor
     out[1] = ratio;
 }
 
+// ---------------------------------------------------------------------------
+// Whole solve per body in one workgroup (bodies whose x, residual fragments
+// and scratch fit the LDS): x (FP64) and the residual's MFMA B fragments stay
+// in LDS across the corrections, each body stops on its own test, and there
+// is no cross-workgroup reduction and one launch per solve.  Per correction
+// the 4 wavefronts take the body's 16-row blocks in turn (inverse fragments
+// streamed from HBM, B fragments from LDS); the residual r = b - A x is
+// formed row by row from the sliced-ELL matrix and x in LDS.
+
+constexpr int kFusedThreads = 256;
+
+template <int MODE> __host__ __device__ constexpr size_t fused_lds_bytes(int P) {
+    // x: 3 P doubles; 16 doubles of reduction scratch; B fragments: 16 P
+    // values (FP32 or BF16)
+    return (3 * (size_t)P + 16) * sizeof(double) + 16 * (size_t)P * (MODE == 0 ? 4 : 2);
+}
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Maximum over the workgroup of three non-negative values (NaN counts as
+// infinity, so a NaN residual never passes the stop test).
+__device__ __forceinline__ void block_max3(const double (&s)[3], double* red, double (&out)[3]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double v = wave_max_f64(s[c] == s[c] ? s[c] : INFINITY);
+        if (lane == 0) red[3 * wave + c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = fmax(fmax(red[c], red[3 + c]), fmax(red[6 + c], red[9 + c]));
+    __syncthreads();
+}
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(kFusedThreads)
+    dense_body_solve_kernel(EllView<T> A, const T* __restrict__ b, T* __restrict__ x, const int* __restrict__ body_row0,
+                            const int* __restrict__ body_n, const int* __restrict__ body_p,
+                            const long long* __restrict__ tile_off, const void* __restrict__ tiles, double tol,
+                            int max_iter, int* __restrict__ body_iters, double* __restrict__ body_ratio,
+                            const int* skip) {
+    if (skip && *skip) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int bd = blockIdx.x;
+    const int r0 = body_row0[bd], nb = body_n[bd], P = body_p[bd];
+    double* x64 = reinterpret_cast<double*>(smem_raw);  // [row][xyz]
+    double* red = x64 + 3 * (size_t)P;
+    void* rhs = red + 16;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < 3 * P; i += kFusedThreads) x64[i] = 0.0;
+    const int words = MODE == 0 ? 16 * P : 8 * P;  // B fragments as 32-bit words; padding stays 0
+    for (int i = tid; i < words; i += kFusedThreads) reinterpret_cast<uint32_t*>(rhs)[i] = 0u;
+    __syncthreads();
+    double s[3] = {0, 0, 0};
+    for (int kl = tid; kl < nb; kl += kFusedThreads) {
+        const int v = r0 + kl;
+        const double d = (double)A.diag[v];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double r = (double)b[3 * (size_t)v + c];
+            put_rhs<MODE>(rhs, rhs_pos<MODE>(0, kl, c), r);
+            s[c] = fmax(s[c], fabs(r / d));
+        }
+    }
+    double bn[3], rn[3];
+    block_max3(s, red, bn);
+    rn[0] = bn[0]; rn[1] = bn[1]; rn[2] = bn[2];
+    int k = 0;
+    bool ok = false;
+    for (;; ++k) {
+        ok = rn[0] <= tol * bn[0] && rn[1] <= tol * bn[1] && rn[2] <= tol * bn[2];
+        if (ok || k == max_iter) break;
+        // x += A~^-1 r
+        for (int rb = wave; rb < P / 16; rb += kFusedThreads / 64) {
+            floatx4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+            if (MODE == 0) {
+                const int nk = P / 4;  // a multiple of 8
+                const float* ta = reinterpret_cast<const float*>(tiles) + tile_off[bd] + (long long)rb * nk * 64;
+                const float* tb = reinterpret_cast<const float*>(rhs);
+                for (int q = 0; q < nk; q += 2) {
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[q * 64 + lane], tb[q * 64 + lane], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[(q + 1) * 64 + lane], tb[(q + 1) * 64 + lane],
+                                                                acc1, 0, 0, 0);
+                }
+            } else {
+                const int nk = P / 32;
+                const bf16x8* ta =
+                    reinterpret_cast<const bf16x8*>(tiles) + (tile_off[bd] + (long long)rb * nk * 512) / 8;
+                const bf16x8* tb = reinterpret_cast<const bf16x8*>(rhs);
+                int q = 0;
+                for (; q + 1 < nk; q += 2) {
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ta[q * 64 + lane], tb[q * 64 + lane], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ta[(q + 1) * 64 + lane], tb[(q + 1) * 64 + lane],
+                                                                   acc1, 0, 0, 0);
+                }
+                if (q < nk)
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ta[q * 64 + lane], tb[q * 64 + lane], acc0, 0, 0, 0);
+            }
+            const int col = lane & 15;
+            if (col < 3) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = rb * 16 + 4 * (lane >> 4) + j;
+                    if (row < nb) x64[3 * row + col] += (double)(acc0[j] + acc1[j]);
+                }
+            }
+        }
+        __syncthreads();
+        // r = b - A x (FP64), packed for the next correction
+        s[0] = s[1] = s[2] = 0;
+        for (int kl = tid; kl < nb; kl += kFusedThreads) {
+            const int v = r0 + kl;
+            const double d = (double)A.diag[v];
+            double ax[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ax[c] = d * x64[3 * kl + c];
+            const int sl = v / kSliceD, ln = v % kSliceD;
+            const size_t base = (size_t)A.slice_ptr[sl] + ln;
+            const int width = A.slice_w[sl];
+            for (int e = 0; e < width; ++e) {
+                const size_t pos = base + (size_t)e * kSliceD;
+                const int u = (int)A.ecol[pos] - r0;  // the body's own rows (padding: the row itself, value 0)
+                const double m = (double)A.eval[pos];
+                if ((unsigned)u < (unsigned)nb) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ax[c] += m * x64[3 * u + c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double r = (double)b[3 * (size_t)v + c] - ax[c];
+                put_rhs<MODE>(rhs, rhs_pos<MODE>(0, kl, c), r);
+                s[c] = fmax(s[c], fabs(r / d));
+            }
+        }
+        block_max3(s, red, rn);
+    }
+    for (int i = tid; i < 3 * nb; i += kFusedThreads) x[3 * (size_t)r0 + i] = (T)x64[i];
+    if (tid == 0) {
+        body_iters[bd] = ok ? k : -1 - max_iter;
+        double ratio = 0;
+        for (int c = 0; c < 3; ++c) ratio = fmax(ratio, bn[c] > 0 ? rn[c] / bn[c] : rn[c]);
+        body_ratio[bd] = ratio;
+    }
+}
+
+// The worst body: most corrections (-1 - max_iter if one stopped there) and
+// the largest residual ratio.
+__global__ void dense_body_report_kernel(int nb, int max_iter, const int* iters, const double* ratio, double* out,
+                                         const int* skip) {
+    if (skip && *skip) return;
+    __shared__ int worst_it;
+    __shared__ int failed;
+    __shared__ double worst_ratio[kFusedThreads];
+    if (threadIdx.x == 0) {
+        worst_it = 0;
+        failed = 0;
+    }
+    __syncthreads();
+    double rmax = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        const int it = iters[i];
+        if (it < 0) atomicOr(&failed, 1);
+        else atomicMax(&worst_it, it);
+        rmax = fmax(rmax, ratio[i]);
+    }
+    worst_ratio[threadIdx.x] = rmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int i = 0; i < (int)blockDim.x; ++i) r = fmax(r, worst_ratio[i]);
+        out[0] = failed ? (double)(-1 - max_iter) : (double)worst_it;
+        out[1] = r;
+    }
+}
+
 inline unsigned blocks_of(long long n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
 
 }  // namespace
@@ -243,6 +423,7 @@ void DenseRefinedSolver<T>::setup(const EllView<T>& A, const std::vector<int>& r
     nb_ = nc;
     max_n_ = *std::max_element(nrows.begin(), nrows.end());
     SBS_REQUIRE(max_n_ <= 4096, "dense solver: a body exceeds 4096 vertices (use a sparse global solver)");
+    fused_lds_ = 0;  // chosen below once the padded sizes are known
     std::vector<long long> toff(nc + 1, 0), roff(nc + 1, 0);
     std::vector<int2> items;
     for (int b = 0; b < nc; ++b) {
@@ -252,6 +433,11 @@ void DenseRefinedSolver<T>::setup(const EllView<T>& A, const std::vector<int>& r
         for (int rb = 0; rb < pad[b] / 16; ++rb) items.push_back(make_int2(b, rb));
     }
     nitems_ = (int)items.size();
+    {
+        const int pmax = *std::max_element(pad.begin(), pad.end());
+        const size_t lds = prec == DensePrecision::FP32 ? fused_lds_bytes<0>(pmax) : fused_lds_bytes<1>(pmax);
+        fused_lds_ = lds <= kFusedLdsMax ? lds : 0;
+    }
     const size_t esz = prec == DensePrecision::FP32 ? 4 : 2;
     tile_bytes_ = (size_t)toff[nc] * esz;
     std::vector<unsigned char> tiles(tile_bytes_, 0);
@@ -316,6 +502,8 @@ void DenseRefinedSolver<T>::setup(const EllView<T>& A, const std::vector<int>& r
     // padding rows and the 13 unused columns of the packed right-hand side stay zero
     SBS_HIP(hipMemset(rhs_.get(), 0, rhs_.bytes()));
     report_.alloc(2);
+    body_iters_.alloc(nc);
+    body_ratio_.alloc(nc);
     if (!host_report_) SBS_HIP(hipHostMalloc((void**)&host_report_, 2 * sizeof(double), hipHostMallocDefault));
 }
 
@@ -328,6 +516,18 @@ template <class T> void DenseRefinedSolver<T>::solve(const T* b, T* x, double to
         slots_ = need;
     }
     last_max_iter_ = max_iter;
+    if (fused_lds_) {
+        const bool f32 = prec_ == DensePrecision::FP32;
+        auto kern = f32 ? dense_body_solve_kernel<T, 0> : dense_body_solve_kernel<T, 1>;
+        hipLaunchKernelGGL(kern, dim3(nb_), dim3(kFusedThreads), fused_lds_, stream_, A_, b, x,
+                           (const int*)body_row0_.get(), (const int*)body_n_.get(), (const int*)body_p_.get(),
+                           (const long long*)tile_off_.get(), (const void*)tiles_.get(), tol, max_iter,
+                           body_iters_.get(), body_ratio_.get(), skip);
+        hipLaunchKernelGGL(dense_body_report_kernel, dim3(1), dim3(kFusedThreads), 0, stream_, nb_, max_iter,
+                           (const int*)body_iters_.get(), (const double*)body_ratio_.get(), report_.get(), skip);
+        SBS_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, stream_, slots_, norms_.get());
     const unsigned vb = blocks_of(n_, kTpbD);
     const bool f32 = prec_ == DensePrecision::FP32;

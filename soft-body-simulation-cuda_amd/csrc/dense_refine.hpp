@@ -9,6 +9,10 @@
 //                              r  = b - A x   (FP64, the assembled sparse A)
 // until max |D^-1 r| <= tol max |D^-1 b| per component (D = diag A; the pinned
 // rows make an unscaled residual meaningless), or max_iter corrections.
+// Bodies whose x, residual fragments and scratch fit 64 KB of LDS (up to 736
+// padded rows with FP32 fragments, 1152 with BF16) run the whole loop in one
+// workgroup per body, each body with its own stop test; larger ones run each
+// correction and residual as launches over all bodies with one stop test.
 #pragma once
 
 #include <cstdint>
@@ -46,6 +50,8 @@ public:
     int max_body() const { return max_n_; }
     size_t inverse_bytes() const { return tile_bytes_; }
     DensePrecision precision() const { return prec_; }
+    // true when every body is solved by one workgroup in one launch per solve
+    bool one_workgroup_per_body() const { return fused_lds_ != 0; }
     void set_stream(hipStream_t s) { stream_ = s; }
 
 private:
@@ -63,6 +69,9 @@ private:
     DevBuf<double> x64_;                 // 3n
     DevBuf<unsigned long long> norms_;   // per iteration: max |D^-1 r| xyz, then max |D^-1 b| xyz
     DevBuf<double> report_;              // [iterations, ratio]
+    DevBuf<int> body_iters_;             // one-workgroup-per-body solve: corrections per body
+    DevBuf<double> body_ratio_;          //   and the residual ratio each reached
+    size_t fused_lds_ = 0;               // LDS of that solve; 0: bodies too large, per-row-block launches
     double* host_report_ = nullptr;
     int slots_ = 0;
     int last_max_iter_ = 0;

@@ -180,17 +180,19 @@ template <class T> __device__ __forceinline__ void rotation_tr1690(const T (&F)[
 // R <- (R + R^-T) / 2 from R = F (Higham 1986, unscaled): in the SVD basis
 // each singular value goes s -> (s + 1/s) / 2, which converges
 // quadratically to 1, so R -> U V^T, the same rotation TR1690 gives when
-// det F > 0.  R^-T is the cofactor matrix over the determinant.  Five
-// iterations bring every singular value in about [0.27, 3.7] to 1 within
-// FP32 rounding.  Returns false (R unusable) when det F <= 0 or an entry of
-// the last iteration still moved by more than 5e-4 (then the error after it
-// exceeds ~1.3e-7) or is not finite.
+// det F > 0.  R^-T is the cofactor matrix over the determinant.  A step that
+// moves no entry by more than 5e-4 leaves an error below ~1.3e-7 (the error
+// after a step is about half the square of the step before it).  A
+// wavefront stops after the 4th step when every lane has settled so (or
+// will take the SVD); otherwise it runs a 5th, which settles singular values
+// in about [0.27, 3.7].  Returns false (R unusable) when det F <= 0 or the
+// last step moved an entry by more than 5e-4 or was not finite.
 __device__ __forceinline__ bool rotation_newton(const float (&F)[3][3], float (&R)[3][3]) {
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) R[r][c] = F[r][c];
-    bool ok = true;
+    bool det_ok = true, conv = false;
 #pragma unroll
     for (int it = 0; it < 5; ++it) {
         float C[3][3];
@@ -204,18 +206,20 @@ __device__ __forceinline__ bool rotation_newton(const float (&F)[3][3], float (&
         C[2][1] = R[0][2] * R[1][0] - R[0][0] * R[1][2];
         C[2][2] = R[0][0] * R[1][1] - R[0][1] * R[1][0];
         const float det = R[0][0] * C[0][0] + R[0][1] * C[0][1] + R[0][2] * C[0][2];
-        if (it == 0) ok = det > 0.0f;
+        if (it == 0) det_ok = det > 0.0f;
         const float h = 0.5f * __builtin_amdgcn_rcpf(det);
+        conv = true;
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float n = fmaf(0.5f, R[r][c], h * C[r][c]);
-                if (it == 4) ok = ok && fabsf(n - R[r][c]) <= 5e-4f;  // false for NaN
+                if (it >= 3) conv = conv && fabsf(n - R[r][c]) <= 5e-4f;  // false for NaN
                 R[r][c] = n;
             }
+        if (it == 3 && __all(conv || !det_ok)) break;  // wavefront-uniform
     }
-    return ok;
+    return det_ok && conv;
 }
 
 // Projection of computeLocal (src/simulation/solver/projective/pdUtil.cu:112-125):

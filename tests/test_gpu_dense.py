@@ -137,3 +137,20 @@ def test_dense_256_bodies_of_6cube_vs_fp64_oracle(gpu):
         e, m = rel_err(X, r64), np.abs(X - r64).max()
         assert e < min(1e-5, 1.5 * e_orc), (prec, e, e_orc)
         assert m < min(5e-4, 1.5 * m_orc), (prec, m, m_orc)
+
+
+@pytest.mark.parametrize("dense_precision", [0, 1])
+def test_dense_body_of_1000_vertices(gpu, dense_precision):
+    """A 9^3-hex body (1000 vertices, padded to 1024 rows): with FP32
+    fragments its x and residual fragments exceed 64 KB of LDS, so each
+    correction and residual runs as a launch over all row blocks; with BF16
+    fragments the body is solved in one workgroup.  Both against the oracle's
+    direct mode over 20 steps."""
+    s = mesh.make_scene([dict(nx=9, ny=9, nz=9, seed=4, pin_top=True, E=2e5)])
+    assert s.nv == 1000
+    p = params(n_iter=10, dense_precision=dense_precision, dense_max_iter=16)
+    X, info, worst = run_dense(s, p, 20, 32)
+    r = run_oracle(s, p, 20, False)
+    assert rel_err(X, r) < TOL[32] and max_rel(X, r) < TOL[32]
+    assert info["bodies"] == 1 and info["max_body"] == 1000
+    assert info["ratio"] <= 1e-10
