@@ -220,17 +220,22 @@ __global__ void refine_report_kernel(int max_iter, const unsigned long long* nor
 // Whole solve per body in one workgroup (bodies whose x, residual fragments
 // and scratch fit the LDS): x (FP64) and the residual's MFMA B fragments stay
 // in LDS across the corrections, each body stops on its own test, and there
-// is no cross-workgroup reduction and one launch per solve.  Per correction
-// the 4 wavefronts take the body's 16-row blocks in turn (inverse fragments
-// streamed from HBM, B fragments from LDS); the residual r = b - A x is
-// formed row by row from the sliced-ELL matrix and x in LDS.
+// is no cross-workgroup reduction and one launch per solve.  A correction
+// splits the body's (16-row block, K segment) products over the 16
+// wavefronts -- inverse fragments streamed from HBM 8 (BF16: 4) per lane in
+// flight, B fragments from LDS -- and sums the segments' partial products in
+// a fixed order; the residual r = b - A x is formed row by row from the
+// sliced-ELL matrix and x in LDS.
 
-constexpr int kFusedThreads = 256;
+constexpr int kFusedThreads = 1024;
+constexpr int kFusedWaves = kFusedThreads / 64;
 
-template <int MODE> __host__ __device__ constexpr size_t fused_lds_bytes(int P) {
-    // x: 3 P doubles; 16 doubles of reduction scratch; B fragments: 16 P
-    // values (FP32 or BF16)
-    return (3 * (size_t)P + 16) * sizeof(double) + 16 * (size_t)P * (MODE == 0 ? 4 : 2);
+template <int MODE> __host__ __device__ constexpr size_t fused_lds_bytes(int P, int nseg) {
+    // x: 3 P doubles; 3 doubles per wavefront of reduction scratch; B
+    // fragments: 16 P values (FP32 or BF16); partial products of the K
+    // segments: 3 P floats each (none with one segment)
+    return (3 * (size_t)P + 3 * kFusedWaves) * sizeof(double) + 16 * (size_t)P * (MODE == 0 ? 4 : 2) +
+           (nseg > 1 ? (size_t)nseg * 3 * P * sizeof(float) : 0);
 }
 
 __device__ __forceinline__ double wave_max_f64(double v) {
@@ -249,7 +254,11 @@ __device__ __forceinline__ void block_max3(const double (&s)[3], double* red, do
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[c] = fmax(fmax(red[c], red[3 + c]), fmax(red[6 + c], red[9 + c]));
+    for (int c = 0; c < 3; ++c) {
+        double m = red[c];
+        for (int w = 1; w < kFusedWaves; ++w) m = fmax(m, red[3 * w + c]);
+        out[c] = m;
+    }
     __syncthreads();
 }
 
@@ -257,8 +266,8 @@ template <class T, int MODE>
 __global__ void __launch_bounds__(kFusedThreads)
     dense_body_solve_kernel(EllView<T> A, const T* __restrict__ b, T* __restrict__ x, const int* __restrict__ body_row0,
                             const int* __restrict__ body_n, const int* __restrict__ body_p,
-                            const long long* __restrict__ tile_off, const void* __restrict__ tiles, double tol,
-                            int max_iter, int* __restrict__ body_iters, double* __restrict__ body_ratio,
+                            const long long* __restrict__ tile_off, const void* __restrict__ tiles, int nseg,
+                            double tol, int max_iter, int* __restrict__ body_iters, double* __restrict__ body_ratio,
                             const int* skip) {
     if (skip && *skip) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -266,7 +275,8 @@ __global__ void __launch_bounds__(kFusedThreads)
     const int r0 = body_row0[bd], nb = body_n[bd], P = body_p[bd];
     double* x64 = reinterpret_cast<double*>(smem_raw);  // [row][xyz]
     double* red = x64 + 3 * (size_t)P;
-    void* rhs = red + 16;
+    void* rhs = red + 3 * kFusedWaves;
+    float* part = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(rhs) + 16 * (size_t)P * (MODE == 0 ? 4 : 2));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < 3 * P; i += kFusedThreads) x64[i] = 0.0;
     const int words = MODE == 0 ? 16 * P : 8 * P;  // B fragments as 32-bit words; padding stays 0
@@ -286,35 +296,53 @@ __global__ void __launch_bounds__(kFusedThreads)
     double bn[3], rn[3];
     block_max3(s, red, bn);
     rn[0] = bn[0]; rn[1] = bn[1]; rn[2] = bn[2];
+    const int nrb = P / 16, nk = MODE == 0 ? P / 4 : P / 32, seg = (nk + nseg - 1) / nseg;
     int k = 0;
     bool ok = false;
     for (;; ++k) {
         ok = rn[0] <= tol * bn[0] && rn[1] <= tol * bn[1] && rn[2] <= tol * bn[2];
         if (ok || k == max_iter) break;
-        // x += A~^-1 r
-        for (int rb = wave; rb < P / 16; rb += kFusedThreads / 64) {
+        // x += A~^-1 r: (row block, K segment) products
+        for (int item = wave; item < nrb * nseg; item += kFusedWaves) {
+            const int rb = item / nseg, sg = item - rb * nseg;
+            const int q0 = sg * seg, q1 = min(nk, q0 + seg);
             floatx4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
             if (MODE == 0) {
-                const int nk = P / 4;  // a multiple of 8
                 const float* ta = reinterpret_cast<const float*>(tiles) + tile_off[bd] + (long long)rb * nk * 64;
                 const float* tb = reinterpret_cast<const float*>(rhs);
-                for (int q = 0; q < nk; q += 2) {
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[q * 64 + lane], tb[q * 64 + lane], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[(q + 1) * 64 + lane], tb[(q + 1) * 64 + lane],
-                                                                acc1, 0, 0, 0);
+                int q = q0;
+                for (; q + 8 <= q1; q += 8) {
+                    float fa[8], fb[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) fa[j] = ta[(q + j) * 64 + lane];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) fb[j] = tb[(q + j) * 64 + lane];
+#pragma unroll
+                    for (int j = 0; j < 8; j += 2) {
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[j], fb[j], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[j + 1], fb[j + 1], acc1, 0, 0, 0);
+                    }
                 }
+                for (; q < q1; ++q)
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[q * 64 + lane], tb[q * 64 + lane], acc0, 0, 0, 0);
             } else {
-                const int nk = P / 32;
                 const bf16x8* ta =
                     reinterpret_cast<const bf16x8*>(tiles) + (tile_off[bd] + (long long)rb * nk * 512) / 8;
                 const bf16x8* tb = reinterpret_cast<const bf16x8*>(rhs);
-                int q = 0;
-                for (; q + 1 < nk; q += 2) {
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ta[q * 64 + lane], tb[q * 64 + lane], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ta[(q + 1) * 64 + lane], tb[(q + 1) * 64 + lane],
-                                                                   acc1, 0, 0, 0);
+                int q = q0;
+                for (; q + 4 <= q1; q += 4) {
+                    bf16x8 fa[4], fb[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) fa[j] = ta[(q + j) * 64 + lane];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) fb[j] = tb[(q + j) * 64 + lane];
+#pragma unroll
+                    for (int j = 0; j < 4; j += 2) {
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j], fb[j], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j + 1], fb[j + 1], acc1, 0, 0, 0);
+                    }
                 }
-                if (q < nk)
+                for (; q < q1; ++q)
                     acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ta[q * 64 + lane], tb[q * 64 + lane], acc0, 0, 0, 0);
             }
             const int col = lane & 15;
@@ -322,11 +350,22 @@ __global__ void __launch_bounds__(kFusedThreads)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int row = rb * 16 + 4 * (lane >> 4) + j;
-                    if (row < nb) x64[3 * row + col] += (double)(acc0[j] + acc1[j]);
+                    if (row < nb) {
+                        if (nseg == 1) x64[3 * row + col] += (double)(acc0[j] + acc1[j]);
+                        else part[(size_t)sg * 3 * P + 3 * row + col] = acc0[j] + acc1[j];
+                    }
                 }
             }
         }
         __syncthreads();
+        if (nseg > 1) {  // the segments' partial products, summed in segment order
+            for (int i = tid; i < 3 * nb; i += kFusedThreads) {
+                double sum = 0;
+                for (int sg = 0; sg < nseg; ++sg) sum += (double)part[(size_t)sg * 3 * P + i];
+                x64[i] += sum;
+            }
+            __syncthreads();
+        }
         // r = b - A x (FP64), packed for the next correction
         s[0] = s[1] = s[2] = 0;
         for (int kl = tid; kl < nb; kl += kFusedThreads) {
@@ -372,7 +411,7 @@ __global__ void dense_body_report_kernel(int nb, int max_iter, const int* iters,
     if (skip && *skip) return;
     __shared__ int worst_it;
     __shared__ int failed;
-    __shared__ double worst_ratio[kFusedThreads];
+    __shared__ double worst_ratio[256];
     if (threadIdx.x == 0) {
         worst_it = 0;
         failed = 0;
@@ -434,9 +473,17 @@ void DenseRefinedSolver<T>::setup(const EllView<T>& A, const std::vector<int>& r
     }
     nitems_ = (int)items.size();
     {
+        // the most K segments (4, 2, 1) whose partial products still fit
         const int pmax = *std::max_element(pad.begin(), pad.end());
-        const size_t lds = prec == DensePrecision::FP32 ? fused_lds_bytes<0>(pmax) : fused_lds_bytes<1>(pmax);
-        fused_lds_ = lds <= kFusedLdsMax ? lds : 0;
+        fused_lds_ = 0;
+        for (int ns : {4, 2, 1}) {
+            const size_t lds = prec == DensePrecision::FP32 ? fused_lds_bytes<0>(pmax, ns) : fused_lds_bytes<1>(pmax, ns);
+            if (lds <= kFusedLdsMax) {
+                fused_lds_ = lds;
+                fused_nseg_ = ns;
+                break;
+            }
+        }
     }
     const size_t esz = prec == DensePrecision::FP32 ? 4 : 2;
     tile_bytes_ = (size_t)toff[nc] * esz;
@@ -521,9 +568,9 @@ template <class T> void DenseRefinedSolver<T>::solve(const T* b, T* x, double to
         auto kern = f32 ? dense_body_solve_kernel<T, 0> : dense_body_solve_kernel<T, 1>;
         hipLaunchKernelGGL(kern, dim3(nb_), dim3(kFusedThreads), fused_lds_, stream_, A_, b, x,
                            (const int*)body_row0_.get(), (const int*)body_n_.get(), (const int*)body_p_.get(),
-                           (const long long*)tile_off_.get(), (const void*)tiles_.get(), tol, max_iter,
+                           (const long long*)tile_off_.get(), (const void*)tiles_.get(), fused_nseg_, tol, max_iter,
                            body_iters_.get(), body_ratio_.get(), skip);
-        hipLaunchKernelGGL(dense_body_report_kernel, dim3(1), dim3(kFusedThreads), 0, stream_, nb_, max_iter,
+        hipLaunchKernelGGL(dense_body_report_kernel, dim3(1), dim3(256), 0, stream_, nb_, max_iter,
                            (const int*)body_iters_.get(), (const double*)body_ratio_.get(), report_.get(), skip);
         SBS_HIP(hipGetLastError());
         return;

@@ -11,8 +11,9 @@
 // rows make an unscaled residual meaningless), or max_iter corrections.
 // Bodies whose x, residual fragments and scratch fit 64 KB of LDS (up to 736
 // padded rows with FP32 fragments, 1152 with BF16) run the whole loop in one
-// workgroup per body, each body with its own stop test; larger ones run each
-// correction and residual as launches over all bodies with one stop test.
+// workgroup of 16 wavefronts per body, each body with its own stop test;
+// larger ones run each correction and residual as launches over all bodies
+// with one stop test.
 #pragma once
 
 #include <cstdint>
@@ -72,6 +73,7 @@ private:
     DevBuf<int> body_iters_;             // one-workgroup-per-body solve: corrections per body
     DevBuf<double> body_ratio_;          //   and the residual ratio each reached
     size_t fused_lds_ = 0;               // LDS of that solve; 0: bodies too large, per-row-block launches
+    int fused_nseg_ = 1;                 // K segments of its products (partial sums in LDS)
     double* host_report_ = nullptr;
     int slots_ = 0;
     int last_max_iter_ = 0;
