@@ -738,20 +738,53 @@ __global__ void ipc_dot_kernel(int n, const double* a, const double* b, double* 
     block_sums<1>(v, parts);
 }
 
-// q += the element blocks (blocks [0, ntb)) and the active contact blocks
-// (the blocks after) times x.
-__global__ void ipc_mv_blocks_kernel(int nt, int ntb, const uint32_t* tet, const double* ht, long long nq,
-                                     const QueryKey* qk, const double* qd, double dhat2, const double* hq,
-                                     const unsigned char* fixed, const double* x, double* y, const int* stop) {
+// The tets' 12x12 Hessian blocks summed into 3x3 blocks per (vertex, vertex)
+// pair, once per Newton iteration: entry e of block k is the sum, in
+// ascending tet order, of the matching entries of the tets that hold both
+// vertices (`contrib`: tet << 4 | a << 2 | b for row vertex a and column
+// vertex b of the tet).  Blocks touching a fixed vertex are zero (the
+// elimination the element product applies).
+__global__ void ipc_assemble_rows_kernel(long long nnzb, const int* row_of, const int* col,
+                                         const long long* contrib_ptr, const uint32_t* contrib, const double* ht,
+                                         const unsigned char* fixed, double* val) {
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= 9 * nnzb) return;
+    const long long k = g / 9;
+    const int e = (int)(g - 9 * k), r = e / 3, c = e - 3 * r;
+    double s = 0.0;
+    if (!fixed[row_of[k]] && !fixed[col[k]]) {
+        for (long long q = contrib_ptr[k]; q < contrib_ptr[k + 1]; ++q) {
+            const uint32_t code = contrib[q];
+            const size_t t = code >> 4;
+            const int a = (code >> 2) & 3, b = code & 3;
+            s += ht[144 * t + (size_t)(3 * a + r) * 12 + 3 * b + c];
+        }
+    }
+    val[9 * k + e] = s;
+}
+
+// q += the assembled elastic rows (threads [0, 3 nv): one per row and
+// component, the row's blocks in column order) and the active contact
+// blocks (the blocks after) times x.
+__global__ void ipc_mv_blocks_kernel(int nv, int nrb, const long long* rowptr, const int* col, const double* val,
+                                     long long nq, const QueryKey* qk, const double* qd, double dhat2,
+                                     const double* hq, const unsigned char* fixed, const double* x, double* y,
+                                     const int* stop) {
     if (*stop) return;
-    if ((int)blockIdx.x < ntb) {
-        const int t = blockIdx.x * kT + threadIdx.x;
-        if (t >= nt) return;
-        const uint32_t vv[4] = {tet[4 * (size_t)t], tet[4 * (size_t)t + 1], tet[4 * (size_t)t + 2],
-                                tet[4 * (size_t)t + 3]};
-        mv_block(vv, ht + 144 * (size_t)t, fixed, x, y);
+    if ((int)blockIdx.x < nrb) {
+        const int g = blockIdx.x * kT + threadIdx.x;
+        if (g >= 3 * nv) return;
+        const int v = g / 3, r = g - 3 * v;
+        if (fixed[v]) return;
+        double s = 0.0;
+        for (long long k = rowptr[v]; k < rowptr[v + 1]; ++k) {
+            const double* b = val + 9 * k + 3 * r;
+            const size_t j = 3 * (size_t)col[k];
+            s += b[0] * x[j] + b[1] * x[j + 1] + b[2] * x[j + 2];
+        }
+        atomicAdd(&y[g], s);
     } else {
-        const long long i = (long long)(blockIdx.x - ntb) * kT + threadIdx.x;
+        const long long i = (long long)(blockIdx.x - nrb) * kT + threadIdx.x;
         if (i >= nq || qd[i] > dhat2) return;
         const QueryKey k = qk[i];
         const uint32_t vv[4] = {k.v0, k.v1, k.v2, k.v3};
@@ -864,6 +897,7 @@ IpcEngine::IpcEngine(const IpcData& data) {
         std::vector<uint32_t> tet(4 * (size_t)nt_);
         SBS_HIP(hipMemcpy(tet.data(), data.Tet, tet.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
         for (uint32_t v : tet) SBS_REQUIRE(v < (uint32_t)nv_, "IPCSolver: tet index out of range");
+        build_rows(tet);
         // contact areas (computeInvDmV0, solverUtil.cuh:123-136, femSolver.cu:13-16):
         // over a vertex's tets, a third of the summed areas of the tet's three
         // faces through it, divided by the number of tets (accumulated from 0)
@@ -919,6 +953,57 @@ IpcEngine::IpcEngine(const IpcData& data) {
     SBS_HIP(hipGetLastError());
     col_ = std::make_unique<CollisionDetector<double>>(nv_, ntri_, data.Tri, data.TriFathers, stream_);
     SBS_HIP(hipStreamSynchronize(stream_));
+}
+
+// The pattern of the assembled elastic Hessian: per vertex the sorted list of
+// the vertices it shares a tet with (itself included), and per 3x3 block the
+// (tet, a, b) entries that sum into it, in ascending tet order.
+void IpcEngine::build_rows(const std::vector<uint32_t>& tet) {
+    std::vector<std::vector<int>> nb(nv_);
+    for (int t = 0; t < nt_; ++t)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) nb[tet[4 * (size_t)t + a]].push_back((int)tet[4 * (size_t)t + b]);
+    std::vector<long long> rowptr(nv_ + 1, 0);
+    std::vector<int> col, row_of;
+    for (int v = 0; v < nv_; ++v) {
+        auto& l = nb[v];
+        std::sort(l.begin(), l.end());
+        l.erase(std::unique(l.begin(), l.end()), l.end());
+        rowptr[v + 1] = rowptr[v] + (long long)l.size();
+        for (int j : l) {
+            col.push_back(j);
+            row_of.push_back(v);
+        }
+    }
+    nnzb_ = rowptr[nv_];
+    std::vector<long long> cnt(nnzb_ + 1, 0);
+    auto block = [&](int i, int j) {
+        const auto& l = nb[i];
+        return rowptr[i] + (long long)(std::lower_bound(l.begin(), l.end(), j) - l.begin());
+    };
+    for (int t = 0; t < nt_; ++t)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) cnt[block((int)tet[4 * (size_t)t + a], (int)tet[4 * (size_t)t + b]) + 1]++;
+    for (long long k = 0; k < nnzb_; ++k) cnt[k + 1] += cnt[k];
+    std::vector<uint32_t> contrib(cnt[nnzb_]);
+    std::vector<long long> fill(cnt.begin(), cnt.end() - 1);
+    SBS_REQUIRE((size_t)nt_ < (1u << 28), "IPCSolver: too many tets for the assembled Hessian");
+    for (int t = 0; t < nt_; ++t)  // ascending t, so each block's list is in tet order
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                contrib[fill[block((int)tet[4 * (size_t)t + a], (int)tet[4 * (size_t)t + b])]++] =
+                    ((uint32_t)t << 4) | ((uint32_t)a << 2) | (uint32_t)b;
+    row_ptr_.alloc(nv_ + 1);
+    row_col_.alloc(std::max<long long>(nnzb_, 1));
+    row_of_.alloc(std::max<long long>(nnzb_, 1));
+    row_val_.alloc(9 * std::max<long long>(nnzb_, 1));
+    contrib_ptr_.alloc(nnzb_ + 1);
+    contrib_.alloc(std::max<size_t>(contrib.size(), 1));
+    SBS_HIP(hipMemcpy(row_ptr_.get(), rowptr.data(), rowptr.size() * sizeof(long long), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(row_col_.get(), col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(row_of_.get(), row_of.data(), row_of.size() * sizeof(int), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(contrib_ptr_.get(), cnt.data(), cnt.size() * sizeof(long long), hipMemcpyHostToDevice));
+    SBS_HIP(hipMemcpy(contrib_.get(), contrib.data(), contrib.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 }
 
 IpcEngine::~IpcEngine() {
@@ -1112,7 +1197,7 @@ void IpcEngine::update_kappa(const double* X) {
 int IpcEngine::solve() {
     constexpr int kPcgChunk = 16;
     const int n = 3 * nv_;
-    const unsigned nb = nblk(n), nbv = nblk(nv_), ntb = nblk(nt_);
+    const unsigned nb = nblk(n), nbv = nblk(nv_), nrb = nblk(n);
     const unsigned nbq = nq_ > 0 ? nblk((size_t)nq_) : 0;
     double* out = out_.get();  // [0] rz0, [1] rr, [2] pq, [3] alpha, [4] beta, [5] |b|, [6..7] rz by parity
     int* ctl = pcg_ctl_.get();
@@ -1139,8 +1224,9 @@ int IpcEngine::solve() {
         const int chunk = std::min(kPcgChunk, p_.lin_max_iter - launched);
         for (int j = 0; j < chunk; ++j) {
             const int it = launched + j;
-            hipLaunchKernelGGL(ipc_mv_blocks_kernel, dim3(ntb + nbq), dim3(kT), 0, stream_, nt_, (int)ntb, d_.Tet,
-                               (const double*)ht_.get(), nq_, nq_ > 0 ? col_->queries() : nullptr,
+            hipLaunchKernelGGL(ipc_mv_blocks_kernel, dim3(nrb + nbq), dim3(kT), 0, stream_, nv_, (int)nrb,
+                               (const long long*)row_ptr_.get(), (const int*)row_col_.get(),
+                               (const double*)row_val_.get(), nq_, nq_ > 0 ? col_->queries() : nullptr,
                                (const double*)qd_.get(), p_.dhat * p_.dhat, (const double*)hq_.get(),
                                (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get(), (const int*)ctl);
             hipLaunchKernelGGL(ipc_dot_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)pvec,
@@ -1165,6 +1251,11 @@ int IpcEngine::solve() {
 // p = H^-1 g; false when the gradient holds NaN or Inf.
 bool IpcEngine::search_direction(const double* X) {
     gradient(X, true, kappa_);
+    hipLaunchKernelGGL(ipc_assemble_rows_kernel, dim3(nblk(9 * (size_t)nnzb_)), dim3(kT), 0, stream_, nnzb_,
+                       (const int*)row_of_.get(), (const int*)row_col_.get(), (const long long*)contrib_ptr_.get(),
+                       (const uint32_t*)contrib_.get(), (const double*)ht_.get(), (const unsigned char*)fixed_.get(),
+                       row_val_.get());
+    SBS_HIP(hipGetLastError());
     const double gm = max_abs(g_.get(), 3 * (size_t)nv_);
     if (!std::isfinite(gm)) return false;
     stats_.linear_iterations = solve();

@@ -94,6 +94,7 @@ private:
     void gradient(const double* X, bool hessian, double coef_barrier_kappa);
     void update_kappa(const double* X);
     bool search_direction(const double* X);
+    void build_rows(const std::vector<uint32_t>& tet);
     int solve();
     double reduce_sum(const double* parts, int n);
     double max_abs(const double* v, size_t n);
@@ -115,6 +116,13 @@ private:
     DevBuf<unsigned char> fixed_;
     DevBuf<double> xn_, xtmp_, p_dir_, g_, diag_;
     DevBuf<double> ht_, hq_;                // 144 per tet / per query
+    // the tets' Hessian assembled per Newton iteration into 3x3 blocks per
+    // (vertex, vertex) pair: rows in vertex order, blocks in column order
+    long long nnzb_ = 0;
+    DevBuf<long long> row_ptr_, contrib_ptr_;
+    DevBuf<int> row_col_, row_of_;
+    DevBuf<double> row_val_;                // 9 per block, row-major
+    DevBuf<uint32_t> contrib_;              // tet << 4 | a << 2 | b per summed entry
     DevBuf<int> qtype_;                     // distance type per candidate query
     DevBuf<double> qd_;                     // squared distance per candidate query
     size_t qcap_ = 0;
