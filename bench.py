@@ -251,7 +251,7 @@ def batched_scene(nbodies: int = 1024, seed: int = 0):
     return mesh.make_scene(bodies), bodies
 
 
-def batched_measure(steps: int, warmup: int, sample: int, fp64_check: bool = True):
+def batched_measure(steps: int, warmup: int, sample: int, fp64_check: bool = True, polar: bool = False):
     """The 1024 uneven bodies on the batched solver (every body its own
     Jacobi-PCG + Chebyshev, one workgroup per body, one launch per size class,
     workgroup sizes from the autotuner at the first update), against one
@@ -262,7 +262,7 @@ def batched_measure(steps: int, warmup: int, sample: int, fp64_check: bool = Tru
     print(f"# batched workload: {scene.nv} vertices, {scene.nt} tets, {len(bodies)} bodies", file=sys.stderr,
           flush=True)
     p = SolverParams(numIterations=10, dt=1.0 / 60.0, gravity=9.8, tol=1e-6, rho=0.9992, pcg_tol=1e-6,
-                     pcg_floor=1e-8, pcg_max_iter=200, pcg_chebyshev=True, pcg_adaptive=True)
+                     pcg_floor=1e-8, pcg_max_iter=200, pcg_chebyshev=True, pcg_adaptive=True, polar_rotation=polar)
     out = {}
     runs = [(32, p)]
     if fp64_check:
@@ -320,6 +320,7 @@ def batched_measure(steps: int, warmup: int, sample: int, fp64_check: bool = Tru
     info = out[32]["info"]
     st = info["body_stats"]
     res = {"steps_per_s": round(sps_b, 3), "ms_per_step": round(1e3 * out[32]["seconds"] / steps, 3),
+           "local_step_rotation": "Newton polar iteration, TR1690 SVD fallback" if polar else "TR1690 SVD",
            "bodies": len(bodies), "verts": scene.nv, "tets": scene.nt,
            "classes": [{k: c[k] for k in ("min_tets", "max_tets", "bodies", "threads")} |
                        {"tuner_ms": [(t, round(ms, 3)) for t, ms in c["tried"]]} for c in info["classes"]],
@@ -346,7 +347,7 @@ def batched_measure(steps: int, warmup: int, sample: int, fp64_check: bool = Tru
 
 
 def run_batched_workload(args):
-    r = batched_measure(args.steps, args.warmup, args.per_body_sample)
+    r = batched_measure(args.steps, args.warmup, args.per_body_sample, polar=args.rotation == "polar")
     line = {"metric": "PD steps/sec, 1024 uneven bodies (3..20 hexes per axis), batched per-body solves vs "
                       "per-body solvers", "value": r["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
@@ -679,7 +680,8 @@ def main():
         configs = {}
         for name, fn in (("config0_8cube_direct", lambda: config0_measure(100)),
                          ("config2_256_dense_bodies", lambda: dense_measure(8, 2)),
-                         ("config3_1024_batched_bodies", lambda: batched_measure(10, 2, args.per_body_sample)),
+                         ("config3_1024_batched_bodies", lambda: batched_measure(10, 2, args.per_body_sample,
+                                                                                 polar=args.rotation == "polar")),
                          ("config4_ipc_two_16cubes", lambda: ipc_measure(args.ipc_steps,
                                                                            0 if args.no_cpu else 3,
                                                                            args.cpu_budget))):
