@@ -7,6 +7,8 @@
 #include <cstring>
 #include <tuple>
 
+#include <hipcub/hipcub.hpp>
+
 #include "ipc_distance.hpp"
 #include "ipc_engine.hpp"
 
@@ -679,45 +681,10 @@ __global__ void ipc_fix_kernel(int nv, const unsigned char* fixed, double* g, do
     }
 }
 
-// y = H x, element by element: inertia diagonal (identity on fixed rows),
-// then each tet's and each active query's 12x12 block on free rows/columns.
-// (`stop`: the PCG control word of the solve; a finished solve skips its launches.)
-__global__ void ipc_mv_vertex_kernel(int nv, const double* mass, const unsigned char* fixed, const double* x,
-                                     double* y, const double* hv, const int* stop) {
-    const int v = blockIdx.x * kT + threadIdx.x;
-    if (v >= nv || *stop) return;
-    const double m = fixed[v] ? 1.0 : mass[v];
-    const double xv[3] = {x[3 * (size_t)v], x[3 * (size_t)v + 1], x[3 * (size_t)v + 2]};
-    for (int k = 0; k < 3; ++k) {
-        double yk = m * xv[k];
-        if (hv && !fixed[v]) {  // the implicit barrier's block on free vertices
-            const double* h = hv + 9 * (size_t)v + 3 * k;
-            yk += h[0] * xv[0] + h[1] * xv[1] + h[2] * xv[2];
-        }
-        y[3 * (size_t)v + k] = yk;
-    }
-}
-__device__ __forceinline__ void mv_block(const uint32_t (&vv)[4], const double* H, const unsigned char* fixed,
-                                         const double* x, double* y) {
-    double xl[12];
-    bool fr[4];
-    for (int a = 0; a < 4; ++a) {
-        fr[a] = fixed[vv[a]] != 0;
-        for (int r = 0; r < 3; ++r) xl[3 * a + r] = fr[a] ? 0.0 : x[3 * (size_t)vv[a] + r];
-    }
-    for (int a = 0; a < 4; ++a) {
-        if (fr[a]) continue;
-        for (int r = 0; r < 3; ++r) {
-            const double* row = H + 12 * (3 * a + r);
-            double s = 0.0;
-            for (int j = 0; j < 12; ++j) s += row[j] * xl[j];
-            atomicAdd(&y[3 * (size_t)vv[a] + r], s);
-        }
-    }
-}
-// Jacobi-PCG pieces (n = 3 N_v).
+// Start of a Jacobi-PCG solve (n = 3 N_v): x = 0, r = b, z = D^-1 b and the
+// block partials of r.z and r.r.
 __global__ void ipc_pcg_init_kernel(int n, const double* b, const double* diag, double* x, double* r, double* z,
-                                    double* p, double* parts) {
+                                    double* parts) {
     const int i = blockIdx.x * kT + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (i < n) {
@@ -725,17 +692,10 @@ __global__ void ipc_pcg_init_kernel(int n, const double* b, const double* diag, 
         x[i] = 0.0;
         r[i] = ri;
         z[i] = zi;
-        p[i] = zi;
         v[0] = ri * zi;
         v[1] = ri * ri;
     }
     block_sums<2>(v, parts);
-}
-__global__ void ipc_dot_kernel(int n, const double* a, const double* b, double* parts, const int* stop) {
-    if (*stop) return;
-    const int i = blockIdx.x * kT + threadIdx.x;
-    double v[1] = {i < n ? a[i] * b[i] : 0.0};
-    block_sums<1>(v, parts);
 }
 
 // The tets' 12x12 Hessian blocks summed into 3x3 blocks per (vertex, vertex)
@@ -763,50 +723,259 @@ __global__ void ipc_assemble_rows_kernel(long long nnzb, const int* row_of, cons
     val[9 * k + e] = s;
 }
 
-// q += the assembled elastic rows (threads [0, 3 nv): one per row and
-// component, the row's blocks in column order) and the active contact
-// blocks (the blocks after) times x.
-__global__ void ipc_mv_blocks_kernel(int nv, int nrb, const long long* rowptr, const int* col, const double* val,
-                                     long long nq, const QueryKey* qk, const double* qd, double dhat2,
-                                     const double* hq, const unsigned char* fixed, const double* x, double* y,
-                                     const int* stop) {
-    if (*stop) return;
-    if ((int)blockIdx.x < nrb) {
-        const int g = blockIdx.x * kT + threadIdx.x;
-        if (g >= 3 * nv) return;
-        const int v = g / 3, r = g - 3 * v;
-        if (fixed[v]) return;
-        double s = 0.0;
-        for (long long k = rowptr[v]; k < rowptr[v + 1]; ++k) {
-            const double* b = val + 9 * k + 3 * r;
-            const size_t j = 3 * (size_t)col[k];
-            s += b[0] * x[j] + b[1] * x[j + 1] + b[2] * x[j + 2];
-        }
-        atomicAdd(&y[g], s);
-    } else {
-        const long long i = (long long)(blockIdx.x - nrb) * kT + threadIdx.x;
-        if (i >= nq || qd[i] > dhat2) return;
-        const QueryKey k = qk[i];
-        const uint32_t vv[4] = {k.v0, k.v1, k.v2, k.v3};
-        mv_block(vv, hq + 144 * (size_t)i, fixed, x, y);
+// The active queries' 12x12 barrier Hessians assembled per Newton iteration
+// into 3x3 blocks per (vertex, vertex) pair, like the elastic rows: every
+// (query, a, b) with both vertices free gets the key row * nv + col (inactive
+// ones the sentinel nv^2, which sorts last); a stable sort by key puts each
+// pair's entries together in ascending query order; the heads of the runs
+// number the blocks, each block sums its run's 3x3 pieces in that order, and
+// the rows are the runs of equal row vertex.
+__global__ void ipc_contact_expand_kernel(long long nq, const QueryKey* __restrict__ qk, const double* __restrict__ qd,
+                                          double dhat2, const unsigned char* __restrict__ fixed,
+                                          unsigned long long nvl, unsigned long long* __restrict__ key,
+                                          uint32_t* __restrict__ code) {
+    const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+    if (t >= 16 * nq) return;
+    const long long i = t >> 4;
+    const int a = (int)(t >> 2) & 3, b = (int)t & 3;
+    unsigned long long k = nvl * nvl;
+    if (qd[i] <= dhat2) {
+        const QueryKey q = qk[i];
+        const uint32_t vv[4] = {q.v0, q.v1, q.v2, q.v3};
+        if (!fixed[vv[a]] && !fixed[vv[b]]) k = (unsigned long long)vv[a] * nvl + vv[b];
     }
+    key[t] = k;
+    code[t] = (uint32_t)t;  // 16 i + 4 a + b
+}
+__global__ void ipc_contact_heads_kernel(long long ne, const unsigned long long* __restrict__ key,
+                                         unsigned long long sentinel, int* __restrict__ flag) {
+    const long long e = (long long)blockIdx.x * kT + threadIdx.x;
+    if (e >= ne) return;
+    flag[e] = key[e] != sentinel && (e == 0 || key[e] != key[e - 1]);
+}
+// count = [blocks, entries before the first sentinel]
+__global__ void ipc_contact_blocks_kernel(long long ne, const unsigned long long* __restrict__ key,
+                                          unsigned long long nvl, const int* __restrict__ flag,
+                                          const int* __restrict__ pos, int* __restrict__ head, int* __restrict__ brow,
+                                          int* __restrict__ bcol, int* __restrict__ count) {
+    const long long e = (long long)blockIdx.x * kT + threadIdx.x;
+    if (e >= ne) return;
+    const unsigned long long sentinel = nvl * nvl;
+    const unsigned long long k = key[e];
+    if (flag[e]) {
+        const int b = pos[e];
+        head[b] = (int)e;
+        brow[b] = (int)(k / nvl);
+        bcol[b] = (int)(k % nvl);
+    }
+    if (e == ne - 1) count[0] = pos[e] + flag[e];
+    const bool valid = k != sentinel;
+    if ((e == 0 && !valid) || (valid && (e == ne - 1 || key[e + 1] == sentinel))) count[1] = valid ? (int)e + 1 : 0;
+}
+__global__ void ipc_contact_values_kernel(long long nbmax, const int* __restrict__ count, const int* __restrict__ head,
+                                          const uint32_t* __restrict__ code, const double* __restrict__ hq,
+                                          double* __restrict__ val) {
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= 9 * nbmax) return;
+    const int nb = count[0];
+    const long long b = g / 9;
+    if (b >= nb) return;
+    const int rc = (int)(g - 9 * b), r = rc / 3, c = rc - 3 * r;
+    const int e1 = b + 1 < nb ? head[b + 1] : count[1];
+    double s = 0.0;
+    for (int e = head[b]; e < e1; ++e) {
+        const uint32_t t = code[e];
+        const size_t i = t >> 4;
+        const int a = (t >> 2) & 3, bb = t & 3;
+        s += hq[144 * i + (size_t)(3 * a + r) * 12 + 3 * bb + c];
+    }
+    val[9 * b + rc] = s;
+}
+// crow[v] = the first block whose row is >= v (v = 0..nv)
+__global__ void ipc_contact_rows_kernel(int nv, const int* __restrict__ count, const int* __restrict__ brow,
+                                        int* __restrict__ crow) {
+    const int v = blockIdx.x * kT + threadIdx.x;
+    if (v > nv) return;
+    int lo = 0, hi = count[0];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (brow[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    crow[v] = lo;
 }
 
-// The scalars of a solve stay on the device: out = [rz0, rr, pq, alpha, beta,
-// |b|, rz (even iterations), rz (odd iterations)], ctl = [finished, completed
-// iterations].  Every block of the two kernels below sums the previous
-// launch's block partials itself (the same fixed order in every block, so
-// every block takes the same decision); block 0 records the scalars.
-//
-// Iteration `it`, first half: pq = p.q, alpha = rz / pq, then x += alpha p,
-// r -= alpha q, z = D^-1 r and the partials of r.z, r.r.  H not positive
-// along p finishes the solve with the iterate kept.
-__global__ void ipc_pcg_step_kernel(int n, int nb_dot, const double* dot_parts, int it, const double* p,
-                                    const double* q, const double* diag, double* x, double* r, double* z,
-                                    double* upd_parts, double* out, int* ctl) {
-    if (ctl[0]) return;
+// Jacobi-PCG on H p = g in two launches per iteration (n = 3 N_v).  The
+// scalars stay on the device: out = [rz0, rr, pq, alpha, beta, |b|, rz (even
+// iterations), rz (odd iterations)], ctl = [finished, completed iterations].
+// Every block sums the previous launch's block partials itself in the same
+// fixed order, so every block takes the same decision; block 0 records the
+// scalars.  The search direction is double-buffered: the product launch of
+// iteration `it` forms p_it = z + beta p_{it-1} (p_0 = z) wherever it needs
+// it (its own rows, the columns of its rows' blocks, the vertices of its
+// contact blocks) with one expression, so every copy is the same value.
+// Both launches request their memory before the block-wide steps (the stop
+// word, the partial sums), so those round trips overlap.
+__device__ __forceinline__ double dir_value(bool first, double zj, double pj, double beta) {
+    return first ? zj : fma(beta, pj, zj);
+}
+
+// The stop word of the solve, the same in every thread of the block (a
+// block of a launch after the one that finished the solve returns whole).
+__device__ __forceinline__ bool solve_stopped(const int* ctl) {
+    __shared__ int stop;
+    if (threadIdx.x == 0) stop = ctl[0];
+    __syncthreads();
+    return stop != 0;
+}
+
+constexpr int kRowLanes = 4;  // lanes per (vertex, component) row of the product
+constexpr int kRowPer = 4;    // blocks per lane whose loads are in flight together
+
+// Product launch of iteration it, kRowLanes lanes per row and component:
+//   it > 0: the stop test of iteration it-1 (|r| <= tol |b| or it >= max_iter)
+//   and beta = rz_it / rz_{it-1};
+//   p_it and q = M p + (the implicit barrier's block) p + the assembled
+//   elastic row times p + the assembled contact row times p (identity on
+//   fixed rows), lane l taking the blocks l, l + kRowLanes, ... of each list;
+//   block partials of p.q.
+template <bool kFirst>
+__global__ void __launch_bounds__(kT) ipc_pcg_prod_kernel(
+    int nv, const long long* __restrict__ rowptr, const int* __restrict__ col, const double* __restrict__ val,
+    const int* __restrict__ crow, const int* __restrict__ ccol, const double* __restrict__ cval,
+    const unsigned char* __restrict__ fixed, const double* __restrict__ mass, const double* __restrict__ hv,
+    const double* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q,
+    int nb_upd, const double* __restrict__ upd_parts, int it, int max_iter, double tol, double* __restrict__ parts,
+    double* __restrict__ out, int* __restrict__ ctl) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    const int g = t / kRowLanes, l = t % kRowLanes;  // row g = 3 v + r
+    const bool live = g < 3 * nv;
+    const int v = live ? g / 3 : 0, r = live ? g - 3 * v : 0;
+    bool work = false;
+    long long k0 = 0, k1 = 0;
+    int c0 = 0, c1 = 0;
+    double own_z[3] = {0, 0, 0}, own_p[3] = {0, 0, 0}, m = 1.0, hb[3] = {0, 0, 0};
+    if (live) {
+        work = fixed[v] == 0;
+        k0 = rowptr[v];
+        k1 = rowptr[v + 1];
+        if (crow) {
+            c0 = crow[v];
+            c1 = crow[v + 1];
+        }
+        for (int c = 0; c < 3; ++c) {
+            own_z[c] = z[3 * (size_t)v + c];
+            if (!kFirst) own_p[c] = pold[3 * (size_t)v + c];
+        }
+        if (work) {
+            m = mass[v];
+            if (hv)  // the implicit barrier's block on free vertices
+                for (int c = 0; c < 3; ++c) hb[c] = hv[9 * (size_t)v + 3 * r + c];
+        }
+    }
+    if (solve_stopped(ctl)) return;
+    // the operands of kRowPer blocks of this lane, loaded before the scalars
+    double bm[kRowPer][3], xz[kRowPer][3], xp[kRowPer][3];
+    auto load_blocks = [&](const auto* __restrict__ cl, const double* __restrict__ vl, long long kb, long long ke) {
+#pragma unroll
+        for (int u = 0; u < kRowPer; ++u) {
+            const long long kk = kb + l + (long long)kRowLanes * u;
+            const long long kc = kk < ke ? kk : ke - 1;
+            const size_t j = 3 * (size_t)cl[kc];
+            for (int c = 0; c < 3; ++c) {
+                bm[u][c] = vl[9 * kc + 3 * r + c];
+                xz[u][c] = z[j + c];
+                xp[u][c] = kFirst ? 0.0 : pold[j + c];
+            }
+        }
+    };
+    if (work && k0 < k1) load_blocks(col, val, k0, k1);
+    double beta = 0.0;
+    if (!kFirst) {
+        double tot[2];
+        sum_parts<2>(nb_upd, upd_parts, tot);
+        const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+        if (sqrt(tot[1]) <= tol * out[5] || it >= max_iter) {
+            if (lead) {
+                out[1] = tot[1];
+                ctl[1] = it;
+                ctl[0] = 1;
+            }
+            return;  // the same decision in every block
+        }
+        beta = tot[0] / out[6 + ((it - 1) & 1)];
+        if (lead) {
+            out[1] = tot[1];
+            out[4] = beta;
+            out[6 + (it & 1)] = tot[0];
+            ctl[1] = it;
+        }
+    }
+    auto apply = [&](long long kb, long long ke, double& s) {
+#pragma unroll
+        for (int u = 0; u < kRowPer; ++u) {
+            if (kb + l + (long long)kRowLanes * u >= ke) break;
+            const double p0 = dir_value(kFirst, xz[u][0], xp[u][0], beta);
+            const double p1 = dir_value(kFirst, xz[u][1], xp[u][1], beta);
+            const double p2 = dir_value(kFirst, xz[u][2], xp[u][2], beta);
+            s += bm[u][0] * p0 + bm[u][1] * p1 + bm[u][2] * p2;
+        }
+    };
+    double s = 0.0, sc = 0.0;  // this lane's elastic and contact blocks, each in order
+    if (work) {
+        for (long long kb = k0; kb < k1; kb += kRowLanes * kRowPer) {
+            if (kb != k0) load_blocks(col, val, kb, k1);
+            apply(kb, k1, s);
+        }
+        for (long long kb = c0; kb < c1; kb += kRowLanes * kRowPer) {
+            load_blocks(ccol, cval, kb, c1);
+            apply(kb, c1, sc);
+        }
+    }
+    // the row's lanes: (s0 + s1) + (s2 + s3)
+    for (int o = 1; o < kRowLanes; o <<= 1) {
+        s += __shfl_xor(s, o);
+        sc += __shfl_xor(sc, o);
+    }
+    double pq[1] = {0.0};
+    if (live && l == 0) {
+        double pv[3];
+        for (int c = 0; c < 3; ++c) pv[c] = dir_value(kFirst, own_z[c], own_p[c], beta);
+        double y = m * pv[r];
+        if (work) {
+            y += hb[0] * pv[0] + hb[1] * pv[1] + hb[2] * pv[2];
+            y += s;
+            y += sc;
+        }
+        pnew[g] = pv[r];
+        q[g] = y;
+        pq[0] = pv[r] * y;
+    }
+    block_sums<1>(pq, parts);
+}
+
+// Update launch of iteration it: alpha = rz_it / p.q, x += alpha p,
+// r -= alpha q, z = D^-1 r and the block partials of r.z and r.r.  H not positive along p
+// finishes the solve with the iterate kept.
+__global__ void __launch_bounds__(kT) ipc_pcg_update_kernel(int n, int nb_prod, const double* __restrict__ prod_parts,
+                                                            int it, const double* __restrict__ p,
+                                                            const double* __restrict__ q,
+                                                            const double* __restrict__ diag, double* __restrict__ x,
+                                                            double* __restrict__ r, double* __restrict__ z,
+                                                            double* __restrict__ upd_parts, double* __restrict__ out,
+                                                            int* __restrict__ ctl) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    double pi = 0.0, qi = 0.0, ri = 0.0, di = 1.0, xi = 0.0;
+    if (i < n) {
+        pi = p[i];
+        qi = q[i];
+        ri = r[i];
+        di = diag[i];
+        xi = x[i];
+    }
+    if (solve_stopped(ctl)) return;
     double tot[1];
-    sum_parts<1>(nb_dot, dot_parts, tot);
+    sum_parts<1>(nb_prod, prod_parts, tot);
     const double pq = tot[0];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (!(pq > 0.0)) {
@@ -817,68 +986,22 @@ __global__ void ipc_pcg_step_kernel(int n, int nb_dot, const double* dot_parts, 
         }
         return;
     }
-    const double a = out[6 + (it & 1)] / pq;
+    const double al = out[6 + (it & 1)] / pq;
     if (lead) {
         out[2] = pq;
-        out[3] = a;
+        out[3] = al;
     }
-    const int i = blockIdx.x * kT + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (i < n) {
-        x[i] += a * p[i];
-        const double ri = r[i] - a * q[i], zi = ri / diag[i];
+        x[i] = xi + al * pi;
+        ri = ri - al * qi;
+        const double zi = ri / di;
         r[i] = ri;
         z[i] = zi;
         v[0] = ri * zi;
         v[1] = ri * ri;
     }
     block_sums<2>(v, upd_parts);
-}
-
-// Iteration `it`, second half (one thread per vertex): the iteration is
-// complete; the solve finishes when |r| <= tol |b| or at the iteration
-// limit, else beta = rz' / rz, p = z + beta p, and the next product starts
-// with q = M p (+ the implicit barrier's block on free vertices).
-__global__ void ipc_pcg_dir_kernel(int nv, int nb_upd, const double* upd_parts, int it, int max_iter, double tol,
-                                   const double* z, double* p, const double* mass, const unsigned char* fixed,
-                                   const double* hv, double* q, double* out, int* ctl) {
-    if (ctl[0]) return;
-    double tot[2];
-    sum_parts<2>(nb_upd, upd_parts, tot);
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const int done_it = it + 1;
-    if (sqrt(tot[1]) <= tol * out[5] || done_it >= max_iter) {
-        if (lead) {
-            out[1] = tot[1];
-            ctl[1] = done_it;
-            ctl[0] = 1;
-        }
-        return;
-    }
-    const double beta = tot[0] / out[6 + (it & 1)];
-    if (lead) {
-        out[1] = tot[1];
-        out[4] = beta;
-        out[6 + (done_it & 1)] = tot[0];
-        ctl[1] = done_it;
-    }
-    const int v = blockIdx.x * kT + threadIdx.x;
-    if (v >= nv) return;
-    double pv[3];
-    for (int k = 0; k < 3; ++k) {
-        const size_t j = 3 * (size_t)v + k;
-        pv[k] = z[j] + beta * p[j];
-        p[j] = pv[k];
-    }
-    const double m = fixed[v] ? 1.0 : mass[v];
-    for (int k = 0; k < 3; ++k) {
-        double yk = m * pv[k];
-        if (hv && !fixed[v]) {
-            const double* h = hv + 9 * (size_t)v + 3 * k;
-            yk += h[0] * pv[0] + h[1] * pv[1] + h[2] * pv[2];
-        }
-        q[3 * (size_t)v + k] = yk;
-    }
 }
 
 }  // namespace
@@ -938,13 +1061,16 @@ IpcEngine::IpcEngine(const IpcData& data) {
     v0_.alloc(nt_);
     fixed_.alloc(nv_);
     xn_.alloc(n3); xtmp_.alloc(n3); p_dir_.alloc(n3); g_.alloc(n3); diag_.alloc(n3);
-    r_.alloc(n3); z_.alloc(n3); q_.alloc(n3);
+    r_.alloc(n3); z_.alloc(n3); q_.alloc(n3); p2_.alloc(n3);
+    crow_.alloc(nv_ + 1);
+    ccount_.alloc(2);
     ht_.alloc(144 * (size_t)nt_);
     hv_.alloc(9 * (size_t)nv_);
     parts_.alloc(2 * (size_t)nblk(std::max<size_t>(n3, (size_t)nt_)) + 64);
     out_.alloc(8);
     pcg_ctl_.alloc(2);
-    pcg_parts_.alloc(3 * (size_t)nblk(n3));
+    pcg_parts_.alloc(2 * (size_t)nblk(n3));
+    prod_parts_.alloc(nblk((size_t)kRowLanes * n3));
     SBS_HIP(hipHostMalloc((void**)&host_, 8 * sizeof(double), hipHostMallocDefault));
     SBS_HIP(hipHostMalloc((void**)&host_ctl_, 2 * sizeof(int), hipHostMallocDefault));
     hipLaunchKernelGGL(ipc_precompute_kernel, dim3(nblk(nt_)), dim3(kT), 0, stream_, nt_, data.Tet,
@@ -1189,22 +1315,20 @@ void IpcEngine::update_kappa(const double* X) {
     kappa_ = gb > 1e-9 ? std::max(100.0, ge / gb) : 100.0;
 }
 
-// Jacobi-PCG on H p = g, element by element; returns the iterations.  The
-// scalars stay on the device and an iteration is four launches (products,
-// p.q, the update with alpha, the direction with beta and the next product's
-// diagonal part); the host reads the control word once per kPcgChunk
-// iterations, and launches after the solve finished return at once.
+// Jacobi-PCG on H p = g; returns the iterations.  An iteration is two
+// launches (the product with the previous iteration's stop test, then the
+// update); the host reads the control word once per kPcgChunk iterations, and
+// launches after the solve finished return at once.
 int IpcEngine::solve() {
     constexpr int kPcgChunk = 16;
     const int n = 3 * nv_;
-    const unsigned nb = nblk(n), nbv = nblk(nv_), nrb = nblk(n);
-    const unsigned nbq = nq_ > 0 ? nblk((size_t)nq_) : 0;
+    const unsigned nb = nblk(n), nrb = nblk((size_t)kRowLanes * n);
     double* out = out_.get();  // [0] rz0, [1] rr, [2] pq, [3] alpha, [4] beta, [5] |b|, [6..7] rz by parity
     int* ctl = pcg_ctl_.get();
-    double* dot_parts = pcg_parts_.get();
-    double* upd_parts = dot_parts + nb;
+    double* upd_parts = pcg_parts_.get();
+    double* prod_parts = prod_parts_.get();
     hipLaunchKernelGGL(ipc_pcg_init_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)g_.get(),
-                       (const double*)diag_.get(), p_dir_.get(), r_.get(), z_.get(), xtmp_.get(), upd_parts);
+                       (const double*)diag_.get(), p_dir_.get(), r_.get(), z_.get(), upd_parts);
     hipLaunchKernelGGL(sum_parts_kernel<2>, dim3(1), dim3(kT), 0, stream_, (int)nb, (const double*)upd_parts, out);
     SBS_HIP(hipMemcpyAsync(host_, out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
     SBS_HIP(hipStreamSynchronize(stream_));
@@ -1216,35 +1340,85 @@ int IpcEngine::solve() {
     host_ctl_[1] = 0;
     SBS_HIP(hipMemcpyAsync(out + 5, host_ + 5, 2 * sizeof(double), hipMemcpyHostToDevice, stream_));
     SBS_HIP(hipMemcpyAsync(ctl, host_ctl_, 2 * sizeof(int), hipMemcpyHostToDevice, stream_));
-    double* pvec = xtmp_.get();  // the PCG search direction (x_tmp is free during the solve)
+    double* dir[2] = {xtmp_.get(), p2_.get()};  // p by iteration parity (x_tmp is free during the solve)
     const double* hv = hv_live_ ? (const double*)hv_.get() : nullptr;
-    hipLaunchKernelGGL(ipc_mv_vertex_kernel, dim3(nbv), dim3(kT), 0, stream_, nv_, d_.mass,
-                       (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get(), hv, (const int*)ctl);
+    const bool contact = nq_ > 0;
+    auto product = [&](int it) {
+        auto kern = it == 0 ? ipc_pcg_prod_kernel<true> : ipc_pcg_prod_kernel<false>;
+        hipLaunchKernelGGL(kern, dim3(nrb), dim3(kT), 0, stream_, nv_, (const long long*)row_ptr_.get(),
+                           (const int*)row_col_.get(), (const double*)row_val_.get(),
+                           contact ? (const int*)crow_.get() : nullptr, (const int*)cbcol_.get(),
+                           (const double*)cval_.get(), (const unsigned char*)fixed_.get(), d_.mass, hv,
+                           (const double*)z_.get(), (const double*)dir[it & 1], dir[(it + 1) & 1], q_.get(), (int)nb,
+                           (const double*)upd_parts, it, p_.lin_max_iter, p_.lin_tol, prod_parts, out, ctl);
+    };
     for (int launched = 0; launched < p_.lin_max_iter;) {
         const int chunk = std::min(kPcgChunk, p_.lin_max_iter - launched);
         for (int j = 0; j < chunk; ++j) {
             const int it = launched + j;
-            hipLaunchKernelGGL(ipc_mv_blocks_kernel, dim3(nrb + nbq), dim3(kT), 0, stream_, nv_, (int)nrb,
-                               (const long long*)row_ptr_.get(), (const int*)row_col_.get(),
-                               (const double*)row_val_.get(), nq_, nq_ > 0 ? col_->queries() : nullptr,
-                               (const double*)qd_.get(), p_.dhat * p_.dhat, (const double*)hq_.get(),
-                               (const unsigned char*)fixed_.get(), (const double*)pvec, q_.get(), (const int*)ctl);
-            hipLaunchKernelGGL(ipc_dot_kernel, dim3(nb), dim3(kT), 0, stream_, n, (const double*)pvec,
-                               (const double*)q_.get(), dot_parts, (const int*)ctl);
-            hipLaunchKernelGGL(ipc_pcg_step_kernel, dim3(nb), dim3(kT), 0, stream_, n, (int)nb,
-                               (const double*)dot_parts, it, (const double*)pvec, (const double*)q_.get(),
+            product(it);
+            hipLaunchKernelGGL(ipc_pcg_update_kernel, dim3(nb), dim3(kT), 0, stream_, n, (int)nrb,
+                               (const double*)prod_parts, it, (const double*)dir[(it + 1) & 1], (const double*)q_.get(),
                                (const double*)diag_.get(), p_dir_.get(), r_.get(), z_.get(), upd_parts, out, ctl);
-            hipLaunchKernelGGL(ipc_pcg_dir_kernel, dim3(nbv), dim3(kT), 0, stream_, nv_, (int)nb,
-                               (const double*)upd_parts, it, p_.lin_max_iter, p_.lin_tol, (const double*)z_.get(),
-                               pvec, d_.mass, (const unsigned char*)fixed_.get(), hv, q_.get(), out, ctl);
         }
-        SBS_HIP(hipGetLastError());
         launched += chunk;
+        if (launched == p_.lin_max_iter) product(launched);  // only the stop test of the last iteration
+        SBS_HIP(hipGetLastError());
         SBS_HIP(hipMemcpyAsync(host_ctl_, ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, stream_));
         SBS_HIP(hipStreamSynchronize(stream_));
         if (host_ctl_[0]) break;
     }
     return host_ctl_[1];
+}
+
+// The contact rows of this Newton iteration from the active queries' Hessians.
+void IpcEngine::assemble_contact() {
+    const long long ne = 16 * nq_;
+    const unsigned long long nvl = (unsigned long long)nv_;
+    const unsigned long long sentinel = nvl * nvl;
+    int bits = 1;
+    while (bits < 64 && (sentinel >> bits) != 0) ++bits;
+    SBS_REQUIRE(ne < INT32_MAX && sentinel / nvl == nvl, "IPCSolver: too many contact pairs");
+    if ((size_t)ne > ecap_) {
+        ecap_ = (size_t)ne + (size_t)ne / 4 + 64;
+        ckey_.alloc(ecap_);
+        ckey_s_.alloc(ecap_);
+        ccode_.alloc(ecap_);
+        ccode_s_.alloc(ecap_);
+        cflag_.alloc(ecap_);
+        cpos_.alloc(ecap_);
+        chead_.alloc(ecap_);
+        cbrow_.alloc(ecap_);
+        cbcol_.alloc(ecap_);
+        cval_.alloc(9 * ecap_);
+    }
+    size_t b1 = 0, b2 = 0;
+    SBS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, ckey_.get(), ckey_s_.get(), ccode_.get(), ccode_s_.get(),
+                                               (int)ne, 0, bits, stream_));
+    SBS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, cflag_.get(), cpos_.get(), (int)ne, stream_));
+    if (std::max(b1, b2) > ctmp_bytes_) {
+        ctmp_bytes_ = std::max(b1, b2);
+        ctmp_.alloc(ctmp_bytes_);
+    }
+    const unsigned eb = nblk((size_t)ne);
+    hipLaunchKernelGGL(ipc_contact_expand_kernel, dim3(eb), dim3(kT), 0, stream_, nq_, col_->queries(),
+                       (const double*)qd_.get(), p_.dhat * p_.dhat, (const unsigned char*)fixed_.get(), nvl,
+                       ckey_.get(), ccode_.get());
+    SBS_HIP(hipcub::DeviceRadixSort::SortPairs(ctmp_.get(), b1, ckey_.get(), ckey_s_.get(), ccode_.get(),
+                                               ccode_s_.get(), (int)ne, 0, bits, stream_));
+    hipLaunchKernelGGL(ipc_contact_heads_kernel, dim3(eb), dim3(kT), 0, stream_, ne,
+                       (const unsigned long long*)ckey_s_.get(), sentinel, cflag_.get());
+    SBS_HIP(hipcub::DeviceScan::ExclusiveSum(ctmp_.get(), b2, cflag_.get(), cpos_.get(), (int)ne, stream_));
+    int* count = ccount_.get();
+    hipLaunchKernelGGL(ipc_contact_blocks_kernel, dim3(eb), dim3(kT), 0, stream_, ne,
+                       (const unsigned long long*)ckey_s_.get(), nvl, (const int*)cflag_.get(),
+                       (const int*)cpos_.get(), chead_.get(), cbrow_.get(), cbcol_.get(), count);
+    hipLaunchKernelGGL(ipc_contact_values_kernel, dim3(nblk(9 * (size_t)ne)), dim3(kT), 0, stream_, ne,
+                       (const int*)count, (const int*)chead_.get(), (const uint32_t*)ccode_s_.get(),
+                       (const double*)hq_.get(), cval_.get());
+    hipLaunchKernelGGL(ipc_contact_rows_kernel, dim3(nblk((size_t)nv_ + 1)), dim3(kT), 0, stream_, nv_,
+                       (const int*)count, (const int*)cbrow_.get(), crow_.get());
+    SBS_HIP(hipGetLastError());
 }
 
 // SearchDirection (ipc.cu:230-239): gradient and Hessian at X, elimination,
@@ -1255,6 +1429,7 @@ bool IpcEngine::search_direction(const double* X) {
                        (const int*)row_of_.get(), (const int*)row_col_.get(), (const long long*)contrib_ptr_.get(),
                        (const uint32_t*)contrib_.get(), (const double*)ht_.get(), (const unsigned char*)fixed_.get(),
                        row_val_.get());
+    if (nq_ > 0) assemble_contact();
     SBS_HIP(hipGetLastError());
     const double gm = max_abs(g_.get(), 3 * (size_t)nv_);
     if (!std::isfinite(gm)) return false;

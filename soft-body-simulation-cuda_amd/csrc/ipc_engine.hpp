@@ -96,6 +96,7 @@ private:
     bool search_direction(const double* X);
     void build_rows(const std::vector<uint32_t>& tet);
     int solve();
+    void assemble_contact();
     double reduce_sum(const double* parts, int n);
     double max_abs(const double* v, size_t n);
     double body_step(const double* X, const double* p);
@@ -127,14 +128,23 @@ private:
     DevBuf<double> qd_;                     // squared distance per candidate query
     size_t qcap_ = 0;
     long long nq_ = 0;
-    DevBuf<double> r_, z_, q_;              // PCG vectors
+    DevBuf<double> r_, z_, q_, p2_;         // PCG vectors (p double-buffered in x_tmp and p2)
+    // the active queries' Hessians assembled per Newton iteration into 3x3
+    // blocks per (vertex, vertex) pair: rows crow, columns cbcol, values cval
+    size_t ecap_ = 0, ctmp_bytes_ = 0;
+    DevBuf<unsigned long long> ckey_, ckey_s_;
+    DevBuf<uint32_t> ccode_, ccode_s_;
+    DevBuf<int> cflag_, cpos_, chead_, cbrow_, cbcol_, crow_, ccount_;
+    DevBuf<double> cval_;
+    DevBuf<unsigned char> ctmp_;
     DevBuf<double> parts_, out_;            // reduction partials, results
     DevBuf<double> area_;                   // contact area per vertex (the implicit barrier's weight)
     DevBuf<double> hv_;                     // 9 per vertex: the implicit barrier's Hessian blocks
     bool hv_live_ = false;                  // hv_ holds this Newton iteration's blocks
     double* host_ = nullptr;                // pinned scalars
     DevBuf<int> pcg_ctl_;                   // PCG control: [finished, completed iterations]
-    DevBuf<double> pcg_parts_;              // PCG block partials: p.q, then r.z and r.r
+    DevBuf<double> pcg_parts_;              // PCG block partials of r.z and r.r
+    DevBuf<double> prod_parts_;             // PCG block partials of p.q
     int* host_ctl_ = nullptr;               // its pinned copy
 };
 
