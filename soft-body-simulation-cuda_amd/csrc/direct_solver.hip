@@ -113,10 +113,19 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
             const int tmax = min(kBlk, s1 - i0);
-            for (int t = 0; t < tmax; ++t) {
-                const T y = lane_value(sv, t) * tile[t * kTileLd + t];
-                if (lane > t) sv -= tile[lane * kTileLd + t] * y;
-                else if (lane == t) sv = y;
+            // this lane's tile row and the reciprocal diagonal into registers
+            // first: the 64 dependent steps then wait on no LDS read
+            T lrow[kBlk];
+#pragma unroll
+            for (int t = 0; t < kBlk; ++t) lrow[t] = tile[lane * kTileLd + t];
+            const T dinv = tile[lane * kTileLd + lane];
+#pragma unroll
+            for (int t = 0; t < kBlk; ++t) {
+                if (t < tmax) {
+                    const T y = lane_value(sv, t) * lane_value(dinv, t);
+                    if (lane > t) sv -= lrow[t] * y;
+                    else if (lane == t) sv = y;
+                }
             }
             if (r < s1) {
                 ring[r & mask] = sv;
@@ -154,10 +163,17 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) sv -= part[w * kBlk + lane];
             const int tmax = min(kBlk, s1 - i0);
-            for (int t = tmax - 1; t >= 0; --t) {
-                const T xv = lane_value(sv, t) * tile[t * kTileLd + t];
-                if (lane < t) sv -= tile[t * kTileLd + lane] * xv;
-                else if (lane == t) sv = xv;
+            T lcol[kBlk];  // this lane's tile column
+#pragma unroll
+            for (int t = 0; t < kBlk; ++t) lcol[t] = tile[t * kTileLd + lane];
+            const T dinv = tile[lane * kTileLd + lane];
+#pragma unroll
+            for (int t = kBlk - 1; t >= 0; --t) {
+                if (t < tmax) {
+                    const T xv = lane_value(sv, t) * lane_value(dinv, t);
+                    if (lane < t) sv -= lcol[t] * xv;
+                    else if (lane == t) sv = xv;
+                }
             }
             if (r < s1) {
                 ring[r & mask] = sv;
