@@ -54,7 +54,7 @@ __device__ inline double lane_value(double v, int l) {
 // y is also kept in `ybuf` for the backward sweep.  LDS grows with the
 // bandwidth, not with the number of rows.
 template <class T>
-__global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int R, const int* __restrict__ sys,
+__global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int R, int nsys, const int* __restrict__ sys,
                                                               const T* __restrict__ Ld, const int* __restrict__ perm,
                                                               const T* __restrict__ b, T* __restrict__ x,
                                                               T* __restrict__ ybuf, const int* __restrict__ skip) {
@@ -65,7 +65,11 @@ __global__ void __launch_bounds__(kThreads) band_solve_kernel(int n, int bw, int
     T* tile = ring + R;
     T* part = tile + kBlk * kTileLd;
     const int mask = R - 1;
-    const int comp = blockIdx.x % 3, sy = blockIdx.x / 3;
+    // the three components of a system on one XCD (block b runs on XCD
+    // b % 8), so their sweeps read the system's factor through one L2
+    const int xcd = blockIdx.x % 8, jx = blockIdx.x / 8;
+    const int comp = jx % 3, sy = 8 * (jx / 3) + xcd;
+    if (sy >= nsys) return;
     const int s0 = sys[sy], s1 = sys[sy + 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const size_t step = (size_t)kWaves * n;
@@ -249,8 +253,9 @@ void BandCholesky<T>::factor(int n, const std::vector<int>& rowptr, const std::v
 
 template <class T> void BandCholesky<T>::solve(const T* b, T* x, hipStream_t stream, const int* skip) const {
     SBS_REQUIRE(n_ > 0, "BandCholesky: solve before factor");
-    hipLaunchKernelGGL(band_solve_kernel<T>, dim3(3 * nsys_), dim3(kThreads), lds_bytes<T>(ring_), stream, n_, bw_,
-                       ring_, (const int*)sys_.get(), ld_.get(), perm_.get(), b, x, ybuf_.get(), skip);
+    hipLaunchKernelGGL(band_solve_kernel<T>, dim3(24 * ceil_div((size_t)nsys_, 8)), dim3(kThreads),
+                       lds_bytes<T>(ring_), stream, n_, bw_, ring_, nsys_, (const int*)sys_.get(), ld_.get(), perm_.get(),
+                       b, x, ybuf_.get(), skip);
     SBS_HIP(hipGetLastError());
 }
 
