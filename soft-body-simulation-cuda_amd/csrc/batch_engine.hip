@@ -92,12 +92,20 @@ template <int NQ> __device__ __forceinline__ void block_sum(double (&v)[NQ], dou
     }
 }
 
-template <class T>
+// kLdsP: the PCG search direction of the body lives in LDS (3 values per
+// vertex of the class's largest body), so the product's neighbour reads are
+// LDS reads instead of L2 round trips; the values are the same.
+template <class T, bool kLdsP>
 __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int first) {
     __shared__ double red[2][kMaxWaves * 9];
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
     int rb = 0;
     const int b = a.body_list[first + blockIdx.x];
     const int v0 = a.vptr[b], v1 = a.vptr[b + 1], t0 = a.tptr[b], t1 = a.tptr[b + 1];
+    T* const lp = reinterpret_cast<T*>(dyn_lds);
+    const size_t pbase = 3 * (size_t)v0;
+    // entry j (3 per vertex, global numbering) of the search direction
+    auto P = [&](size_t j) -> T& { return kLdsP ? lp[j - pbase] : a.p[j]; };
     const int nrow = v1 - v0, width = a.ew[b];
     const size_t ebase = (size_t)a.eptr[b];
     const int B = blockDim.x, tid = threadIdx.x;
@@ -205,7 +213,7 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
                 const T dbk = di * bv[k];
                 a.x[3 * (size_t)i + k] = xs[k];
                 a.r[3 * (size_t)i + k] = rk;
-                a.p[3 * (size_t)i + k] = zk;
+                P(3 * (size_t)i + k) = zk;
                 acc[k] += (double)(rk * zk);
                 acc[3 + k] += (double)(zk * zk);
                 acc[6 + k] += (double)(dbk * dbk);
@@ -228,7 +236,7 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
             double pq[3] = {0, 0, 0};
             for (int i = v0 + tid; i < v1; i += B) {
                 const T dA = a.diagA[i];
-                const T* pi = a.p + 3 * (size_t)i;
+                const T pi[3] = {P(3 * (size_t)i), P(3 * (size_t)i + 1), P(3 * (size_t)i + 2)};
                 T qv[3] = {dA * pi[0], dA * pi[1], dA * pi[2]};
                 const int il = i - v0;
                 for (int j = 0; j < width; ++j) {
@@ -236,7 +244,7 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
                     const uint32_t u = a.ecol[e];
                     const T val = a.eval[e];
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) qv[k] += val * a.p[3 * (size_t)u + k];
+                    for (int k = 0; k < 3; ++k) qv[k] += val * P(3 * (size_t)u + k);
                 }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -256,7 +264,7 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const size_t j = 3 * (size_t)i + k;
-                    const T pk = a.p[j], qk = a.q[j];
+                    const T pk = P(j), qk = a.q[j];
                     a.x[j] += alpha[k] * pk;
                     const T rk = a.r[j] - alpha[k] * qk;
                     const T zk = di * rk;
@@ -278,7 +286,7 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const size_t j = 3 * (size_t)i + k;
-                    a.p[j] = a.q[j] + beta[k] * a.p[j];
+                    P(j) = a.q[j] + beta[k] * P(j);
                 }
 #pragma unroll
             for (int k = 0; k < 3; ++k) act[k] = act[k] && (T)nz[3 + k] > thr[k];
@@ -479,6 +487,7 @@ template <class T> BatchEngine<T>::BatchEngine(const BatchSystem<T>& s) {
         bc.min_tets = tptr_h_[mem.back() + 1] - tptr_h_[mem.back()];
         bc.max_tets = tptr_h_[mem.front() + 1] - tptr_h_[mem.front()];
         bc.bodies = (int)mem.size();
+        for (int b : mem) bc.max_verts = std::max(bc.max_verts, vptr_h_[b + 1] - vptr_h_[b]);
         bc.threads = round64(bc.max_tets / 16);  // until the tuner chooses
         cls_first_.push_back((int)list.size());
         list.insert(list.end(), mem.begin(), mem.end());
@@ -495,6 +504,15 @@ template <class T> BatchEngine<T>::BatchEngine(const BatchSystem<T>& s) {
         SBS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         events_.push_back(e);
     }
+    // LDS a workgroup may take for the search direction: the device's per-block
+    // limit less the kernel's static 2.3 KB (64 KB unless it reports more)
+    int dev = 0, lim = 0;
+    SBS_HIP(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0)
+        lim = 64 * 1024;
+    lds_p_limit_ = (size_t)lim - 4096;
+    SBS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_step_kernel<T, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p_limit_));
 }
 
 template <class T> BatchEngine<T>::~BatchEngine() {
@@ -576,7 +594,13 @@ void BatchEngine<T>::launch_class(int c, T* X, T* V, T* XT, const T* DBCX, const
     a.wdbc = (T)1e6 * (dt_inv * dt_inv);
     a.tol2 = (T)(p.pcg_tol * p.pcg_tol);
     a.floor2 = (T)(p.pcg_floor * p.pcg_floor);
-    hipLaunchKernelGGL(batch_step_kernel<T>, dim3(cls_[c].bodies), dim3(threads), 0, s, a, cls_first_[c]);
+    const size_t lds = 3 * (size_t)cls_[c].max_verts * sizeof(T);
+    if (lds <= lds_p_limit_)
+        hipLaunchKernelGGL((batch_step_kernel<T, true>), dim3(cls_[c].bodies), dim3(threads), lds, s, a,
+                           cls_first_[c]);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<T, false>), dim3(cls_[c].bodies), dim3(threads), 0, s, a,
+                           cls_first_[c]);
     SBS_HIP(hipGetLastError());
 }
 

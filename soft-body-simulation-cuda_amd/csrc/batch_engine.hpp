@@ -37,6 +37,7 @@ struct BatchBody {
 
 struct BatchClass {
     int min_tets = 0, max_tets = 0, bodies = 0, threads = 0;  // threads per workgroup chosen by the tuner
+    int max_verts = 0;                                        // vertices of the class's largest body
     std::vector<std::pair<int, double>> tried;               // (threads, ms of one step) the tuner measured
 };
 
@@ -81,6 +82,7 @@ private:
 
     int nv_ = 0, nt_ = 0, nb_ = 0;
     std::vector<int> vptr_h_, tptr_h_;
+    size_t lds_p_limit_ = 0;  // largest search direction (bytes) a workgroup keeps in LDS
     DevBuf<int> vptr_, tptr_, eptr_, ew_, body_list_;
     DevBuf<uint32_t> perm_, ecol_, inc_;
     DevBuf<int> inc_ptr_;
