@@ -1468,11 +1468,12 @@ bool IpcEngine::step(IpcData& data, const IpcParams& p) {
         double alpha = 0.95 * std::min(af, 0.9 * toi);
         alpha = alpha < 1e-12 ? 0.0 : std::min(1.0, alpha);
         if (stats_.newton_iterations == 1) stats_.alpha0 = alpha;
+        double E = E_last;
         while (true) {
             hipLaunchKernelGGL(ipc_axpy_kernel, dim3(vb), dim3(kT), 0, stream_, n3, (const double*)X,
                                (const double*)p_dir_.get(), alpha, xt);
             update_queries(xt);
-            const double E = energy(xt);
+            E = energy(xt);
             if (!std::isfinite(E)) return false;
             if (E > E_last) {
                 alpha /= 2;
@@ -1482,10 +1483,10 @@ bool IpcEngine::step(IpcData& data, const IpcParams& p) {
             }
             if (alpha < DBL_EPSILON) break;
         }
+        // X takes the last trial point: its queries and energy are the ones
+        // just computed there (the reference recomputes both at X, ipc.cu:209-214)
         SBS_HIP(hipMemcpyAsync(X, xt, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        update_queries(X);
-        E_last = energy(X);
-        if (!std::isfinite(E_last)) return false;
+        E_last = E;
         if (!search_direction(X)) return false;
     }
     stats_.energy = E_last;
