@@ -1317,8 +1317,9 @@ void IpcEngine::update_kappa(const double* X) {
 
 // Jacobi-PCG on H p = g; returns the iterations.  An iteration is two
 // launches (the product with the previous iteration's stop test, then the
-// update); the host reads the control word once per kPcgChunk iterations, and
-// launches after the solve finished return at once.
+// update); the host reads the control word at points predicted from the
+// previous solve (see the loop), and launches after the solve finished
+// return at once.
 int IpcEngine::solve() {
     constexpr int kPcgChunk = 16;
     const int n = 3 * nv_;
@@ -1353,7 +1354,11 @@ int IpcEngine::solve() {
                            (const double*)upd_parts, it, p_.lin_max_iter, p_.lin_tol, prod_parts, out, ctl);
     };
     for (int launched = 0; launched < p_.lin_max_iter;) {
-        const int chunk = std::min(kPcgChunk, p_.lin_max_iter - launched);
+        // the first check after 9/10 of the previous solve's iterations, then
+        // every kPcgChunk or launched/8 (a check costs the GPU ~30 us idle, a
+        // launch after the solve finished ~2.5 us)
+        const int want = launched == 0 ? std::max(kPcgChunk, pcg_last_ * 9 / 10) : std::max(kPcgChunk, launched / 8);
+        const int chunk = std::min(want, p_.lin_max_iter - launched);
         for (int j = 0; j < chunk; ++j) {
             const int it = launched + j;
             product(it);
@@ -1368,6 +1373,7 @@ int IpcEngine::solve() {
         SBS_HIP(hipStreamSynchronize(stream_));
         if (host_ctl_[0]) break;
     }
+    pcg_last_ = host_ctl_[1];
     return host_ctl_[1];
 }
 

@@ -146,6 +146,7 @@ private:
     DevBuf<double> pcg_parts_;              // PCG block partials of r.z and r.r
     DevBuf<double> prod_parts_;             // PCG block partials of p.q
     int* host_ctl_ = nullptr;               // its pinned copy
+    int pcg_last_ = 0;                      // iterations of the previous solve (when to check first)
 };
 
 }  // namespace sbs
