@@ -409,7 +409,7 @@ def ipc_measure(nsteps: int = 100, cpu_steps: int = 3, cpu_budget: float = 20.0)
     return {"steps": nsteps, "steps_per_s": round((len(times) - 1) / el, 3) if nsteps > 1 else None,
             "ms_per_step": round(1e3 * el / max(len(times) - 1, 1), 3), "first_step_ms_incl_prepare":
             round(1e3 * times[0], 1), "newton_iterations_per_step": round(newton / nsteps, 2),
-            "pcg_iterations_per_newton_solve": round(lin / max(newton, 1), 1), "max_queries": queries,
+            "pcg_iterations_per_solve": round(lin / max(newton + nsteps, 1), 1), "max_queries": queries,
             "max_kappa": kappa, "first_step_with_barrier": first_contact, "gap_after_m": gap,
             "verts": scene.nv, "tets": scene.nt, "tris": scene.ntri, "elastic": "neohookean", "cpu_baseline": cpu}
 

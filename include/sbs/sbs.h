@@ -408,7 +408,8 @@ typedef struct SbsIpcParams {
 typedef struct SbsIpcStats {
     int newton_iterations;    /* of the last step                              */
     int line_search_halvings; /* of the last step                              */
-    int linear_iterations;    /* PCG iterations of the last solve              */
+    int linear_iterations;    /* PCG iterations of the last step's solves
+                                 (newton_iterations + 1 solves)                */
     int failed;               /* latched: a step failed (reset clears it)      */
     long long queries;        /* contact queries kept at the last update       */
     double kappa;             /* barrier stiffness of the last step            */

@@ -1439,7 +1439,7 @@ bool IpcEngine::search_direction(const double* X) {
     SBS_HIP(hipGetLastError());
     const double gm = max_abs(g_.get(), 3 * (size_t)nv_);
     if (!std::isfinite(gm)) return false;
-    stats_.linear_iterations = solve();
+    stats_.linear_iterations += solve();
     return true;
 }
 
@@ -1451,6 +1451,7 @@ bool IpcEngine::step(IpcData& data, const IpcParams& p) {
     const unsigned vb = nblk(n3);
     stats_.newton_iterations = 0;
     stats_.line_search_halvings = 0;
+    stats_.linear_iterations = 0;
     hipLaunchKernelGGL(ipc_begin_kernel, dim3(vb), dim3(kT), 0, stream_, n3, (const double*)data.X,
                        (const double*)data.V, h, p.damp, data.XTilde, xn_.get());
     SBS_HIP(hipGetLastError());
