@@ -32,36 +32,45 @@ __device__ __forceinline__ D3 ldp(const double* X, uint32_t v) {
     return {X[3 * (size_t)v], X[3 * (size_t)v + 1], X[3 * (size_t)v + 2]};
 }
 
-// Block sums of NQ per-thread values (fixed tree order), written to
-// parts[NQ * block + k].  Every thread of the block calls it.
+// Block sums of NQ per-thread values, written to parts[NQ * block + k]:
+// a butterfly sum in each wavefront (lane 0's total is used), then the
+// wavefront totals in index order.  Every thread of the block calls it.
 template <int NQ> __device__ void block_sums(const double (&v)[NQ], double* parts) {
-    __shared__ double red[NQ][kT];
-    for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] = v[k];
-    __syncthreads();
-    for (int s = kT / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        __syncthreads();
+    __shared__ double red[NQ][kT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k < NQ; ++k) {
+        const double t = wave_sum(v[k]);
+        if (lane == 0) red[k][w] = t;
     }
-    if ((int)threadIdx.x < NQ) parts[NQ * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    __syncthreads();
+    if ((int)threadIdx.x < NQ) {
+        double t = 0.0;
+        for (int j = 0; j < kT / 64; ++j) t += red[threadIdx.x][j];
+        parts[NQ * (size_t)blockIdx.x + threadIdx.x] = t;
+    }
 }
 
-// Sums over blocks of parts[NQ * b + k] in a fixed order (one block of kT
-// threads, every thread calls it); thread 0 gets the totals.
+// Sums over blocks of parts[NQ * b + k] (one block of kT threads, every
+// thread calls it): thread t adds blocks t, t + kT, ..., then the wavefront
+// and block sums as above; every thread gets the totals, the same in every
+// block that calls it.
 template <int NQ> __device__ void sum_parts(int nb, const double* parts, double (&tot)[NQ]) {
-    __shared__ double red[NQ][kT];
+    __shared__ double red[NQ][kT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double acc[NQ];
     for (int k = 0; k < NQ; ++k) acc[k] = 0.0;
     for (int b = threadIdx.x; b < nb; b += kT)
         for (int k = 0; k < NQ; ++k) acc[k] += parts[NQ * (size_t)b + k];
-    for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int s = kT / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        __syncthreads();
+    for (int k = 0; k < NQ; ++k) {
+        const double t = wave_sum(acc[k]);
+        if (lane == 0) red[k][w] = t;
     }
-    for (int k = 0; k < NQ; ++k) tot[k] = red[k][0];
+    __syncthreads();
+    for (int k = 0; k < NQ; ++k) {
+        double t = 0.0;
+        for (int j = 0; j < kT / 64; ++j) t += red[k][j];
+        tot[k] = t;
+    }
 }
 
 // out[k] = sum over blocks of parts[NQ * b + k], in block order (one block).
