@@ -821,8 +821,8 @@ __global__ void ipc_contact_rows_kernel(int nv, const int* __restrict__ count, c
 // fixed order, so every block takes the same decision; block 0 records the
 // scalars.  The search direction is double-buffered: the product launch of
 // iteration `it` forms p_it = z + beta p_{it-1} (p_0 = z) wherever it needs
-// it (its own rows, the columns of its rows' blocks, the vertices of its
-// contact blocks) with one expression, so every copy is the same value.
+// it (its own rows and the columns of its rows' elastic and contact blocks)
+// with one expression, so every copy is the same value.
 // Both launches request their memory before the block-wide steps (the stop
 // word, the partial sums), so those round trips overlap.
 __device__ __forceinline__ double dir_value(bool first, double zj, double pj, double beta) {
