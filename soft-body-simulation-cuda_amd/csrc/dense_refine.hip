@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(kFusedThreads)
                             const int* __restrict__ body_n, const int* __restrict__ body_p,
                             const long long* __restrict__ tile_off, const void* __restrict__ tiles, int nseg,
                             double tol, int max_iter, int* __restrict__ body_iters, double* __restrict__ body_ratio,
-                            const int* skip) {
+                            const int* skip, int warm) {
     if (skip && *skip) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int bd = blockIdx.x;
@@ -278,10 +278,41 @@ __global__ void __launch_bounds__(kFusedThreads)
     void* rhs = red + 3 * kFusedWaves;
     float* part = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(rhs) + 16 * (size_t)P * (MODE == 0 ? 4 : 2));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < 3 * P; i += kFusedThreads) x64[i] = 0.0;
+    // x starts at 0, or with `warm` at the caller's x (the previous iterate)
+    for (int i = tid; i < 3 * P; i += kFusedThreads)
+        x64[i] = (warm && i < 3 * nb) ? (double)x[3 * (size_t)r0 + i] : 0.0;
     const int words = MODE == 0 ? 16 * P : 8 * P;  // B fragments as 32-bit words; padding stays 0
     for (int i = tid; i < words; i += kFusedThreads) reinterpret_cast<uint32_t*>(rhs)[i] = 0u;
     __syncthreads();
+    // r = b - A x (FP64) packed as the next correction's B operand; s = max |r / d|
+    auto residual = [&](double (&s)[3]) {
+        s[0] = s[1] = s[2] = 0;
+        for (int kl = tid; kl < nb; kl += kFusedThreads) {
+            const int v = r0 + kl;
+            const double d = (double)A.diag[v];
+            double ax[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ax[c] = d * x64[3 * kl + c];
+            const int sl = v / kSliceD, ln = v % kSliceD;
+            const size_t base = (size_t)A.slice_ptr[sl] + ln;
+            const int width = A.slice_w[sl];
+            for (int e = 0; e < width; ++e) {
+                const size_t pos = base + (size_t)e * kSliceD;
+                const int u = (int)A.ecol[pos] - r0;  // the body's own rows (padding: the row itself, value 0)
+                const double m = (double)A.eval[pos];
+                if ((unsigned)u < (unsigned)nb) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ax[c] += m * x64[3 * u + c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double r = (double)b[3 * (size_t)v + c] - ax[c];
+                put_rhs<MODE>(rhs, rhs_pos<MODE>(0, kl, c), r);
+                s[c] = fmax(s[c], fabs(r / d));
+            }
+        }
+    };
     double s[3] = {0, 0, 0};
     for (int kl = tid; kl < nb; kl += kFusedThreads) {
         const int v = r0 + kl;
@@ -289,13 +320,18 @@ __global__ void __launch_bounds__(kFusedThreads)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double r = (double)b[3 * (size_t)v + c];
-            put_rhs<MODE>(rhs, rhs_pos<MODE>(0, kl, c), r);
+            if (!warm) put_rhs<MODE>(rhs, rhs_pos<MODE>(0, kl, c), r);
             s[c] = fmax(s[c], fabs(r / d));
         }
     }
     double bn[3], rn[3];
-    block_max3(s, red, bn);
-    rn[0] = bn[0]; rn[1] = bn[1]; rn[2] = bn[2];
+    block_max3(s, red, bn);  // the stop test is relative to b, from either start
+    if (warm) {
+        residual(s);
+        block_max3(s, red, rn);
+    } else {
+        rn[0] = bn[0]; rn[1] = bn[1]; rn[2] = bn[2];
+    }
     const int nrb = P / 16, nk = MODE == 0 ? P / 4 : P / 32, seg = (nk + nseg - 1) / nseg;
     int k = 0;
     bool ok = false;
@@ -366,33 +402,7 @@ __global__ void __launch_bounds__(kFusedThreads)
             }
             __syncthreads();
         }
-        // r = b - A x (FP64), packed for the next correction
-        s[0] = s[1] = s[2] = 0;
-        for (int kl = tid; kl < nb; kl += kFusedThreads) {
-            const int v = r0 + kl;
-            const double d = (double)A.diag[v];
-            double ax[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) ax[c] = d * x64[3 * kl + c];
-            const int sl = v / kSliceD, ln = v % kSliceD;
-            const size_t base = (size_t)A.slice_ptr[sl] + ln;
-            const int width = A.slice_w[sl];
-            for (int e = 0; e < width; ++e) {
-                const size_t pos = base + (size_t)e * kSliceD;
-                const int u = (int)A.ecol[pos] - r0;  // the body's own rows (padding: the row itself, value 0)
-                const double m = (double)A.eval[pos];
-                if ((unsigned)u < (unsigned)nb) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) ax[c] += m * x64[3 * u + c];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double r = (double)b[3 * (size_t)v + c] - ax[c];
-                put_rhs<MODE>(rhs, rhs_pos<MODE>(0, kl, c), r);
-                s[c] = fmax(s[c], fabs(r / d));
-            }
-        }
+        residual(s);
         block_max3(s, red, rn);
     }
     for (int i = tid; i < 3 * nb; i += kFusedThreads) x[3 * (size_t)r0 + i] = (T)x64[i];
@@ -554,7 +564,8 @@ void DenseRefinedSolver<T>::setup(const EllView<T>& A, const std::vector<int>& r
     if (!host_report_) SBS_HIP(hipHostMalloc((void**)&host_report_, 2 * sizeof(double), hipHostMallocDefault));
 }
 
-template <class T> void DenseRefinedSolver<T>::solve(const T* b, T* x, double tol, int max_iter, const int* skip) {
+template <class T>
+void DenseRefinedSolver<T>::solve(const T* b, T* x, double tol, int max_iter, const int* skip, bool warm) {
     SBS_REQUIRE(nb_ > 0, "dense solver: not set up");
     SBS_REQUIRE(max_iter >= 1 && max_iter <= 64, "dense solver: max_iter must be in [1, 64]");
     const int need = 4 * (max_iter + 3);
@@ -569,7 +580,7 @@ template <class T> void DenseRefinedSolver<T>::solve(const T* b, T* x, double to
         hipLaunchKernelGGL(kern, dim3(nb_), dim3(kFusedThreads), fused_lds_, stream_, A_, b, x,
                            (const int*)body_row0_.get(), (const int*)body_n_.get(), (const int*)body_p_.get(),
                            (const long long*)tile_off_.get(), (const void*)tiles_.get(), fused_nseg_, tol, max_iter,
-                           body_iters_.get(), body_ratio_.get(), skip);
+                           body_iters_.get(), body_ratio_.get(), skip, (int)warm);
         hipLaunchKernelGGL(dense_body_report_kernel, dim3(1), dim3(256), 0, stream_, nb_, max_iter,
                            (const int*)body_iters_.get(), (const double*)body_ratio_.get(), report_.get(), skip);
         SBS_HIP(hipGetLastError());

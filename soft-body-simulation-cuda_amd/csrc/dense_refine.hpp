@@ -43,7 +43,9 @@ public:
                const std::vector<T>& val, DensePrecision prec, hipStream_t stream);
     // x = A^-1 b for 3 interleaved right-hand sides (device arrays, 3n each).
     // Skipped entirely when *skip != 0 (device flag; may be null).
-    void solve(const T* b, T* x, double tol, int max_iter, const int* skip);
+    // x = A^-1 b to tol (max-norm residual relative to b); with `warm` the
+    // one-workgroup-per-body form starts from the caller's x instead of 0
+    void solve(const T* b, T* x, double tol, int max_iter, const int* skip, bool warm = false);
     // Corrections applied by the last solve (-1 - n if it stopped at max_iter)
     // and the scaled residual ratio reached; waits for the stream.
     void last_result(int* iterations, double* ratio);

@@ -1846,11 +1846,11 @@ template <class T> void PdEngine<T>::build_batch(const PdParams& params) {
     }
 }
 
-template <class T> void PdEngine<T>::direct_solve(const T* b, T* x, const PdParams& params) {
+template <class T> void PdEngine<T>::direct_solve(const T* b, T* x, const PdParams& params, bool warm) {
     const int* skip = &reinterpret_cast<CtlBlock*>(ctl_.get())->outer_done;
     if (dense_) {
         dense_->set_stream(stream_);
-        dense_->solve(b, x, params.dense_tol, params.dense_max_iter, skip);
+        dense_->solve(b, x, params.dense_tol, params.dense_max_iter, skip, warm);
         return;
     }
     SBS_REQUIRE(direct_ != nullptr, "direct solve requested before factorisation");
@@ -2281,7 +2281,7 @@ template <class T> void PdEngine<T>::enqueue_step(PdData<T>& data, const PdParam
                 hipLaunchKernelGGL(rhs_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, g, rhs_.get(), skip);
                 tick(kKRhs, false);
                 tick(kKDirect, true);
-                direct_solve(rhs_.get(), sn_.get(), params);
+                direct_solve(rhs_.get(), sn_.get(), params, true);  // sn holds the previous iterate
                 tick(kKDirect, false);
                 hipLaunchKernelGGL(outer_error_kernel<T>, dim3(vb), dim3(kTpb), 0, stream_, nv_, (const T*)sn_.get(),
                                    (T*)nullptr, prev_.get(), slots_.get(), skip);

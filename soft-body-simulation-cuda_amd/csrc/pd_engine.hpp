@@ -206,7 +206,7 @@ private:
     static int cap_margin(int used, int moved, int steady);
     bool is_direct() const;
     void direct_factor(const PdParams& params);
-    void direct_solve(const T* b, T* x, const PdParams& params);
+    void direct_solve(const T* b, T* x, const PdParams& params, bool warm = false);
     void tick(int id, bool start, int launches = 1);  // a stop covers `launches` launches
     void flush_timing();
 
