@@ -92,6 +92,22 @@ template <int NQ> __device__ __forceinline__ void block_sum(double (&v)[NQ], dou
     }
 }
 
+// Entries j0 .. j0 + NB - 1 of row il of a body's ELL matrix
+// (positions past the row's width repeat its last entry; the caller skips
+// them), requested together so their latencies overlap.
+template <class T> constexpr int kRowBatch = sizeof(T) == 4 ? 8 : 4;  // FP64: within the 128 registers
+template <class T, int NB = kRowBatch<T>>
+__device__ __forceinline__ void row_entries(const BatchArgs<T>& a, size_t ebase, int nrow, int il, int width, int j0,
+                                            uint32_t (&uu)[NB], T (&vv)[NB]) {
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+        const int j = j0 + jj < width ? j0 + jj : width - 1;
+        const size_t e = ebase + (size_t)j * nrow + il;
+        uu[jj] = a.ecol[e];
+        vv[jj] = a.eval[e];
+    }
+}
+
 // kLdsP: the PCG search direction of the body lives in LDS (3 values per
 // vertex of the class's largest body), so the product's neighbour reads are
 // LDS reads instead of L2 round trips; the values are the same.
@@ -199,12 +215,20 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
 #pragma unroll
             for (int k = 0; k < 3; ++k) Ax[k] = dA * xs[k];
             const int il = i - v0;
-            for (int j = 0; j < width; ++j) {
-                const size_t e = ebase + (size_t)j * nrow + il;
-                const uint32_t u = a.ecol[e];
-                const T val = a.eval[e];
+            for (int j0 = 0; j0 < width; j0 += kRowBatch<T>) {
+                uint32_t uu[kRowBatch<T>];
+                T vv[kRowBatch<T>];
+                row_entries(a, ebase, nrow, il, width, j0, uu, vv);
+                T xu[kRowBatch<T>][3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) Ax[k] += val * a.sn[3 * (size_t)u + k];
+                for (int jj = 0; jj < kRowBatch<T>; ++jj)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) xu[jj][k] = a.sn[3 * (size_t)uu[jj] + k];
+#pragma unroll
+                for (int jj = 0; jj < kRowBatch<T>; ++jj)
+                    if (j0 + jj < width)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) Ax[k] += vv[jj] * xu[jj][k];
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -239,12 +263,15 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
                 const T pi[3] = {P(3 * (size_t)i), P(3 * (size_t)i + 1), P(3 * (size_t)i + 2)};
                 T qv[3] = {dA * pi[0], dA * pi[1], dA * pi[2]};
                 const int il = i - v0;
-                for (int j = 0; j < width; ++j) {
-                    const size_t e = ebase + (size_t)j * nrow + il;
-                    const uint32_t u = a.ecol[e];
-                    const T val = a.eval[e];
+                for (int j0 = 0; j0 < width; j0 += kRowBatch<T>) {
+                    uint32_t uu[kRowBatch<T>];
+                    T vv[kRowBatch<T>];
+                    row_entries(a, ebase, nrow, il, width, j0, uu, vv);
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) qv[k] += val * P(3 * (size_t)u + k);
+                    for (int jj = 0; jj < kRowBatch<T>; ++jj)
+                        if (j0 + jj < width)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) qv[k] += vv[jj] * P(3 * (size_t)uu[jj] + k);
                 }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
