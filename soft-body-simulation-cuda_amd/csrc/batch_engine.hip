@@ -194,11 +194,22 @@ __global__ void __launch_bounds__(1024) batch_step_kernel(BatchArgs<T> a, int fi
             const T cv = a.c0[i];
 #pragma unroll
             for (int k = 0; k < 3; ++k) bv[k] = cv * a.sn_old[3 * (size_t)i + k];
-            for (int e = a.inc_ptr[i]; e < a.inc_ptr[i + 1]; ++e) {
-                const uint32_t code = a.inc[e];
-                const T* f = a.force + 12 * (size_t)(code >> 2) + 3 * (code & 3u);
+            // the vertex's corner forces in incidence order, kRowBatch requested together
+            const int e0 = a.inc_ptr[i], e1 = a.inc_ptr[i + 1];
+            for (int eb = e0; eb < e1; eb += kRowBatch<T>) {
+                T fv[kRowBatch<T>][3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) bv[k] += f[k];
+                for (int jj = 0; jj < kRowBatch<T>; ++jj) {
+                    const uint32_t code = a.inc[eb + jj < e1 ? eb + jj : e1 - 1];
+                    const T* f = a.force + 12 * (size_t)(code >> 2) + 3 * (code & 3u);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) fv[jj][k] = f[k];
+                }
+#pragma unroll
+                for (int jj = 0; jj < kRowBatch<T>; ++jj)
+                    if (eb + jj < e1)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) bv[k] += fv[jj][k];
             }
             if (a.dbc[i] > (T)0) {
                 // DBCX wdbc - c o (translation of b = DBCX wdbc by A o = c o):
