@@ -296,18 +296,29 @@ __global__ void __launch_bounds__(kFusedThreads)
             const int sl = v / kSliceD, ln = v % kSliceD;
             const size_t base = (size_t)A.slice_ptr[sl] + ln;
             const int width = A.slice_w[sl];
-            for (int e = 0; e < width; ++e) {
-                const size_t pos = base + (size_t)e * kSliceD;
-                const int u = (int)A.ecol[pos] - r0;  // the body's own rows (padding: the row itself, value 0)
-                const double m = (double)A.eval[pos];
-                if ((unsigned)u < (unsigned)nb) {
+            // the row's entries 8 at a time, requested together, summed in order
+            for (int e0 = 0; e0 < width; e0 += 8) {
+                int uu[8];
+                T mm[8];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) ax[c] += m * x64[3 * u + c];
+                for (int j = 0; j < 8; ++j) {
+                    const size_t pos = base + (size_t)(e0 + j < width ? e0 + j : width - 1) * kSliceD;
+                    uu[j] = (int)A.ecol[pos] - r0;  // the body's own rows (padding: the row itself, value 0)
+                    mm[j] = A.eval[pos];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (e0 + j < width && (unsigned)uu[j] < (unsigned)nb) {
+                        const double m = (double)mm[j];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) ax[c] += m * x64[3 * uu[j] + c];
+                    }
                 }
             }
+            const T bv[3] = {b[3 * (size_t)v], b[3 * (size_t)v + 1], b[3 * (size_t)v + 2]};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double r = (double)b[3 * (size_t)v + c] - ax[c];
+                const double r = (double)bv[c] - ax[c];
                 put_rhs<MODE>(rhs, rhs_pos<MODE>(0, kl, c), r);
                 s[c] = fmax(s[c], fabs(r / d));
             }
