@@ -696,12 +696,19 @@ struct PcgStep {
 };
 
 // Row r < 3 of the wavefront gets the total of quantity K0 + r of a slot (a
-// lane adds two stripes, then the 16-lane row sum); row 3 gets 0.
-template <int K0> __device__ __forceinline__ double slot_rows3(const double* slot) {
+// lane adds two stripes, then the 16-lane row sum); row 3 gets 0.  The lane
+// value is loaded by every lane (row 3 at component 2's address and then
+// zeroed) with no branch around it, so the loads of several quantities are
+// in flight together instead of each waiting in its own masked block.
+template <int K0> __device__ __forceinline__ double slot_pair3(const double* slot) {
     static_assert(kStripes == 32, "two stripes per lane, 16 lanes per quantity");
     const int lane = threadIdx.x & 63, r = lane >> 4, s = lane & 15;
-    const double v = r < 3 ? slot[s * kSlotQ + K0 + r] + slot[(s + 16) * kSlotQ + K0 + r] : 0.0;
-    return row16_total(v);
+    const int k = K0 + (r < 3 ? r : 2);
+    const double v = slot[s * kSlotQ + k] + slot[(s + 16) * kSlotQ + k];
+    return r < 3 ? v : 0.0;
+}
+template <int K0> __device__ __forceinline__ double slot_rows3(const double* slot) {
+    return row16_total(slot_pair3<K0>(slot));
 }
 
 // Stop test and scalars of iteration i, identical in every wavefront; block 0
@@ -718,43 +725,49 @@ template <class T> __device__ __forceinline__ PcgStep pcg_state(const PcgArgs<T>
     const bool writer = blockIdx.x == 0 && threadIdx.x < 64;
     bool act;
     double gamma, thr, alpha = 0.0, beta = 0.0;
+    const PcgState* prev = i > 0 ? &a.ctl->st[(i - 1) & 1] : nullptr;
+    bool carried = false;  // the solve stopped earlier: the state is carried forward
+    int c_iters = 0, c_unconverged = 0;
     if (i == 0) {
         const double* sl = iter_slot(a.slots, -1);
-        const double g = slot_rows3<0>(sl), dl = slot_rows3<3>(sl), uu = slot_rows3<6>(sl);
-        const double dd = slot_rows3<0>(a.slots + kSlotInit * kSlotSize);
+        const double vg = slot_pair3<0>(sl), vd = slot_pair3<3>(sl), vu = slot_pair3<6>(sl);
+        const double vb = slot_pair3<0>(a.slots + kSlotInit * kSlotSize);
+        const double g = row16_total(vg), dl = row16_total(vd), uu = row16_total(vu), dd = row16_total(vb);
         thr = fmax(a.tol2 * uu, a.floor2 * dd);
         act = uu > thr;
         gamma = g;
         alpha = (act && dl != 0.0) ? g / dl : 0.0;
     } else {
-        const PcgState* prev = &a.ctl->st[(i - 1) & 1];
-        if (prev->done) {  // stopped earlier: carry the state forward
-            for (int k = 0; k < 3; ++k) {
-                out.alpha[k] = prev->alpha[k];
-                out.beta[k] = prev->beta[k];
-            }
-            out.iters = prev->iters;
-            out.done = 1;
-            out.unconverged = prev->unconverged;
-            if (writer && threadIdx.x == 0) *rec = *prev;
-            return out;
-        }
+        // The stop flag, the stripes of launch i-1 and this component's state
+        // are requested together and consumed without a branch (the scalars are
+        // formed in every case and selected), so they cost one memory round
+        // trip instead of one each.
         const double* sl = iter_slot(a.slots, i - 1);
-        const double g = slot_rows3<0>(sl), dl = slot_rows3<3>(sl), uu = slot_rows3<6>(sl);
-        thr = prev->thr[c];
-        act = prev->active[c] && uu > thr;
-        const double pg = prev->gamma[c], pa = prev->alpha[c];
-        if (act && pg != 0.0) {
-            beta = g / pg;
-            const double den = dl - beta * g / pa;
-            alpha = den != 0.0 ? g / den : 0.0;
+        const double vg = slot_pair3<0>(sl), vd = slot_pair3<3>(sl), vu = slot_pair3<6>(sl);
+        const double pthr = prev->thr[c], pg = prev->gamma[c], pa = prev->alpha[c], pb = prev->beta[c];
+        const int pact = prev->active[c];
+        carried = prev->done != 0;
+        c_iters = prev->iters;
+        c_unconverged = prev->unconverged;
+        const double g = row16_total(vg), dl = row16_total(vd), uu = row16_total(vu);
+        thr = pthr;
+        act = !carried && pact && uu > thr;
+        const double bt = g / pg;
+        const double den = dl - bt * g / pa;
+        const double al = g / den;
+        const bool ok = act && pg != 0.0;
+        beta = ok ? bt : 0.0;
+        alpha = ok && den != 0.0 ? al : 0.0;
+        if (carried) {
+            alpha = pa;
+            beta = pb;
         }
         gamma = g;
     }
     const bool any = __ballot(act && row < 3) != 0ull;
-    out.iters = i;
-    out.done = !any || i >= a.max_iter;
-    out.unconverged = any && i >= a.max_iter;
+    out.iters = carried ? c_iters : i;
+    out.done = carried || !any || i >= a.max_iter;
+    out.unconverged = carried ? c_unconverged : (any && i >= a.max_iter);
     out.alpha[0] = readlane_f64<0>(alpha);
     out.alpha[1] = readlane_f64<16>(alpha);
     out.alpha[2] = readlane_f64<32>(alpha);
@@ -762,18 +775,22 @@ template <class T> __device__ __forceinline__ PcgStep pcg_state(const PcgArgs<T>
     out.beta[1] = readlane_f64<16>(beta);
     out.beta[2] = readlane_f64<32>(beta);
     if (writer) {
-        if ((lane & 15) == 0 && row < 3) {
-            rec->gamma[c] = gamma;
-            rec->alpha[c] = alpha;
-            rec->beta[c] = beta;
-            rec->thr[c] = thr;
-            rec->active[c] = act;
-        }
-        if (threadIdx.x == 0) {
-            rec->iters = out.iters;
-            rec->done = out.done;
-            rec->unconverged = out.unconverged;
-            rec->pad = 0;
+        if (carried) {
+            if (threadIdx.x == 0) *rec = *prev;
+        } else {
+            if ((lane & 15) == 0 && row < 3) {
+                rec->gamma[c] = gamma;
+                rec->alpha[c] = alpha;
+                rec->beta[c] = beta;
+                rec->thr[c] = thr;
+                rec->active[c] = act;
+            }
+            if (threadIdx.x == 0) {
+                rec->iters = out.iters;
+                rec->done = out.done;
+                rec->unconverged = out.unconverged;
+                rec->pad = 0;
+            }
         }
     }
     return out;
@@ -1104,6 +1121,18 @@ template <class T> __device__ __forceinline__ HaloGroup halo_group(const HaloVie
     const int* gd = h.grp + kGrpInts * (size_t)g;
     return HaloGroup{gd[0], gd[1], gd[2], gd[3], gd[4], gd[5]};
 }
+// The descriptor as one vector load (lane k of each wavefront reads entry
+// k mod kGrpInts) and its broadcast: a kernel that issues the load early and
+// broadcasts later keeps the load in flight with its other first loads (the
+// broadcasts are wavefront operations and stay where they are written).
+template <class T> __device__ __forceinline__ int halo_group_lane(const HaloView<T>& h, unsigned g) {
+    return h.grp[kGrpInts * (size_t)g + (threadIdx.x & (kGrpInts - 1))];
+}
+__device__ __forceinline__ HaloGroup halo_group_bcast(int v) {
+    return HaloGroup{__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 1),
+                     __builtin_amdgcn_readlane(v, 2), __builtin_amdgcn_readlane(v, 3),
+                     __builtin_amdgcn_readlane(v, 4), __builtin_amdgcn_readlane(v, 5)};
+}
 
 // The halo kernels keep the PCG vectors that neighbours read scaled by D^-1,
 // in three planes per buffer (xyz interleaved, internal row order):
@@ -1271,10 +1300,8 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     if (a.skip && *a.skip) return;
     __shared__ T lu[3][kHaloStride];
     const unsigned g = xcd_block(blockIdx.x, gridDim.x);
-    const HaloGroup G = halo_group(a.h, g);
+    const int gdl = halo_group_lane(a.h, g);
     const int t = threadIdx.x;
-    const bool row = t < G.rows;
-    const size_t ir = (size_t)G.row0 + (row ? t : 0);
     const int rd = i & 1, wr = rd ^ 1;
     const Planes<T> Pi = planes(a.hrec[rd], a.nv), Po = planes(a.hrec[wr], a.nv);
     const uint32_t* __restrict__ hh = a.h.halo + (size_t)g * kHaloStride;
@@ -1282,7 +1309,10 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) hidx[k] = hh[t + k * kTpb];  // in bounds: kHaloStride per workgroup
     const PcgStep st = pcg_state(a, i);
+    const HaloGroup G = halo_group_bcast(gdl);
     if (st.done) return;
+    const bool row = t < G.rows;
+    const size_t ir = (size_t)G.row0 + (row ? t : 0);
     T hq[kHaloPer][9];  // R', W', S' of the staged halo rows
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) {
