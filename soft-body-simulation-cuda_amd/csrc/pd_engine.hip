@@ -1220,14 +1220,13 @@ __global__ void __launch_bounds__(kTpb) pcg_init1_halo_kernel(PcgArgs<T> a) {
         RowBatch<T> bt;
         row_batch(a.h, G.first, G.width, t, 0, bt);
         const uint32_t* __restrict__ hv = a.h.halo_v + (size_t)g * kHaloStride;
+        // every slot (the padded list holds valid vertices), so the loads go out together
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) {
             const int e = t + k * kTpb;
-            if (e < G.hn) {
-                T q0, q1, q2;
-                ld3(a.x0, hv[e], q0, q1, q2);
-                lx[0][e] = q0; lx[1][e] = q1; lx[2][e] = q2;
-            }
+            T q0, q1, q2;
+            ld3(a.x0, hv[e], q0, q1, q2);
+            lx[0][e] = q0; lx[1][e] = q1; lx[2][e] = q2;
         }
         ld3(a.x0, v, x0, x1, x2);
         __syncthreads();
@@ -1259,14 +1258,13 @@ __global__ void __launch_bounds__(kTpb) pcg_init2_halo_kernel(PcgArgs<T> a) {
     RowBatch<T> b0;
     row_batch(a.h, G.first, G.width, t, 0, b0);
     const uint32_t* __restrict__ hh = a.h.halo + (size_t)g * kHaloStride;
+    // every slot (the padded list holds valid rows), so the loads go out together
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) {
         const int e = t + k * kTpb;
-        if (e < G.hn) {
-            T q0, q1, q2;
-            ld3(P.r, hh[e], q0, q1, q2);
-            lu[0][e] = q0; lu[1][e] = q1; lu[2][e] = q2;
-        }
+        T q0, q1, q2;
+        ld3(P.r, hh[e], q0, q1, q2);
+        lu[0][e] = q0; lu[1][e] = q1; lu[2][e] = q2;
     }
     T u0, u1, u2;
     ld3(P.r, ir, u0, u1, u2);
@@ -1313,15 +1311,16 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
     if (st.done) return;
     const bool row = t < G.rows;
     const size_t ir = (size_t)G.row0 + (row ? t : 0);
-    T hq[kHaloPer][9];  // R', W', S' of the staged halo rows
+    // R', W', S' of the staged halo rows.  Every slot is loaded (the list is
+    // padded with the workgroup's first row, so a slot past the halo reads a
+    // valid row and stages a value no column refers to): loads under a
+    // per-slot condition would each wait for the ones before them.
+    T hq[kHaloPer][9];
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) {
-        const int e = t + k * kTpb;
-        if (e < G.hn) {
-            ld3(Pi.r, hidx[k], hq[k][0], hq[k][1], hq[k][2]);
-            ld3(Pi.w, hidx[k], hq[k][3], hq[k][4], hq[k][5]);
-            if (!kFirst) ld3(Pi.s, hidx[k], hq[k][6], hq[k][7], hq[k][8]);
-        }
+        ld3(Pi.r, hidx[k], hq[k][0], hq[k][1], hq[k][2]);
+        ld3(Pi.w, hidx[k], hq[k][3], hq[k][4], hq[k][5]);
+        if (!kFirst) ld3(Pi.s, hidx[k], hq[k][6], hq[k][7], hq[k][8]);
     }
     T r0, r1, r2, w0, w1, w2, so0 = 0, so1 = 0, so2 = 0;
     ld3(Pi.r, ir, r0, r1, r2);
@@ -1338,15 +1337,13 @@ pcg_iter_halo_kernel(PcgArgs<T> a, int i) {
 #pragma unroll
     for (int k = 0; k < kHaloPer; ++k) {
         const int e = t + k * kTpb;
-        if (e < G.hn) {
-            T sq0 = hq[k][3], sq1 = hq[k][4], sq2 = hq[k][5];
-            if (!kFirst) {
-                sq0 = fma(be0, hq[k][6], sq0); sq1 = fma(be1, hq[k][7], sq1); sq2 = fma(be2, hq[k][8], sq2);
-            }
-            lu[0][e] = fma(-al0, sq0, hq[k][0]);
-            lu[1][e] = fma(-al1, sq1, hq[k][1]);
-            lu[2][e] = fma(-al2, sq2, hq[k][2]);
+        T sq0 = hq[k][3], sq1 = hq[k][4], sq2 = hq[k][5];
+        if (!kFirst) {
+            sq0 = fma(be0, hq[k][6], sq0); sq1 = fma(be1, hq[k][7], sq1); sq2 = fma(be2, hq[k][8], sq2);
         }
+        lu[0][e] = fma(-al0, sq0, hq[k][0]);
+        lu[1][e] = fma(-al1, sq1, hq[k][1]);
+        lu[2][e] = fma(-al2, sq2, hq[k][2]);
     }
     // the second batch is requested once the staged halo values are out of
     // registers, so its latency overlaps the barrier and the first batch
