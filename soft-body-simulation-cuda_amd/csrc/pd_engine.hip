@@ -1048,16 +1048,20 @@ __global__ void __launch_bounds__(kChunkTets) local_chunk_kernel(
         }
     }
     __syncthreads();
-    for (int wi = threadIdx.x; wi < 3 * nu; wi += kChunkTets) {
-        const int i = wi / 3, r = wi - 3 * i;
+    // one thread per chunk vertex sums all three components, so each
+    // incidence code is read and decoded once
+    for (int i = threadIdx.x; i < nu; i += kChunkTets) {
         const int k1 = ends[i];
-        T acc = 0;
+        T a0 = 0, a1 = 0, a2 = 0;
         SBS_UNROLL(SBS_GATHER_UNROLL)
         for (int k = i ? ends[i - 1] : 0; k < k1; ++k) {
             const unsigned code = codes[k];
-            acc += fs[((code & 3u) * 3 + r) * kChunkTets + (code >> 2)];
+            const T* f = fs + (code & 3u) * 3 * kChunkTets + (code >> 2);
+            a0 += f[0];
+            a1 += f[kChunkTets];
+            a2 += f[2 * kChunkTets];
         }
-        partial[3 * (size_t)entry_pos[e0 + i] + r] = acc;  // vertex-major: the consumers read contiguously
+        st3(partial, entry_pos[e0 + i], a0, a1, a2);  // vertex-major: the consumers read contiguously
     }
 }
 
