@@ -938,9 +938,10 @@ __global__ void __launch_bounds__(kTpb) pcg_finish_kernel(PcgArgs<T> a, int L, i
 //      touch it, in tet order);
 //   2. one thread per tet: F = Ds Dm^-1, 3x3 SVD in registers, R (or R - F),
 //      the 4 corner forces of w R Dm^-T G^T written to LDS (plain stores);
-//   3. one thread per (vertex, component) sums its incidences in list order
-//      and writes the chunk's partial force for that vertex at the vertex's
-//      slot of the vertex-major partial array.
+//   3. one thread per chunk vertex sums its incidences in list order (the
+//      three components together, each code decoded once) and writes the
+//      chunk's partial force for that vertex at the vertex's slot of the
+//      vertex-major partial array.
 // No atomics anywhere, so the result is deterministic; float atomics on LDS
 // serialise on the shared corners of neighbouring tets and were the
 // bottleneck of the scatter form.
