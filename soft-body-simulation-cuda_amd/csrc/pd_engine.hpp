@@ -115,7 +115,7 @@ struct KernelTime {
 };
 
 #ifndef SBS_CHUNK_TETS
-#define SBS_CHUNK_TETS 512
+#define SBS_CHUNK_TETS 256
 #endif
 constexpr int kChunkTets = SBS_CHUNK_TETS;  // tets per local-step chunk (one workgroup)
 constexpr int kMaxSolveLog = 64;  // per-step log of PCG iterations, by solve index
